@@ -1,0 +1,191 @@
+/*
+ * leo2d.h -- C-ABI of libleo2d, the MI355X-native drop-in for LEO2D's KPM
+ * hot path (Hamiltonian assembly -> Chebyshev recursion -> Jackson DoS).
+ *
+ * The reference runs this path inside Locality::workerChebSolve
+ * (src/locality.cpp:2843-3575): per MPI job it updates positions
+ * (setConfigPositions, :3577), fills a CSR SpMatrix (generateRealH :3900 /
+ * generateCpxH :4485), runs computeDosKPM / computeDosTraceKPM (:5945 /
+ * :6138) on SpMatrix::vectorMultiply (src/matrix/spmatrix.cpp:270,365) and
+ * transforms the moments with Param_tools::densityTransform{,Trace}
+ * (src/params/param_tools.cpp:661,707).  The pair lists it assembles from are
+ * built once by Locality::constructGeom (:785) through Hstruct::getInterPairs
+ * (src/geom/hstruct.cpp:298) and Sheet::getIntraPairs (src/geom/sheet.cpp:624).
+ * This header replaces that body; each entry point cites what it replaces.
+ *
+ * Conventions: plain pointers and sizes, no C++ types.  Every call returns
+ * LEO2D_OK (0) or a negative LEO2D_ERR_* code; no exception crosses the ABI;
+ * leo2d_last_error() gives the message.  A handle owns one HIP stream and its
+ * device memory; host buffers belong to the caller.  A handle is not
+ * thread-safe: use one handle per host thread.
+ */
+#ifndef LEO2D_H
+#define LEO2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LEO2D_OK 0
+#define LEO2D_ERR_ARG -1      /* invalid argument or size */
+#define LEO2D_ERR_DECK -2     /* deck parse / semantic error */
+#define LEO2D_ERR_HIP -3      /* HIP runtime error */
+#define LEO2D_ERR_ALLOC -4    /* device or host allocation failed */
+#define LEO2D_ERR_STATE -5    /* call out of order (e.g. KPM before a matrix) */
+#define LEO2D_ERR_SPECTRUM -6 /* |mu_n| > mu_0 (1 + 1e-6): spectrum not inside [-1, 1] */
+#define LEO2D_ERR_NAN -7      /* non-finite matrix value or moment */
+
+#define LEO2D_MAX_SHEETS 8
+#define LEO2D_MAX_TARGETS 64
+
+typedef struct leo2d_handle leo2d_handle;
+typedef struct leo2d_deck leo2d_deck;
+typedef struct leo2d_geom leo2d_geom;
+
+/* ---- deck: app/src/main.cpp:75-937 (same keys, same token rules) -------- */
+
+/* Parse an hstruct deck given as text.  err (may be NULL) receives the message
+ * of a LEO2D_ERR_DECK failure (e.g. POLY_ORDER % 4 != 0, main.cpp:914-917;
+ * unknown MATERIAL, materials.cpp:28). */
+int leo2d_deck_parse(const char* text, leo2d_deck** out, char* err, int errlen);
+void leo2d_deck_free(leo2d_deck* d);
+/* Typed access to the parsed Job_params bag (job_params.cpp:18-156 defaults). */
+int leo2d_deck_get_int(const leo2d_deck* d, const char* key, int64_t* value);
+int leo2d_deck_get_double(const leo2d_deck* d, const char* key, double* value);
+int leo2d_deck_get_string(const leo2d_deck* d, const char* key, char* buf, int buflen);
+
+/* ---- geometry: Locality::setupSupercell (locality.cpp:131-400), Sheet
+ * enumeration (sheet.cpp:217-285), Hstruct global index and positions
+ * (hstruct.cpp:69-218) -- the arrays constructGeom broadcasts
+ * (locality.cpp:1059-1173). ---------------------------------------------- */
+typedef struct {
+    int32_t n;                                /* total orbitals */
+    int32_t num_sheets;
+    int32_t sheet_count[LEO2D_MAX_SHEETS];    /* orbitals per sheet */
+    int32_t boundary_condition;
+    double supercell[2][2];                   /* global supercell (opts "supercell") */
+    double angle[LEO2D_MAX_SHEETS];           /* radians, after supercell setup */
+} leo2d_geom_info;
+
+int leo2d_geom_build(const leo2d_deck* d, leo2d_geom** out);
+void leo2d_geom_free(leo2d_geom* g);
+int leo2d_geom_get_info(const leo2d_geom* g, leo2d_geom_info* info);
+/* index_to_grid int32[4n] (i, j, l, s), index_to_pos double[3n] */
+int leo2d_geom_arrays(const leo2d_geom* g, int32_t* index_to_grid, double* index_to_pos);
+
+/* ---- jobs: rootChebSolve job list (locality.cpp:1222-1388 SQ/LC shifts,
+ * 1615-1652 K_TYPE 0 grid) and Hstruct::getTargetList (hstruct.cpp:1049). -- */
+typedef struct {
+    int32_t job_id;                           /* 1-based, the .cheb JOBID */
+    int32_t max_jobs;
+    double shift[LEO2D_MAX_SHEETS][3];        /* per-sheet shifts, lattice units */
+    int32_t k_sampling;
+    double k[2];                              /* Bloch vector (1/A) */
+    double k_grid[2];
+    double energy_rescale, energy_shift;      /* H~ = (H + shift I) / rescale */
+    int32_t elec_on, mag_on;
+    double E, B;
+    int32_t complex_matrix;                   /* locality.cpp:2906-2908 */
+    int32_t uniform_strain;
+    double u[2][2];
+    int32_t num_targets;
+    int32_t targets[LEO2D_MAX_TARGETS];
+} leo2d_job;
+
+int leo2d_jobs_count(const leo2d_deck* d, const leo2d_geom* g, int32_t* count);
+int leo2d_job_get(const leo2d_deck* d, const leo2d_geom* g, int32_t index, leo2d_job* job);
+
+/* ---- device handle ------------------------------------------------------ */
+int leo2d_create(int32_t device, leo2d_handle** out);
+void leo2d_destroy(leo2d_handle* h);
+const char* leo2d_last_error(const leo2d_handle* h);
+
+/* Upload the geometry (grids, index and positions) to the device. */
+int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* g);
+
+/* GPU pair search replacing Hstruct::getInterPairs (hstruct.cpp:298-632) and
+ * Sheet::getIntraPairs (sheet.cpp:624-765): same windows, same keep rules,
+ * same per-row order (stable column sort for periodic decks), so the CSR
+ * pattern is bit-identical to the reference's.  Counts returned (may be NULL). */
+int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64_t* n_slots);
+/* Copy a pair list back: which 0 = intra, 1 = inter; ij int32[2P], sc int32[2P],
+ * t double[P] (intra couplings, may be NULL). */
+int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, double* t);
+
+#define LEO2D_PATTERN_REFERENCE 0   /* every reference slot, explicit zeros kept */
+#define LEO2D_PATTERN_PRUNED 1      /* slots outside the coupling support dropped */
+
+/* Per-job Hamiltonian fill on the device, replacing generateRealH /
+ * generateCpxH (locality.cpp:3900-4321, 4485-5158) including
+ * setConfigPositions (locality.cpp:3577-3660, 3849-3857).  nnz (may be NULL)
+ * receives the stored entries. */
+int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz);
+
+/* Install a host CSR as the device Hamiltonian (H~, already rescaled):
+ * row_ptr int64[n+1], col int32[nnz], val double[nnz] or double[2 nnz]. */
+int leo2d_csr_upload(leo2d_handle* h, int32_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                     const double* val, int32_t is_complex);
+/* Shape of the device Hamiltonian. */
+int leo2d_csr_info(leo2d_handle* h, int32_t* n, int64_t* nnz, int32_t* is_complex);
+/* Copy the device Hamiltonian to host buffers sized from leo2d_csr_info. */
+int leo2d_csr_copy(leo2d_handle* h, int64_t* row_ptr, int32_t* col, double* val);
+/* Gershgorin interval [lo, hi] of the device Hamiltonian (FP64). */
+int leo2d_spectral_bounds(leo2d_handle* h, double* lo, double* hi);
+
+/* ---- KPM: computeDosKPM / computeDosTraceKPM (locality.cpp:5945-6346) ---- */
+#define LEO2D_START_TARGETS 0       /* e_t per vector (local DoS) */
+#define LEO2D_START_VECTORS 1       /* caller vectors, [R][n] real or complex */
+#define LEO2D_START_RANDOM_PHASE 2  /* exp(i phi), phi = hash(seed, r, i) in [0, 2 pi) */
+
+#define LEO2D_PREC_FP64 0           /* FP64 recursion */
+#define LEO2D_PREC_MIXED 1          /* FP32 recursion and matrix, FP64 moment sums */
+
+#define LEO2D_SCHEME_REFERENCE 0    /* direct mu_j for j < M/2, doubling for j >= M/4 */
+#define LEO2D_SCHEME_DOUBLING 1     /* doubling identity from j = 1 (no start vector kept) */
+
+typedef struct {
+    int32_t M;              /* moments, multiple of 4 (POLY_ORDER) */
+    int32_t R;              /* start vectors */
+    int32_t start;          /* LEO2D_START_* */
+    const int32_t* targets; /* [R] for LEO2D_START_TARGETS */
+    const double* vectors;  /* [R][n] (x2 interleaved when vectors_complex) */
+    int32_t vectors_complex;
+    uint64_t seed;          /* LEO2D_START_RANDOM_PHASE */
+    int32_t precision;      /* LEO2D_PREC_* */
+    int32_t scheme;         /* LEO2D_SCHEME_* */
+    int32_t block;          /* vectors per device block (power of two <= 128): 0 = auto (<= 64) */
+    int32_t grid;           /* workgroups per step launch (multiple of 8): 0 = auto */
+    int32_t time_steps;     /* 1: time every fused step launch with HIP events */
+} leo2d_kpm_params;
+
+typedef struct {
+    double recursion_ms;    /* HIP events around all steps and reductions */
+    double step_ms_avg;     /* mean duration of one fused step launch */
+    int64_t steps;          /* fused step launches */
+    int64_t nnz;            /* stored entries of H */
+    int32_t n, R, M;
+    int32_t grid;
+} leo2d_timing;
+
+/* mu: host double[R][M]. */
+int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu);
+int leo2d_last_timing(const leo2d_handle* h, leo2d_timing* t);
+
+/* ---- DoS: Param_tools::densityTransform (param_tools.cpp:661-705) --------
+ * rho[t][k] = scale * DCT-III(g . mu[t])_k / (pi sqrt(a^2 - (E_k - b)^2)),
+ * E_k = b + a cos((k + 1/2) pi / M).  mu, E (may be NULL), rho: host. */
+int leo2d_dos(leo2d_handle* h, const double* mu, int32_t T, int32_t M, double a, double b, double scale, double* E,
+              double* rho);
+
+/* ---- whole deck: rootChebSolve + workerChebSolve on one GPU --------------
+ * Writes <out_prefix>.cheb (Param_tools::save, param_tools.cpp:19-285) and,
+ * when MATRIX_SAVE > 0, <out_prefix>_matrix.dat (locality.cpp:4366-4383,
+ * 5213-5228).  out_prefix NULL = the deck's JOB_NAME. */
+int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefix, int32_t pattern);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LEO2D_H */

@@ -1,0 +1,502 @@
+// assemble.hip -- GPU pair search and per-job Hamiltonian fill.
+//
+// Pair search: one thread per orbital row restates, candidate for candidate,
+//   * Sheet::getIntraPairs (sheet.cpp:624-765): 8x8 unit-cell window around
+//     the orbital's own cell in every periodic image; a candidate is kept when
+//     its tabulated hopping is non-zero or it lies in the home image;
+//   * Hstruct::getInterPairs (hstruct.cpp:298-632): 12x12 window (clamped to
+//     the grid) on the sheets below and above; kept when the in-plane distance
+//     is below 12.5 A;
+// and, for periodic decks, the stable column sort of each block
+// (Sheet::orderPairs sheet.cpp:810-849, Hstruct::orderPairs hstruct.cpp:634-666).
+// Geometry arithmetic is compiled without FMA contraction and the rotation
+// cosines come from the host's libm, so every keep/reject decision and window
+// origin matches the reference bit for bit.  Two passes (count, then write)
+// with a device scan in between; the sort is an in-thread insertion sort
+// (rows are concatenations of per-image sorted runs, so it is near-linear).
+//
+// Fill: one thread per row walks its pairs in the reference order and
+// accumulates duplicate (row, col) pairs into one slot exactly as
+// generateRealH / generateCpxH do (locality.cpp:3967-4316, 4664-5153):
+// intralayer values from the hopping table, interlayer values from the
+// coupling model at the job's positions, diagonal shift, rescale, Peierls and
+// Bloch phases.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "assemble.h"
+
+namespace leo2d {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kPI_6 = 0.523598775598298873077;
+constexpr double kA = 2.4768;
+
+inline dim3 rows_grid(int n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
+
+// ---- couplings ----------------------------------------------------------
+__device__ inline double intralayer(int mat, int orb_row, int orb_col, int v0, int v1)
+{
+    // Coupling::Intralayer::graphene (graphene.cpp:13-89): honeycomb shell index
+    int h0 = 2 * v0 + v1;
+    const int h1 = -1 * v0 + v1;
+    if (orb_col == 0 && orb_row == 1) h0 -= 1;
+    else if (orb_col == 1 && orb_row == 0) h0 += 1;
+    const int r = h0 * (h0 + h1) + h1 * h1;
+    if (mat == MAT_GRAPHENE_SIMPLE) return r == 1 ? -2.7 : 0.0;
+    switch (r) {
+    case 0: return 0.3504;
+    case 1: return -2.8922;
+    case 3: return 0.2425;
+    case 4: return -0.2656;
+    case 7: return 0.0235;
+    case 9: return 0.0524;
+    case 12: return -0.0209;
+    case 13: return -0.0148;
+    case 16: return -0.0211;
+    default: return 0.0;
+    }
+}
+
+__device__ inline int iabs_trunc(double z)
+{
+    // graphene.cpp:117 applies the C library's int abs() to a double
+    const int t = (int)z;
+    return t < 0 ? -t : t;
+}
+
+__device__ double c_to_c(int orb_row, int orb_col, double v0, double v1, double v2, double theta_row,
+                         double theta_col)
+{
+    // Coupling::Interlayer::C_to_C (graphene.cpp:91-222)
+    if (iabs_trunc(v2) > 5) return 0.0;
+    const double r2 = v0 * v0 + v1 * v1;
+    if (!(r2 < 64.0)) return 0.0;   // sqrt(r2) < 8 exactly when r2 < 64
+    const double r = sqrt(r2);
+    const double ac = atan2(v1, v0);
+    double theta21 = ac - theta_row;
+    if (orb_row == 1) theta21 += kPI_6;
+    else theta21 -= kPI_6;
+    double theta12 = ac - theta_col + kPI;
+    if (orb_col == 1) theta12 += kPI_6;
+    else theta12 -= kPI_6;
+    const double rs = r / kA;
+    const double z_eq = 3.35;
+    const double z_eps = (fabs(v2) - z_eq) / z_eq;
+    const double lambda_0 = 0.310 - 1.882 * z_eps + 7.741 * z_eps * z_eps;
+    const double xi_0 = 1.750 - 1.618 * z_eps + 1.848 * z_eps * z_eps;
+    const double kappa_0 = 1.990 + 1.007 * z_eps + 2.427 * z_eps * z_eps;
+    const double lambda_3 = -0.068 + 0.399 * z_eps - 1.739 * z_eps * z_eps;
+    const double xi_3 = 3.286 - 0.914 * z_eps + 12.011 * z_eps * z_eps;
+    const double x_3 = 0.500 + 0.322 * z_eps + 0.908 * z_eps * z_eps;
+    const double lambda_6 = -0.008 + 0.046 * z_eps - 0.183 * z_eps * z_eps;
+    const double xi_6 = 2.272 - 0.721 * z_eps - 4.414 * z_eps * z_eps;
+    const double x_6 = 1.217 + 0.027 * z_eps - 0.658 * z_eps * z_eps;
+    const double kappa_6 = 1.562 - 0.371 * z_eps - 0.134 * z_eps * z_eps;
+    const double V0 = lambda_0 * exp(-xi_0 * (rs) * (rs)) * cos(kappa_0 * rs);
+    const double V3 = lambda_3 * rs * rs * exp(-xi_3 * (rs - x_3) * (rs - x_3));
+    const double V6 = lambda_6 * exp(-xi_6 * (rs - x_6) * (rs - x_6)) * sin(kappa_6 * rs);
+    double t = V0 + V3 * (cos(3 * theta12) + cos(3 * theta21)) + V6 * (cos(6 * theta12) + cos(6 * theta21));
+    const double d = 8.0 - r;
+    if (d < 1. && d > 0.) t *= exp(1. - 1. / (d * d));
+    return t;
+}
+
+__device__ inline double interlayer(int mat, int o1, int o2, double dx, double dy, double dz, double th1, double th2)
+{
+    if (mat == MAT_GRAPHENE_SIMPLE) {
+        // t_perp(r) = 0.48 exp(-(r - 3.35)/0.45) eV for 3D distance r < 6 A
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < 36.0)) return 0.0;
+        const double r = sqrt(r2);
+        return 0.48 * exp(-(r - 3.35) / 0.45);
+    }
+    return c_to_c(o1, o2, dx, dy, dz, th1, th2);
+}
+
+// support of a coupling (pruned pattern): decided by exact comparisons only
+__device__ inline bool inter_support(int mat, double dx, double dy, double dz)
+{
+    if (mat == MAT_GRAPHENE_SIMPLE) return dx * dx + dy * dy + dz * dz < 36.0;
+    if (iabs_trunc(dz) > 5) return false;
+    return dx * dx + dy * dy < 64.0;
+}
+
+// ---- pair search ----------------------------------------------------------
+__device__ inline int grid_at(const SheetDev& s, int norb, int i, int j, int l)
+{
+    if (i >= 0 && i < s.H && j >= 0 && j < s.W && l >= 0 && l < norb) return s.grid[((size_t)i * s.W + j) * norb + l];
+    return -1;
+}
+
+__device__ inline int clamp_cell(double v, int lo, int hi)
+{
+    int f = (int)floor(v);
+    if (f < lo) f = lo;
+    if (f > hi) f = hi;
+    return f - lo;
+}
+
+struct PairSink {
+    int32_t* col;
+    int32_t* sc;   // (dx & 0xffff) | (dy << 16)
+    int64_t pos;
+    int count;
+    __device__ inline void push(int c, int dx, int dy)
+    {
+        if (col) {
+            col[pos] = c;
+            sc[pos] = (int32_t)((uint32_t)(dx & 0xffff) | ((uint32_t)dy << 16));
+            ++pos;
+        }
+        ++count;
+    }
+};
+
+__device__ void search_intra(const GeomDev& g, int kh, PairSink& out)
+{
+    const int sh = g.i2g[4 * (size_t)kh + 3];
+    const SheetDev& sd = g.s[sh];
+    const int ih = g.i2g[4 * (size_t)kh + 0], jh = g.i2g[4 * (size_t)kh + 1], l0 = g.i2g[4 * (size_t)kh + 2];
+    const double kp0 = g.lpos[3 * (size_t)kh], kp1 = g.lpos[3 * (size_t)kh + 1];
+    const int S = sd.bc == 1 ? g.sc_search : 0;
+    for (int dx = -S; dx < S + 1; ++dx)
+        for (int dy = -S; dy < S + 1; ++dy) {
+            double scd0 = 0.0, scd1 = 0.0;
+            if (sd.bc == 1) {
+                scd0 = dx * sd.sc[0][0] + dy * sd.sc[1][0];
+                scd1 = dx * sd.sc[0][1] + dy * sd.sc[1][1];
+            }
+            const double p0 = kp0 + scd0, p1 = kp1 + scd1;
+            const int i0 = clamp_cell(g.ainv[0][0] * p0 + g.ainv[0][1] * p1, sd.min0, sd.max0);
+            const int j0 = clamp_cell(g.ainv[1][0] * p0 + g.ainv[1][1] * p1, sd.min1, sd.max1);
+            for (int i = i0 - 4; i < i0 + 4; ++i)
+                for (int j = j0 - 4; j < j0 + 4; ++j)
+                    for (int l = 0; l < g.norb; ++l) {
+                        const int k2 = grid_at(sd, g.norb, i, j, l);
+                        if (k2 == -1) continue;
+                        int gd0 = i - ih, gd1 = j - jh;
+                        if (sd.bc == 1) {
+                            gd0 = gd0 - dx * sd.st[0][0] - dy * sd.st[1][0];
+                            gd1 = gd1 - dx * sd.st[0][1] - dy * sd.st[1][1];
+                        }
+                        const double t = intralayer(sd.mat, l0, l, gd0, gd1);
+                        if (t != 0 || (dx == 0 && dy == 0)) out.push(k2 + sd.base, dx, dy);
+                    }
+        }
+}
+
+__device__ void search_inter(const GeomDev& g, int kh, PairSink& out)
+{
+    const int sh = g.i2g[4 * (size_t)kh + 3];
+    const double kp0 = g.i2p[3 * (size_t)kh], kp1 = g.i2p[3 * (size_t)kh + 1];
+    const int S = g.bc == 1 ? g.sc_search : 0;
+    const double cut2 = 12.5 * 12.5;
+    const double zero = 0.0;
+    for (int dx = -S; dx < S + 1; ++dx)
+        for (int dy = -S; dy < S + 1; ++dy) {
+            double scd0 = 0.0, scd1 = 0.0;
+            if (g.bc == 1) {
+                scd0 = dx * g.gsc[0][0] + dy * g.gsc[1][0];
+                scd1 = dx * g.gsc[0][1] + dy * g.gsc[1][1];
+            }
+            const double p0 = kp0 + scd0, p1 = kp1 + scd1;
+            for (int side = 0; side < 2; ++side) {
+                int s0;
+                if (side == 0) {
+                    if (!(sh > 0)) continue;
+                    s0 = sh - 1;
+                } else {
+                    if (!(sh < g.num_sheets - 1)) continue;
+                    s0 = sh + 1;
+                }
+                const SheetDev& t = g.s[s0];
+                // Hstruct::findNearest: rotate by -angle[s0] (cosines from the host libm)
+                const double x_new = p0 * t.cneg - p1 * t.sneg + zero * g.lat[0][0] + zero * g.lat[1][0];
+                const double y_new0 = p0 * t.sneg + p1 * t.cneg + zero * g.lat[0][1] + zero * g.lat[1][1];
+                const double y_new1 = p1 * t.cneg + p0 * t.sneg + zero * g.lat[0][1] + zero * g.lat[1][1];
+                const int i0 = clamp_cell(g.ainv[0][0] * x_new + g.ainv[0][1] * y_new0, t.min0, t.max0);
+                const int j0 = clamp_cell(g.ainv[1][0] * x_new + g.ainv[1][1] * y_new1, t.min1, t.max1);
+                const int ilo = max(0, i0 - 6), ihi = min(t.H, i0 + 6);
+                const int jlo = max(0, j0 - 6), jhi = min(t.W, j0 + 6);
+                for (int i = ilo; i < ihi; ++i)
+                    for (int j = jlo; j < jhi; ++j)
+                        for (int l = 0; l < g.norb; ++l) {
+                            const int k2 = t.grid[((size_t)i * t.W + j) * g.norb + l];
+                            if (k2 == -1) continue;
+                            const size_t K = (size_t)k2 + t.base;
+                            const double x2 = g.i2p[3 * K], y2 = g.i2p[3 * K + 1];
+                            if ((x2 - p0) * (x2 - p0) + (y2 - p1) * (y2 - p1) < cut2) out.push((int)K, dx, dy);
+                        }
+            }
+        }
+}
+
+__global__ __launch_bounds__(kThreads) void pair_count_kernel(GeomDev g, int32_t* cnt_intra, int64_t* cnt_total)
+{
+    const int kh = blockIdx.x * kThreads + threadIdx.x;
+    if (kh >= g.n) return;
+    PairSink a = {nullptr, nullptr, 0, 0}, b = {nullptr, nullptr, 0, 0};
+    search_intra(g, kh, a);
+    search_inter(g, kh, b);
+    cnt_intra[kh] = a.count;
+    cnt_total[kh] = (int64_t)a.count + b.count;
+}
+
+__device__ void sort_by_col(int32_t* col, int32_t* sc, int64_t n)
+{
+    // stable insertion sort == the reference's bubble sort permutation
+    for (int64_t i = 1; i < n; ++i) {
+        const int32_t c = col[i], s = sc[i];
+        int64_t j = i - 1;
+        while (j >= 0 && col[j] > c) {
+            col[j + 1] = col[j];
+            sc[j + 1] = sc[j];
+            --j;
+        }
+        col[j + 1] = c;
+        sc[j + 1] = s;
+    }
+}
+
+__device__ inline int64_t distinct_runs(const int32_t* col, int64_t n)
+{
+    int64_t r = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (i == 0 || col[i] != col[i - 1]) ++r;
+    return r;
+}
+
+__global__ __launch_bounds__(kThreads) void pair_write_kernel(GeomDev g, const int64_t* pair_rp, const int32_t* cnt_intra,
+                                                              int32_t* pcol, int32_t* psc, int64_t* nslots)
+{
+    const int kh = blockIdx.x * kThreads + threadIdx.x;
+    if (kh >= g.n) return;
+    const int64_t base = pair_rp[kh];
+    PairSink a = {pcol, psc, base, 0};
+    search_intra(g, kh, a);
+    PairSink b = {pcol, psc, base + a.count, 0};
+    search_inter(g, kh, b);
+    const int ni = cnt_intra[kh];
+    const int64_t nt = pair_rp[kh + 1] - base;
+    const int sh = g.i2g[4 * (size_t)kh + 3];
+    if (g.s[sh].bc == 1) sort_by_col(pcol + base, psc + base, ni);
+    if (g.bc == 1) sort_by_col(pcol + base + ni, psc + base + ni, nt - ni);
+    nslots[kh] = distinct_runs(pcol + base, ni) + distinct_runs(pcol + base + ni, nt - ni);
+}
+
+// ---- per-job fill -----------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j, double* i2pos)
+{
+    // Locality::setConfigPositions (locality.cpp:3618-3643, 3849-3857)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= g.n) return;
+    const int s = g.i2g[4 * (size_t)i + 3];
+    const double* a = j.s1a[s];   // rotated lattice of sheet s: a1 = (a[0], a[1]), a2 = (a[2], a[3])
+    const double sh0 = j.shift[s][0], sh1 = j.shift[s][1];
+    double x = g.i2p[3 * (size_t)i + 0] + sh0 * a[0] + sh1 * a[2];
+    double y = g.i2p[3 * (size_t)i + 1] + sh0 * a[1] + sh1 * a[3];
+    if (j.uniform_strain == 1) {
+        const double tx = x, ty = y;
+        x = (1.0 + j.u[0][0]) * tx + j.u[0][1] * ty;
+        y = j.u[1][0] * tx + (1.0 + j.u[1][1]) * ty;
+    }
+    i2pos[3 * (size_t)i + 0] = x;
+    i2pos[3 * (size_t)i + 1] = y;
+    i2pos[3 * (size_t)i + 2] = g.i2p[3 * (size_t)i + 2];
+}
+
+template <bool CPX>
+__global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, const double* i2pos, const int64_t* pair_rp,
+                                                        const int32_t* cnt_intra, const int32_t* pcol,
+                                                        const int32_t* psc, const int64_t* slot_rp, int32_t* scol,
+                                                        double* sval, uint8_t* skeep)
+{
+    const int k_i = blockIdx.x * kThreads + threadIdx.x;
+    if (k_i >= g.n) return;
+    const int64_t q0 = pair_rp[k_i], q1 = pair_rp[k_i + 1];
+    const int64_t qi = q0 + cnt_intra[k_i];
+    int64_t slot = slot_rp[k_i];
+    const int s_i = g.i2g[4 * (size_t)k_i + 3];
+    const int mat = g.s[s_i].mat;
+    const int ih = g.i2g[4 * (size_t)k_i + 0], jh = g.i2g[4 * (size_t)k_i + 1], l0 = g.i2g[4 * (size_t)k_i + 2];
+    const double xi = i2pos[3 * (size_t)k_i], yi = i2pos[3 * (size_t)k_i + 1], zi = i2pos[3 * (size_t)k_i + 2];
+    const double resc = j.energy_rescale, eshift = j.energy_shift;
+    double ar = 0.0, ai = 0.0;
+    bool keep = false;
+    for (int64_t q = q0; q < q1; ++q) {
+        const bool intra = q < qi;
+        const int new_k = pcol[q];
+        const int32_t scp = psc[q];
+        const int sc_i = (int)(int16_t)(scp & 0xffff), sc_j = (int)(int16_t)((uint32_t)scp >> 16);
+        double psx = 0.0, psy = 0.0;
+        if (g.bc == 1) {
+            psx = sc_i * j.sc[0][0] + sc_j * j.sc[1][0];
+            psy = sc_i * j.sc[0][1] + sc_j * j.sc[1][1];
+        }
+        const double x1 = xi + psx, y1 = yi + psy, z1 = zi;
+        const double x2 = i2pos[3 * (size_t)new_k], y2 = i2pos[3 * (size_t)new_k + 1], z2 = i2pos[3 * (size_t)new_k + 2];
+        double t;
+        bool support;
+        if (intra) {
+            const SheetDev& sd = g.s[s_i];
+            int gd0 = g.i2g[4 * (size_t)new_k + 0] - ih, gd1 = g.i2g[4 * (size_t)new_k + 1] - jh;
+            if (sd.bc == 1) {
+                gd0 = gd0 - sc_i * sd.st[0][0] - sc_j * sd.st[1][0];
+                gd1 = gd1 - sc_i * sd.st[0][1] - sc_j * sd.st[1][1];
+            }
+            const double raw_t = intralayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1);
+            const bool diag = CPX ? (new_k == k_i && sc_i == 0 && sc_j == 0) : (new_k == k_i);
+            if (diag) {
+                if (j.elec_on == 1) t = (raw_t + eshift + z1 * j.E) / resc;
+                else t = (raw_t + eshift) / resc;
+            } else {
+                t = raw_t / resc;
+            }
+            support = raw_t != 0 || (diag && (eshift != 0 || j.elec_on == 1));
+        } else {
+            const int s2 = g.i2g[4 * (size_t)new_k + 3];
+            const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+            t = interlayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], dx, dy, dz, g.s[s_i].angle, g.s[s2].angle) / resc;
+            support = inter_support(mat, dx, dy, dz);
+        }
+        if (!CPX) {
+            if (t != 0) ar += t;
+        } else {
+            // peierlsPhase (locality.cpp:7157-7161) and k.(r2 - r1) (locality.cpp:4896-4905)
+            double phase = 3.14e-5 * j.B * (x2 - x1) * (0.5) * (y1 + y2);
+            phase = phase + j.k[0] * (x2 - x1) + j.k[1] * (y2 - y1);
+            double sn, cs;
+            sincos(phase, &sn, &cs);
+            const double tr = t * cs, ti = t * sn;
+            if (tr != 0 || ti != 0) {
+                ar += tr;
+                ai += ti;
+            }
+        }
+        keep = keep || support;
+        const bool last = (q + 1 == q1) || (q + 1 == qi) || pcol[q + 1] != new_k;
+        if (last) {
+            scol[slot] = new_k;
+            if (CPX) {
+                sval[2 * slot] = ar;
+                sval[2 * slot + 1] = ai;
+            } else {
+                sval[slot] = ar;
+            }
+            skeep[slot] = keep ? 1 : 0;
+            ++slot;
+            ar = 0.0;
+            ai = 0.0;
+            keep = false;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void keep_count_kernel(int n, const int64_t* slot_rp, const uint8_t* skeep,
+                                                              int64_t* cnt)
+{
+    const int r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n) return;
+    int64_t c = 0;
+    for (int64_t q = slot_rp[r]; q < slot_rp[r + 1]; ++q) c += skeep[q];
+    cnt[r] = c;
+}
+
+template <bool CPX>
+__global__ __launch_bounds__(kThreads) void compact_kernel(int n, const int64_t* slot_rp, const uint8_t* skeep,
+                                                           const int32_t* scol, const double* sval, const int64_t* out_rp,
+                                                           int32_t* ocol, double* oval)
+{
+    const int r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n) return;
+    int64_t w = out_rp[r];
+    for (int64_t q = slot_rp[r]; q < slot_rp[r + 1]; ++q) {
+        if (!skeep[q]) continue;
+        ocol[w] = scol[q];
+        if (CPX) {
+            oval[2 * w] = sval[2 * q];
+            oval[2 * w + 1] = sval[2 * q + 1];
+        } else {
+            oval[w] = sval[q];
+        }
+        ++w;
+    }
+}
+
+// exclusive scan of n int64 counts into out[0..n] (out[n] = total)
+hipError_t scan_counts(const int64_t* cnt, int64_t* out, int n, hipStream_t s)
+{
+    size_t tmp_bytes = 0;
+    hipError_t e = hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, cnt, out + 1, n, s);
+    if (e != hipSuccess) return e;
+    void* tmp = nullptr;
+    e = hipMallocAsync(&tmp, tmp_bytes + 16, s);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt, out + 1, n, s);
+    hipFreeAsync(tmp, s);
+    if (e != hipSuccess) return e;
+    return hipMemsetAsync(out, 0, sizeof(int64_t), s);
+}
+
+}  // namespace
+
+hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total, int64_t* pair_rp, hipStream_t s)
+{
+    hipLaunchKernelGGL(pair_count_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, cnt_intra, cnt_total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return scan_counts(cnt_total, pair_rp, g.n, s);
+}
+
+hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* pcol, int32_t* psc,
+                       int64_t* nslots, int64_t* slot_rp, hipStream_t s)
+{
+    hipLaunchKernelGGL(pair_write_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, pair_rp, cnt_intra, pcol, psc, nslots);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return scan_counts(nslots, slot_rp, g.n, s);
+}
+
+hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipStream_t s)
+{
+    hipLaunchKernelGGL(positions_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos);
+    return hipGetLastError();
+}
+
+hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
+                      const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
+                      int32_t* scol, double* sval, uint8_t* skeep, hipStream_t s)
+{
+    if (j.complex_matrix)
+        hipLaunchKernelGGL(fill_kernel<true>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
+                           pcol, psc, slot_rp, scol, sval, skeep);
+    else
+        hipLaunchKernelGGL(fill_kernel<false>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
+                           pcol, psc, slot_rp, scol, sval, skeep);
+    return hipGetLastError();
+}
+
+hipError_t prune_count(int n, const int64_t* slot_rp, const uint8_t* skeep, int64_t* cnt, int64_t* out_rp, hipStream_t s)
+{
+    hipLaunchKernelGGL(keep_count_kernel, rows_grid(n), dim3(kThreads), 0, s, n, slot_rp, skeep, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return scan_counts(cnt, out_rp, n, s);
+}
+
+hipError_t prune_compact(int n, int is_complex, const int64_t* slot_rp, const uint8_t* skeep, const int32_t* scol,
+                         const double* sval, const int64_t* out_rp, int32_t* ocol, double* oval, hipStream_t s)
+{
+    if (is_complex)
+        hipLaunchKernelGGL(compact_kernel<true>, rows_grid(n), dim3(kThreads), 0, s, n, slot_rp, skeep, scol, sval,
+                           out_rp, ocol, oval);
+    else
+        hipLaunchKernelGGL(compact_kernel<false>, rows_grid(n), dim3(kThreads), 0, s, n, slot_rp, skeep, scol, sval,
+                           out_rp, ocol, oval);
+    return hipGetLastError();
+}
+
+}  // namespace leo2d

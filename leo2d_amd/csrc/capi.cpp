@@ -1,0 +1,959 @@
+// capi.cpp -- implementation of the C-ABI in include/leo2d.h.
+//
+// One leo2d_handle = one HIP stream + the device state of one worker: the
+// uploaded geometry, the pair lists (built once, like constructGeom), the
+// current Hamiltonian (rebuilt per job, like workerChebSolve) and scratch for
+// the KPM recursion.  No exception crosses the ABI: every entry point catches
+// and converts to a LEO2D_ERR_* code with a message in leo2d_last_error().
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "assemble.h"
+#include "host.h"
+#include "kernels.h"
+
+struct leo2d_deck {
+    leo2d::Deck d;
+};
+
+struct leo2d_geom {
+    leo2d::Deck deck;   // after supercell setup
+    leo2d::Geom g;
+};
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? LEO2D_ERR_ALLOC : LEO2D_ERR_HIP;
+        throw HipError(std::string(what) + ": " + hipGetErrorString(e), code);
+    }
+}
+
+void dalloc(DevBuf& b, size_t bytes, const char* what)
+{
+    b.release();
+    if (bytes == 0) bytes = 16;
+    check(hipMalloc(&b.p, bytes), what);
+    b.bytes = bytes;
+}
+
+struct ArgError : std::runtime_error {
+    explicit ArgError(const std::string& m) : std::runtime_error(m) {}
+};
+struct StateError : std::runtime_error {
+    explicit StateError(const std::string& m) : std::runtime_error(m) {}
+};
+struct NanError : std::runtime_error {
+    explicit NanError(const std::string& m) : std::runtime_error(m) {}
+};
+
+int next_pow2(int x)
+{
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct leo2d_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // geometry
+    bool has_geom = false;
+    leo2d::Geom geom;
+    leo2d::GeomDev gdev{};
+    std::vector<DevBuf> grids;
+    DevBuf i2g, i2p, lpos;
+    // pairs
+    bool has_pairs = false;
+    int64_t n_pairs = 0, n_slots = 0, n_intra = 0;
+    DevBuf cnt_intra, cnt_tmp, pair_rp, pcol, psc, nslots, slot_rp;
+    // matrix
+    bool has_csr = false;
+    int n = 0;
+    int64_t nnz = 0;
+    int is_complex = 0;
+    DevBuf rp, col, val, val32, i2pos;
+    bool val32_valid = false;
+    leo2d_timing timing{};
+    std::vector<hipEvent_t> events;
+
+    ~leo2d_handle()
+    {
+        for (auto& b : grids) b.release();
+        for (DevBuf* b : {&i2g, &i2p, &lpos, &cnt_intra, &cnt_tmp, &pair_rp, &pcol, &psc, &nslots, &slot_rp, &rp, &col,
+                          &val, &val32, &i2pos})
+            b->release();
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename F>
+int guarded(leo2d_handle* h, F&& f)
+{
+    try {
+        if (h) {
+            check(hipSetDevice(h->device), "hipSetDevice");
+        }
+        return f();
+    } catch (const HipError& e) {
+        if (h) h->err = e.what();
+        return e.code;
+    } catch (const leo2d::DeckError& e) {
+        if (h) h->err = e.what();
+        return LEO2D_ERR_DECK;
+    } catch (const ArgError& e) {
+        if (h) h->err = e.what();
+        return LEO2D_ERR_ARG;
+    } catch (const StateError& e) {
+        if (h) h->err = e.what();
+        return LEO2D_ERR_STATE;
+    } catch (const NanError& e) {
+        if (h) h->err = e.what();
+        return LEO2D_ERR_NAN;
+    } catch (const std::bad_alloc&) {
+        if (h) h->err = "host allocation failed";
+        return LEO2D_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        if (h) h->err = e.what();
+        return LEO2D_ERR_ARG;
+    }
+}
+
+hipEvent_t event_at(leo2d_handle* h, size_t i)
+{
+    while (h->events.size() <= i) {
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        h->events.push_back(e);
+    }
+    return h->events[i];
+}
+
+void to_device(DevBuf& b, const void* src, size_t bytes, hipStream_t s, const char* what)
+{
+    dalloc(b, bytes, what);
+    if (bytes) check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s), what);
+}
+
+leo2d::JobDev job_dev(const leo2d::Geom& g, const leo2d_job& j)
+{
+    leo2d::JobDev d{};
+    for (int s = 0; s < g.num_sheets; ++s) {
+        d.shift[s][0] = j.shift[s][0];
+        d.shift[s][1] = j.shift[s][1];
+        // rotated lattice of the sheet (locality.cpp:3633-3638), host libm
+        const double th = g.angle[s];
+        for (int dd = 0; dd < 2; ++dd) {
+            d.s1a[s][2 * dd + 0] = std::cos(th) * g.lat[dd][0] - std::sin(th) * g.lat[dd][1];
+            d.s1a[s][2 * dd + 1] = std::sin(th) * g.lat[dd][0] + std::cos(th) * g.lat[dd][1];
+        }
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) d.sc[a][b] = g.gsc[a][b];
+    if (g.bc == 1 && j.uniform_strain == 1) {
+        // locality.cpp:2947-2960
+        for (int i = 0; i < 2; ++i) {
+            d.sc[i][0] = (1.0 + j.u[0][0]) * g.gsc[i][0] + j.u[0][1] * g.gsc[i][1];
+            d.sc[i][1] = j.u[1][0] * g.gsc[i][0] + (1.0 + j.u[1][1]) * g.gsc[i][1];
+        }
+    }
+    d.energy_rescale = j.energy_rescale;
+    d.energy_shift = j.energy_shift;
+    d.elec_on = j.elec_on;
+    d.E = j.E;
+    d.B = j.B;
+    d.k[0] = j.k[0];
+    d.k[1] = j.k[1];
+    d.complex_matrix = j.complex_matrix;
+    d.uniform_strain = j.uniform_strain;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) d.u[a][b] = j.u[a][b];
+    return d;
+}
+
+void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
+{
+    h->n = n;
+    h->nnz = nnz;
+    h->is_complex = cpx;
+    h->has_csr = true;
+    h->val32_valid = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- deck -------------------------------------------------------------------
+int leo2d_deck_parse(const char* text, leo2d_deck** out, char* err, int errlen)
+{
+    if (!text || !out) return LEO2D_ERR_ARG;
+    try {
+        auto* d = new leo2d_deck;
+        d->d = leo2d::parse_deck(text);
+        *out = d;
+        return LEO2D_OK;
+    } catch (const std::exception& e) {
+        if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", e.what());
+        return LEO2D_ERR_DECK;
+    }
+}
+
+void leo2d_deck_free(leo2d_deck* d) { delete d; }
+
+int leo2d_deck_get_int(const leo2d_deck* d, const char* key, int64_t* value)
+{
+    if (!d || !key || !value) return LEO2D_ERR_ARG;
+    auto it = d->d.opts.i.find(key);
+    if (it == d->d.opts.i.end()) return LEO2D_ERR_ARG;
+    *value = it->second;
+    return LEO2D_OK;
+}
+
+int leo2d_deck_get_double(const leo2d_deck* d, const char* key, double* value)
+{
+    if (!d || !key || !value) return LEO2D_ERR_ARG;
+    auto it = d->d.opts.d.find(key);
+    if (it == d->d.opts.d.end()) return LEO2D_ERR_ARG;
+    *value = it->second;
+    return LEO2D_OK;
+}
+
+int leo2d_deck_get_string(const leo2d_deck* d, const char* key, char* buf, int buflen)
+{
+    if (!d || !key || !buf || buflen <= 0) return LEO2D_ERR_ARG;
+    auto it = d->d.opts.s.find(key);
+    if (it == d->d.opts.s.end()) return LEO2D_ERR_ARG;
+    std::snprintf(buf, (size_t)buflen, "%s", it->second.c_str());
+    return LEO2D_OK;
+}
+
+// ---- geometry and jobs --------------------------------------------------------
+int leo2d_geom_build(const leo2d_deck* d, leo2d_geom** out)
+{
+    if (!d || !out) return LEO2D_ERR_ARG;
+    try {
+        auto* g = new leo2d_geom;
+        g->deck = d->d;
+        leo2d::setup_supercell(g->deck);
+        g->g = leo2d::build_geometry(g->deck);
+        *out = g;
+        return LEO2D_OK;
+    } catch (const leo2d::DeckError&) {
+        return LEO2D_ERR_DECK;
+    } catch (const std::bad_alloc&) {
+        return LEO2D_ERR_ALLOC;
+    } catch (const std::exception&) {
+        return LEO2D_ERR_DECK;
+    }
+}
+
+void leo2d_geom_free(leo2d_geom* g) { delete g; }
+
+int leo2d_geom_get_info(const leo2d_geom* g, leo2d_geom_info* info)
+{
+    if (!g || !info) return LEO2D_ERR_ARG;
+    std::memset(info, 0, sizeof(*info));
+    info->n = g->g.n;
+    info->num_sheets = g->g.num_sheets;
+    for (int s = 0; s < g->g.num_sheets; ++s) {
+        info->sheet_count[s] = g->g.count[s];
+        info->angle[s] = g->g.angle[s];
+    }
+    info->boundary_condition = g->g.bc;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) info->supercell[a][b] = g->g.gsc[a][b];
+    return LEO2D_OK;
+}
+
+int leo2d_geom_arrays(const leo2d_geom* g, int32_t* index_to_grid, double* index_to_pos)
+{
+    if (!g) return LEO2D_ERR_ARG;
+    if (index_to_grid) std::memcpy(index_to_grid, g->g.i2g.data(), sizeof(int32_t) * g->g.i2g.size());
+    if (index_to_pos) std::memcpy(index_to_pos, g->g.i2p.data(), sizeof(double) * g->g.i2p.size());
+    return LEO2D_OK;
+}
+
+int leo2d_jobs_count(const leo2d_deck* d, const leo2d_geom* g, int32_t* count)
+{
+    if (!g || !count) return LEO2D_ERR_ARG;
+    (void)d;
+    try {
+        *count = (int32_t)leo2d::build_jobs(g->deck, g->g).size();
+        return LEO2D_OK;
+    } catch (const std::exception&) {
+        return LEO2D_ERR_DECK;
+    }
+}
+
+int leo2d_job_get(const leo2d_deck* d, const leo2d_geom* g, int32_t index, leo2d_job* job)
+{
+    if (!g || !job) return LEO2D_ERR_ARG;
+    (void)d;
+    try {
+        auto jobs = leo2d::build_jobs(g->deck, g->g);
+        if (index < 0 || index >= (int32_t)jobs.size()) return LEO2D_ERR_ARG;
+        *job = jobs[(size_t)index];
+        return LEO2D_OK;
+    } catch (const std::exception&) {
+        return LEO2D_ERR_DECK;
+    }
+}
+
+// ---- handle -----------------------------------------------------------------
+int leo2d_create(int32_t device, leo2d_handle** out)
+{
+    if (!out) return LEO2D_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return LEO2D_ERR_HIP;
+    if (device < 0 || device >= count) return LEO2D_ERR_ARG;
+    auto* h = new leo2d_handle;
+    h->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+        delete h;
+        return LEO2D_ERR_HIP;
+    }
+    *out = h;
+    return LEO2D_OK;
+}
+
+void leo2d_destroy(leo2d_handle* h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+}
+
+const char* leo2d_last_error(const leo2d_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
+{
+    if (!h || !gin) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        const leo2d::Geom& g = gin->g;
+        if (g.n <= 0) throw ArgError("geometry has no orbitals");
+        int mat0 = g.sheets[0].mat;
+        for (const auto& sd : g.sheets)
+            if (sd.mat != mat0) throw ArgError("sheets with different coupling models are not supported");
+        h->geom = g;
+        h->has_pairs = false;
+        h->has_csr = false;
+        hipStream_t s = h->stream;
+        for (auto& b : h->grids) b.release();
+        h->grids.assign(g.num_sheets, DevBuf());
+        leo2d::GeomDev d{};
+        d.n = g.n;
+        d.num_sheets = g.num_sheets;
+        d.norb = g.norb;
+        d.bc = g.bc;
+        d.sc_search = g.sc_search;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                d.gsc[a][b] = g.gsc[a][b];
+                d.ainv[a][b] = g.ainv[a][b];
+                d.lat[a][b] = g.lat[a][b];
+            }
+        for (int sidx = 0; sidx < g.num_sheets; ++sidx) {
+            to_device(h->grids[sidx], g.grid[sidx].data(), sizeof(int32_t) * g.grid[sidx].size(), s, "grid upload");
+            leo2d::SheetDev& sd = d.s[sidx];
+            const leo2d::SheetData& src = g.sheets[sidx];
+            sd.grid = static_cast<const int32_t*>(h->grids[sidx].p);
+            sd.H = g.H[sidx];
+            sd.W = g.W[sidx];
+            sd.base = g.base[sidx];
+            sd.count = g.count[sidx];
+            sd.min0 = src.min_shape[0];
+            sd.min1 = src.min_shape[1];
+            sd.max0 = src.max_shape[0];
+            sd.max1 = src.max_shape[1];
+            sd.mat = src.mat;
+            sd.bc = src.boundary_condition;
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    sd.sc[a][b] = src.supercell[a][b];
+                    sd.st[a][b] = src.stride[a][b];
+                }
+            sd.angle = g.angle[sidx];
+            const double th = -g.angle[sidx];   // Hstruct::findNearest rotates by -angle
+            sd.cneg = std::cos(th);
+            sd.sneg = std::sin(th);
+        }
+        to_device(h->i2g, g.i2g.data(), sizeof(int32_t) * g.i2g.size(), s, "index upload");
+        to_device(h->i2p, g.i2p.data(), sizeof(double) * g.i2p.size(), s, "position upload");
+        to_device(h->lpos, g.lpos.data(), sizeof(double) * g.lpos.size(), s, "position upload");
+        d.i2g = static_cast<const int32_t*>(h->i2g.p);
+        d.i2p = static_cast<const double*>(h->i2p.p);
+        d.lpos = static_cast<const double*>(h->lpos.p);
+        h->gdev = d;
+        check(hipStreamSynchronize(s), "geometry upload");
+        h->has_geom = true;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64_t* n_slots)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_geom) throw StateError("leo2d_pairs_build before leo2d_set_geometry");
+        const int n = h->gdev.n;
+        hipStream_t s = h->stream;
+        dalloc(h->cnt_intra, sizeof(int32_t) * n, "pair counts");
+        dalloc(h->cnt_tmp, sizeof(int64_t) * n, "pair counts");
+        dalloc(h->pair_rp, sizeof(int64_t) * (n + 1), "pair offsets");
+        check(leo2d::pairs_count(h->gdev, (int32_t*)h->cnt_intra.p, (int64_t*)h->cnt_tmp.p, (int64_t*)h->pair_rp.p, s),
+              "pair count");
+        int64_t total = 0;
+        check(hipMemcpyAsync(&total, (int64_t*)h->pair_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "pair total");
+        std::vector<int32_t> ci(n);
+        check(hipMemcpyAsync(ci.data(), h->cnt_intra.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s), "pair counts");
+        check(hipStreamSynchronize(s), "pair count");
+        int64_t ni = 0;
+        for (int32_t c : ci) ni += c;
+        dalloc(h->pcol, sizeof(int32_t) * (size_t)total, "pair columns");
+        dalloc(h->psc, sizeof(int32_t) * (size_t)total, "pair images");
+        dalloc(h->nslots, sizeof(int64_t) * n, "slot counts");
+        dalloc(h->slot_rp, sizeof(int64_t) * (n + 1), "slot offsets");
+        check(leo2d::pairs_write(h->gdev, (const int64_t*)h->pair_rp.p, (const int32_t*)h->cnt_intra.p,
+                                 (int32_t*)h->pcol.p, (int32_t*)h->psc.p, (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
+              "pair write");
+        int64_t slots = 0;
+        check(hipMemcpyAsync(&slots, (int64_t*)h->slot_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "slot total");
+        check(hipStreamSynchronize(s), "pair write");
+        h->cnt_tmp.release();
+        h->nslots.release();
+        h->n_pairs = total;
+        h->n_intra = ni;
+        h->n_slots = slots;
+        h->has_pairs = true;
+        if (n_intra) *n_intra = ni;
+        if (n_inter) *n_inter = total - ni;
+        if (n_slots) *n_slots = slots;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, double* t)
+{
+    if (!h || (which != 0 && which != 1)) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_pairs) throw StateError("no pair lists: call leo2d_pairs_build");
+        const int n = h->gdev.n;
+        std::vector<int64_t> rp(n + 1);
+        std::vector<int32_t> ci(n), pc(h->n_pairs), ps(h->n_pairs);
+        hipStream_t s = h->stream;
+        check(hipMemcpyAsync(rp.data(), h->pair_rp.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "copy");
+        check(hipMemcpyAsync(ci.data(), h->cnt_intra.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s), "copy");
+        if (h->n_pairs) {
+            check(hipMemcpyAsync(pc.data(), h->pcol.p, sizeof(int32_t) * h->n_pairs, hipMemcpyDeviceToHost, s), "copy");
+            check(hipMemcpyAsync(ps.data(), h->psc.p, sizeof(int32_t) * h->n_pairs, hipMemcpyDeviceToHost, s), "copy");
+        }
+        check(hipStreamSynchronize(s), "copy");
+        const leo2d::Geom& g = h->geom;
+        int64_t w = 0;
+        for (int r = 0; r < n; ++r) {
+            const int64_t b = which == 0 ? rp[r] : rp[r] + ci[r];
+            const int64_t e = which == 0 ? rp[r] + ci[r] : rp[r + 1];
+            for (int64_t q = b; q < e; ++q, ++w) {
+                const int dx = (int)(int16_t)(ps[q] & 0xffff), dy = (int)(int16_t)((uint32_t)ps[q] >> 16);
+                if (ij) { ij[2 * w] = r; ij[2 * w + 1] = pc[q]; }
+                if (sc) { sc[2 * w] = dx; sc[2 * w + 1] = dy; }
+                if (t && which == 0) {
+                    // tabulated intralayer hopping of the pair (graphene.cpp:13-89)
+                    const int s0 = g.i2g[4 * (size_t)r + 3];
+                    const leo2d::SheetData& sd = g.sheets[s0];
+                    int gd0 = g.i2g[4 * (size_t)pc[q]] - g.i2g[4 * (size_t)r];
+                    int gd1 = g.i2g[4 * (size_t)pc[q] + 1] - g.i2g[4 * (size_t)r + 1];
+                    if (sd.boundary_condition == 1) {
+                        gd0 = gd0 - dx * sd.stride[0][0] - dy * sd.stride[1][0];
+                        gd1 = gd1 - dx * sd.stride[0][1] - dy * sd.stride[1][1];
+                    }
+                    const int o1 = g.i2g[4 * (size_t)r + 2], o2 = g.i2g[4 * (size_t)pc[q] + 2];
+                    int h0 = 2 * gd0 + gd1;
+                    const int h1 = -1 * gd0 + gd1;
+                    if (o2 == 0 && o1 == 1) h0 -= 1;
+                    else if (o2 == 1 && o1 == 0) h0 += 1;
+                    const int rr = h0 * (h0 + h1) + h1 * h1;
+                    static const double tab[17] = {0.3504, -2.8922, 0, 0.2425, -0.2656, 0, 0, 0.0235, 0, 0.0524,
+                                                   0, 0, -0.0209, -0.0148, 0, 0, -0.0211};
+                    double v = (rr >= 0 && rr <= 16) ? tab[rr] : 0.0;
+                    if (sd.mat == leo2d::MAT_GRAPHENE_SIMPLE) v = rr == 1 ? -2.7 : 0.0;
+                    t[w] = v;
+                }
+            }
+        }
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out)
+{
+    if (!h || !job || (pattern != LEO2D_PATTERN_REFERENCE && pattern != LEO2D_PATTERN_PRUNED)) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_geom) throw StateError("leo2d_assemble before leo2d_set_geometry");
+        if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
+            throw ArgError("energy_rescale must be finite and non-zero");
+        if (!h->has_pairs) {
+            int rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
+            if (rc != LEO2D_OK) return rc;
+        }
+        const int n = h->gdev.n;
+        const int cpx = job->complex_matrix ? 1 : 0;
+        hipStream_t s = h->stream;
+        leo2d::JobDev jd = job_dev(h->geom, *job);
+        if (!h->i2pos.p || h->i2pos.bytes < sizeof(double) * 3 * (size_t)n)
+            dalloc(h->i2pos, sizeof(double) * 3 * (size_t)n, "job positions");
+        check(leo2d::job_positions(h->gdev, jd, (double*)h->i2pos.p, s), "job positions");
+        DevBuf scol, sval, skeep;
+        dalloc(scol, sizeof(int32_t) * (size_t)h->n_slots, "slot columns");
+        dalloc(sval, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "slot values");
+        dalloc(skeep, (size_t)h->n_slots + 1, "slot flags");
+        check(leo2d::fill_slots(h->gdev, jd, (const double*)h->i2pos.p, (const int64_t*)h->pair_rp.p,
+                                (const int32_t*)h->cnt_intra.p, (const int32_t*)h->pcol.p, (const int32_t*)h->psc.p,
+                                (const int64_t*)h->slot_rp.p, (int32_t*)scol.p, (double*)sval.p, (uint8_t*)skeep.p, s),
+              "hamiltonian fill");
+        h->rp.release();
+        h->col.release();
+        h->val.release();
+        if (pattern == LEO2D_PATTERN_REFERENCE) {
+            dalloc(h->rp, sizeof(int64_t) * (n + 1), "row pointers");
+            check(hipMemcpyAsync(h->rp.p, h->slot_rp.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s), "rp");
+            h->col = scol;
+            h->val = sval;
+            scol.p = nullptr;
+            sval.p = nullptr;
+            skeep.release();
+            set_csr_owned(h, n, h->n_slots, cpx);
+        } else {
+            DevBuf cnt;
+            dalloc(cnt, sizeof(int64_t) * n, "prune counts");
+            dalloc(h->rp, sizeof(int64_t) * (n + 1), "row pointers");
+            check(leo2d::prune_count(n, (const int64_t*)h->slot_rp.p, (const uint8_t*)skeep.p, (int64_t*)cnt.p,
+                                     (int64_t*)h->rp.p, s),
+                  "prune count");
+            int64_t kept = 0;
+            check(hipMemcpyAsync(&kept, (int64_t*)h->rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "prune");
+            check(hipStreamSynchronize(s), "prune");
+            dalloc(h->col, sizeof(int32_t) * (size_t)kept, "columns");
+            dalloc(h->val, sizeof(double) * (cpx ? 2 : 1) * (size_t)kept, "values");
+            check(leo2d::prune_compact(n, cpx, (const int64_t*)h->slot_rp.p, (const uint8_t*)skeep.p,
+                                       (const int32_t*)scol.p, (const double*)sval.p, (const int64_t*)h->rp.p,
+                                       (int32_t*)h->col.p, (double*)h->val.p, s),
+                  "prune compact");
+            check(hipStreamSynchronize(s), "prune compact");
+            cnt.release();
+            scol.release();
+            sval.release();
+            skeep.release();
+            set_csr_owned(h, n, kept, cpx);
+        }
+        check(hipStreamSynchronize(s), "assemble");
+        if (nnz_out) *nnz_out = h->nnz;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_csr_upload(leo2d_handle* h, int32_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                     const double* val, int32_t is_complex)
+{
+    if (!h || n <= 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!col || !val))) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (row_ptr[0] != 0 || row_ptr[n] != nnz) throw ArgError("row_ptr must start at 0 and end at nnz");
+        for (int i = 0; i < n; ++i)
+            if (row_ptr[i + 1] < row_ptr[i]) throw ArgError("row_ptr must be non-decreasing");
+        for (int64_t q = 0; q < nnz; ++q)
+            if (col[q] < 0 || col[q] >= n) throw ArgError("column index out of range");
+        const size_t w = is_complex ? 2 : 1;
+        for (size_t q = 0; q < w * (size_t)nnz; ++q)
+            if (!std::isfinite(val[q])) throw NanError("non-finite matrix value at entry " + std::to_string(q / w));
+        hipStream_t s = h->stream;
+        to_device(h->rp, row_ptr, sizeof(int64_t) * (n + 1), s, "row pointers");
+        to_device(h->col, col, sizeof(int32_t) * (size_t)nnz, s, "columns");
+        to_device(h->val, val, sizeof(double) * w * (size_t)nnz, s, "values");
+        check(hipStreamSynchronize(s), "upload");
+        set_csr_owned(h, n, nnz, is_complex ? 1 : 0);
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_csr_info(leo2d_handle* h, int32_t* n, int64_t* nnz, int32_t* is_complex)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    if (!h->has_csr) {
+        h->err = "no Hamiltonian on the device";
+        return LEO2D_ERR_STATE;
+    }
+    if (n) *n = h->n;
+    if (nnz) *nnz = h->nnz;
+    if (is_complex) *is_complex = h->is_complex;
+    return LEO2D_OK;
+}
+
+int leo2d_csr_copy(leo2d_handle* h, int64_t* row_ptr, int32_t* col, double* val)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr) throw StateError("no Hamiltonian on the device");
+        hipStream_t s = h->stream;
+        if (row_ptr) check(hipMemcpyAsync(row_ptr, h->rp.p, sizeof(int64_t) * (h->n + 1), hipMemcpyDeviceToHost, s), "copy");
+        if (col && h->nnz) check(hipMemcpyAsync(col, h->col.p, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "copy");
+        if (val && h->nnz)
+            check(hipMemcpyAsync(val, h->val.p, sizeof(double) * (h->is_complex ? 2 : 1) * h->nnz, hipMemcpyDeviceToHost, s),
+                  "copy");
+        check(hipStreamSynchronize(s), "copy");
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_spectral_bounds(leo2d_handle* h, double* lo, double* hi)
+{
+    if (!h || !lo || !hi) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr) throw StateError("no Hamiltonian on the device");
+        const int blocks = 1024;
+        DevBuf buf;
+        dalloc(buf, sizeof(double) * 2 * blocks, "bounds");
+        check(leo2d::launch_gershgorin((const int64_t*)h->rp.p, (const int32_t*)h->col.p, h->val.p, h->is_complex, h->n,
+                                       (double*)buf.p, blocks, h->stream),
+              "gershgorin");
+        std::vector<double> v(2 * blocks);
+        check(hipMemcpyAsync(v.data(), buf.p, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, h->stream), "bounds");
+        check(hipStreamSynchronize(h->stream), "bounds");
+        double l = 1e300, u = -1e300;
+        for (int b = 0; b < blocks; ++b) {
+            l = std::min(l, v[2 * b]);
+            u = std::max(u, v[2 * b + 1]);
+        }
+        *lo = l;
+        *hi = u;
+        return LEO2D_OK;
+    });
+}
+
+// ---- KPM --------------------------------------------------------------------
+int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
+{
+    if (!h || !p || !mu) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr) throw StateError("leo2d_kpm_moments before a Hamiltonian is installed");
+        const int M = p->M, R = p->R, n = h->n;
+        if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4 (POLY_ORDER % 4, main.cpp:914)");
+        if (R < 1) throw ArgError("R must be >= 1");
+        if (p->start < 0 || p->start > 2) throw ArgError("unknown start mode");
+        if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
+        if (p->scheme != LEO2D_SCHEME_REFERENCE && p->scheme != LEO2D_SCHEME_DOUBLING) throw ArgError("unknown scheme");
+        if (p->start == LEO2D_START_TARGETS) {
+            if (!p->targets) throw ArgError("targets missing");
+            for (int r = 0; r < R; ++r)
+                if (p->targets[r] < 0 || p->targets[r] >= n) throw ArgError("target index out of range");
+        }
+        if (p->start == LEO2D_START_VECTORS && !p->vectors) throw ArgError("vectors missing");
+        int B = p->block ? p->block : std::min(64, next_pow2(R));
+        if (B < 1 || B > 128 || (B & (B - 1))) throw ArgError("block must be a power of two <= 128");
+        const bool mixed = p->precision == LEO2D_PREC_MIXED;
+        const bool real_vec = !h->is_complex &&
+                              (p->start == LEO2D_START_TARGETS ||
+                               (p->start == LEO2D_START_VECTORS && !p->vectors_complex));
+        int mat_type, vec_type;
+        if (!mixed) {
+            mat_type = h->is_complex ? leo2d::MAT_CPX_F64 : leo2d::MAT_REAL_F64;
+            vec_type = real_vec ? leo2d::VEC_REAL_F64 : leo2d::VEC_CPX_F64;
+        } else {
+            mat_type = h->is_complex ? leo2d::MAT_CPX_F32 : leo2d::MAT_REAL_F32;
+            vec_type = real_vec ? leo2d::VEC_REAL_F32 : leo2d::VEC_CPX_F32;
+        }
+        hipStream_t s = h->stream;
+        const void* dval = h->val.p;
+        if (mixed) {
+            const size_t count = (size_t)h->nnz * (h->is_complex ? 2 : 1);
+            if (!h->val32_valid) {
+                dalloc(h->val32, sizeof(float) * count, "fp32 matrix");
+                check(leo2d::launch_convert_f64_to_f32((const double*)h->val.p, (float*)h->val32.p, (int64_t)count, s),
+                      "fp32 matrix");
+                h->val32_valid = true;
+            }
+            dval = h->val32.p;
+        }
+        // scheme on the device (see kernels.h)
+        int dev_scheme;
+        if (p->scheme == LEO2D_SCHEME_DOUBLING) dev_scheme = 1;
+        else dev_scheme = p->start == LEO2D_START_TARGETS ? 0 : 2;
+        const bool keep_v = dev_scheme == 2;
+        int G = p->grid;
+        if (G <= 0) {
+            const int rows_per_iter = std::max(1, 256 / B);   // 4 wavefronts x 64/B rows
+            const int want = (n + rows_per_iter - 1) / rows_per_iter;
+            G = std::min(2048, std::max(8, (want + 7) / 8 * 8));
+        }
+        if (G % 8 != 0) throw ArgError("grid must be a multiple of 8");
+        const size_t vb = leo2d::vec_bytes(vec_type);
+        DevBuf A, Bf, V, part, dmu, dtg, host_v;
+        dalloc(A, vb * (size_t)n * B, "vector block");
+        dalloc(Bf, vb * (size_t)n * B, "vector block");
+        if (keep_v) dalloc(V, vb * (size_t)n * B, "start vectors");
+        dalloc(part, sizeof(double) * 3 * (size_t)G * B, "partials");
+        dalloc(dmu, sizeof(double) * (size_t)M * B, "moments");
+        dalloc(dtg, sizeof(int32_t) * B, "targets");
+        std::vector<double> hv;
+        double rec_ms = 0.0, step_ms = 0.0;
+        int64_t steps = 0;
+        hipEvent_t e0 = event_at(h, 0), e1 = event_at(h, 1);
+        for (int b0 = 0; b0 < R; b0 += B) {
+            const int Rb = std::min(B, R - b0);
+            check(hipMemsetAsync(dmu.p, 0, sizeof(double) * (size_t)M * B, s), "moments");
+            if (p->start == LEO2D_START_TARGETS) {
+                std::vector<int32_t> tg(B);
+                for (int r = 0; r < B; ++r) tg[r] = p->targets[b0 + std::min(r, Rb - 1)];
+                check(hipMemcpyAsync(dtg.p, tg.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s), "targets");
+                check(hipStreamSynchronize(s), "targets");
+                check(leo2d::launch_init_targets(A.p, vec_type, n, B, (const int32_t*)dtg.p, s), "init");
+            } else if (p->start == LEO2D_START_RANDOM_PHASE) {
+                if (real_vec) throw ArgError("random phases need complex vectors");
+                check(leo2d::launch_init_random_phase(A.p, vec_type, n, B, p->seed, b0, s), "init");
+            } else {
+                const int w = p->vectors_complex ? 2 : 1;
+                hv.assign((size_t)w * n * B, 0.0);
+                std::memcpy(hv.data(), p->vectors + (size_t)w * n * b0, sizeof(double) * (size_t)w * n * Rb);
+                dalloc(host_v, sizeof(double) * hv.size(), "vector upload");
+                check(hipMemcpyAsync(host_v.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, s), "upload");
+                check(leo2d::launch_init_from_host_layout(A.p, vec_type, (const double*)host_v.p, p->vectors_complex, n, B,
+                                                          s),
+                      "init");
+            }
+            if (keep_v) check(hipMemcpyAsync(V.p, A.p, vb * (size_t)n * B, hipMemcpyDeviceToDevice, s), "copy start");
+            check(hipStreamSynchronize(s), "init");
+            leo2d::StepArgs sa{};
+            sa.row_ptr = (const int64_t*)h->rp.p;
+            sa.col = (const int32_t*)h->col.p;
+            sa.val = dval;
+            sa.n = n;
+            sa.R = B;
+            sa.part = (double*)part.p;
+            sa.grid = G;
+            sa.v = keep_v ? V.p : nullptr;
+            leo2d::ReduceArgs ra{};
+            ra.part = (const double*)part.p;
+            ra.grid = G;
+            ra.R = B;
+            ra.M = M;
+            ra.scheme = dev_scheme;
+            ra.targets = (const int32_t*)dtg.p;
+            ra.vec_type = vec_type;
+            ra.mu = (double*)dmu.p;
+            check(hipEventRecord(e0, s), "event");
+            for (int j = 0; j < M / 2; ++j) {
+                void* cur = (j % 2 == 0) ? A.p : Bf.p;
+                void* other = (j % 2 == 0) ? Bf.p : A.p;
+                sa.cur = cur;
+                sa.prev = other;
+                sa.out = other;
+                sa.alpha = j == 0 ? 1.0 : 2.0;
+                sa.beta = j == 0 ? 0 : 1;
+                if (p->time_steps) check(hipEventRecord(event_at(h, 2 + 2 * (size_t)j), s), "event");
+                check(leo2d::launch_kpm_step(mat_type, vec_type, sa, s), "kpm step");
+                if (p->time_steps) check(hipEventRecord(event_at(h, 3 + 2 * (size_t)j), s), "event");
+                ra.j = j;
+                ra.cur = cur;
+                ra.out = other;
+                check(leo2d::launch_kpm_reduce(ra, s), "kpm reduce");
+            }
+            check(hipEventRecord(e1, s), "event");
+            check(hipStreamSynchronize(s), "kpm recursion");
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, e0, e1), "event");
+            rec_ms += ms;
+            if (p->time_steps) {
+                for (int j = 0; j < M / 2; ++j) {
+                    float sm = 0.f;
+                    check(hipEventElapsedTime(&sm, event_at(h, 2 + 2 * (size_t)j), event_at(h, 3 + 2 * (size_t)j)), "event");
+                    step_ms += sm;
+                }
+            }
+            steps += M / 2;
+            std::vector<double> hm((size_t)M * B);
+            check(hipMemcpy(hm.data(), dmu.p, sizeof(double) * (size_t)M * B, hipMemcpyDeviceToHost), "moments");
+            std::memcpy(mu + (size_t)M * b0, hm.data(), sizeof(double) * (size_t)M * Rb);
+        }
+        h->timing.recursion_ms = rec_ms;
+        h->timing.step_ms_avg = p->time_steps && steps ? step_ms / (double)steps : 0.0;
+        h->timing.steps = steps;
+        h->timing.nnz = h->nnz;
+        h->timing.n = n;
+        h->timing.R = R;
+        h->timing.M = M;
+        h->timing.grid = G;
+        // moments of a spectrum inside [-1, 1] satisfy |mu_n| <= mu_0
+        for (int r = 0; r < R; ++r) {
+            const double* m = mu + (size_t)r * M;
+            for (int j = 0; j < M; ++j)
+                if (!std::isfinite(m[j])) {
+                    h->err = "non-finite moment (vector " + std::to_string(r) + ", n = " + std::to_string(j) + ")";
+                    return LEO2D_ERR_NAN;
+                }
+            const double lim = std::fabs(m[0]) * (1.0 + 1e-6) + 1e-12;
+            for (int j = 0; j < M; ++j)
+                if (std::fabs(m[j]) > lim) {
+                    h->err = "|mu_" + std::to_string(j) + "| > mu_0: rescaled spectrum leaves [-1, 1]";
+                    return LEO2D_ERR_SPECTRUM;
+                }
+        }
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_last_timing(const leo2d_handle* h, leo2d_timing* t)
+{
+    if (!h || !t) return LEO2D_ERR_ARG;
+    *t = h->timing;
+    return LEO2D_OK;
+}
+
+int leo2d_dos(leo2d_handle* h, const double* mu, int32_t T, int32_t M, double a, double b, double scale, double* E,
+              double* rho)
+{
+    if (!h || !mu || !rho || T < 1 || M < 2) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!(a > 0.0)) throw ArgError("energy rescale must be positive");
+        hipStream_t s = h->stream;
+        DevBuf dm, de, dr;
+        to_device(dm, mu, sizeof(double) * (size_t)T * M, s, "moments");
+        dalloc(de, sizeof(double) * M, "energies");
+        dalloc(dr, sizeof(double) * (size_t)T * M, "dos");
+        check(leo2d::launch_dos((const double*)dm.p, T, M, a, b, scale, (double*)de.p, (double*)dr.p, s), "dos");
+        check(hipMemcpyAsync(rho, dr.p, sizeof(double) * (size_t)T * M, hipMemcpyDeviceToHost, s), "dos");
+        if (E) check(hipMemcpyAsync(E, de.p, sizeof(double) * M, hipMemcpyDeviceToHost, s), "dos");
+        check(hipStreamSynchronize(s), "dos");
+        return LEO2D_OK;
+    });
+}
+
+// ---- whole deck ---------------------------------------------------------------
+int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefix, int32_t pattern)
+{
+    if (!h || !deck_text) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        leo2d_geom geo;
+        geo.deck = leo2d::parse_deck(deck_text);
+        leo2d::setup_supercell(geo.deck);
+        geo.g = leo2d::build_geometry(geo.deck);
+        const leo2d::Params& o = geo.deck.opts;
+        if (o.geti("diagonalize") != 0) throw leo2d::DeckError("DIAGONALIZE = 1 is not on this path");
+        if (o.geti("observable_type") != 0) throw leo2d::DeckError("OBSERVABLE_TYPE = COND is not on this path");
+        const std::string prefix = out_prefix ? std::string(out_prefix) : o.s.at("job_name");
+        std::vector<leo2d_job> jobs = leo2d::build_jobs(geo.deck, geo.g);
+        int rc = leo2d_set_geometry(h, &geo);
+        if (rc != LEO2D_OK) return rc;
+        rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
+        if (rc != LEO2D_OK) return rc;
+        const int M = (int)o.geti("poly_order");
+        const bool trace = o.geti("kpm_trace") == 1;
+        int samples = 0;
+        std::default_random_engine generator;                          // locality.cpp:2275-2276
+        std::normal_distribution<double> distribution(0.0, 1.0);
+        if (trace) {
+            samples = (int)o.geti("kpm_trace_samps");
+            if (samples == -1) {
+                samples = (int)((int)1.0e6 / ((double)geo.g.n));        // locality.cpp:2257-2270
+                if (samples < 1) samples = 1;
+                if (samples > 100) samples = 100;
+            }
+        }
+        std::vector<std::vector<double>> rho_all;
+        for (size_t q = 0; q < jobs.size(); ++q) {
+            const leo2d_job& job = jobs[q];
+            int64_t nnz = 0;
+            rc = leo2d_assemble(h, &job, pattern, &nnz);
+            if (rc != LEO2D_OK) return rc;
+            leo2d_kpm_params kp{};
+            kp.M = M;
+            kp.precision = LEO2D_PREC_FP64;
+            kp.scheme = LEO2D_SCHEME_REFERENCE;
+            std::vector<double> vecs;
+            if (!trace) {
+                kp.R = job.num_targets;
+                kp.start = LEO2D_START_TARGETS;
+                kp.targets = job.targets;
+            } else {
+                vecs.resize((size_t)samples * geo.g.n);
+                for (size_t t = 0; t < vecs.size(); ++t) vecs[t] = distribution(generator);
+                kp.R = samples;
+                kp.start = LEO2D_START_VECTORS;
+                kp.vectors = vecs.data();
+            }
+            std::vector<double> mu((size_t)kp.R * M);
+            rc = leo2d_kpm_moments(h, &kp, mu.data());
+            if (rc != LEO2D_OK) return rc;
+            const double a = job.energy_rescale, b = job.energy_shift;
+            std::vector<double> rho;
+            if (!trace) {
+                if (o.geti("dos_transform") == 1) {
+                    rho.resize(mu.size());
+                    rc = leo2d_dos(h, mu.data(), kp.R, M, a, b, 1.0, nullptr, rho.data());
+                    if (rc != LEO2D_OK) return rc;
+                } else {
+                    rho = mu;
+                }
+            } else {
+                // densityTransformTrace: average the moments first (param_tools.cpp:732-745)
+                std::vector<double> avg(M, 0.0);
+                for (int i = 0; i < M; ++i)
+                    for (int smp = 0; smp < samples; ++smp) avg[i] = avg[i] + mu[(size_t)smp * M + i] / ((double)samples);
+                rho.resize(M);
+                rc = leo2d_dos(h, avg.data(), 1, M, a, b, 1.0 / (double)geo.g.n, nullptr, rho.data());
+                if (rc != LEO2D_OK) return rc;
+                if (o.geti("verbose_save") == 1) rho = mu;   // verbose save prints the sample moments
+            }
+            rho_all.push_back(rho);
+            if (o.geti("matrix_save") > 0 && q + 1 == jobs.size()) {
+                int32_t nn;
+                int64_t z;
+                int32_t cpx;
+                leo2d_csr_info(h, &nn, &z, &cpx);
+                std::vector<int64_t> rp(nn + 1);
+                std::vector<int32_t> ci((size_t)z);
+                std::vector<double> vv((size_t)z * (cpx ? 2 : 1));
+                rc = leo2d_csr_copy(h, rp.data(), ci.data(), vv.data());
+                if (rc != LEO2D_OK) return rc;
+                std::ofstream f(prefix + "_matrix.dat");
+                f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vv.data(), cpx);
+            }
+        }
+        std::ofstream f(prefix + ".cheb");
+        f << leo2d::cheb_text(geo.deck, jobs, rho_all, M);
+        return LEO2D_OK;
+    });
+}
+
+}  // extern "C"

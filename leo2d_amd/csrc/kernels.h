@@ -1,0 +1,74 @@
+// kernels.h -- launchers for the HIP kernels (kpm.hip, dos.hip, assemble.hip).
+// Internal to libleo2d; the public boundary is include/leo2d.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace leo2d {
+
+// element types of a KPM launch
+enum MatType { MAT_REAL_F64 = 0, MAT_CPX_F64 = 1, MAT_REAL_F32 = 2, MAT_CPX_F32 = 3 };
+enum VecType { VEC_REAL_F64 = 0, VEC_CPX_F64 = 1, VEC_CPX_F32 = 2, VEC_REAL_F32 = 3 };
+
+inline size_t vec_bytes(int vt) { return vt == VEC_CPX_F64 ? 16 : (vt == VEC_REAL_F32 ? 4 : 8); }
+
+// One Chebyshev step over a block of R vectors stored row-major [n][R]:
+//   out = alpha * H * cur - (beta ? prev : 0)        (out may alias prev)
+// and per-vector partial sums for the moment reductions:
+//   part[g][r][0] = sum_i |cur_ir|^2
+//   part[g][r][1] = sum_i Re(conj(out_ir) cur_ir)
+//   part[g][r][2] = sum_i Re(conj(v_ir) cur_ir)      (only when v != nullptr)
+struct StepArgs {
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const void* val;
+    int n;
+    int R;
+    const void* cur;
+    const void* prev;
+    void* out;
+    const void* v;
+    double alpha;
+    int beta;
+    double* part;   // [grid][R][3]
+    int grid;       // number of workgroups (multiple of 8)
+};
+hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStream_t s);
+
+// Deterministic reduction of the partials of one step into moments.
+//   scheme 0 (reference, target starts): mu_j = Re cur[target_r][r] for
+//            2 <= j < M/2, dots used for j >= M/4 (locality.cpp:6002-6034)
+//   scheme 1 (doubling from j = 1): mu_{2j}, mu_{2j+1} from the dots
+//   scheme 2 (reference, vector starts): mu_j = part[..][2] for j < M/2
+struct ReduceArgs {
+    const double* part;
+    int grid;
+    int R;
+    int M;
+    int j;          // step index (0 = first product T1 = H T0)
+    int scheme;
+    const int32_t* targets;  // [R] (scheme 0)
+    const void* cur;         // T_j   (scheme 0, j >= 2)
+    const void* out;         // T_j+1 (scheme 0, j == 0)
+    int vec_type;
+    double* mu;              // [R][M] device
+};
+hipError_t launch_kpm_reduce(const ReduceArgs& a, hipStream_t s);
+
+// start vectors
+hipError_t launch_init_targets(void* x, int vec_type, int n, int R, const int32_t* targets, hipStream_t s);
+hipError_t launch_init_random_phase(void* x, int vec_type, int n, int R, uint64_t seed, int r0, hipStream_t s);
+hipError_t launch_init_from_host_layout(void* x, int vec_type, const double* src, int src_complex, int n, int R,
+                                        hipStream_t s);
+hipError_t launch_convert_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s);
+
+// Jackson-kernel DoS: rho[t][k] = DCT-III(g * mu[t]) / (pi sqrt(a^2 - (E_k - b)^2)) * scale
+hipError_t launch_dos(const double* mu, int T, int M, double a, double b, double scale, double* E, double* rho,
+                      hipStream_t s);
+
+// Gershgorin interval of a CSR matrix: lo = min_i (Re H_ii - R_i), hi = max_i (Re H_ii + R_i)
+hipError_t launch_gershgorin(const int64_t* rp, const int32_t* col, const void* val, int is_complex, int n,
+                             double* lohi /*[2*blocks]*/, int blocks, hipStream_t s);
+
+}  // namespace leo2d

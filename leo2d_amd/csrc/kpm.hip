@@ -1,0 +1,442 @@
+// kpm.hip -- Chebyshev (KPM) recursion over a block of R start vectors.
+//
+// One launch = one Chebyshev step for all R vectors:
+//     T_{j+1} = 2 H T_j - T_{j-1}        (T_1 = H T_0 on the first step)
+// fused with the per-vector partial sums of <T_j|T_j> and Re<T_{j+1}|T_j>
+// (and Re<v|T_j> for the reference's vector-start scheme).  Replaces the
+// reference's per-vector SpMV loop (locality.cpp:5945-6346 calling
+// SpMatrix::vectorMultiply, spmatrix.cpp:270-411): H is streamed once per step
+// for all R vectors instead of once per vector, and the two vector copies and
+// the subtract of the reference loop are fused away (T_{j+1} overwrites T_{j-1}).
+//
+// Layout: vectors row-major [n][R] so that the gather of one column index
+// reads R contiguous entries (R = 64 complex FP64 = 1 KiB per nonzero per
+// wavefront, one fully coalesced load).  Lanes of a wavefront map to vectors;
+// a wavefront owns 64/R rows when R < 64.
+//
+// Scheduling: a fixed grid of G workgroups (G multiple of 8).  Each workgroup
+// owns one contiguous chunk of rows and the workgroup -> chunk map is
+// XCD-aware: hardware dispatches workgroup b to XCD (b mod 8), so logical
+// chunk = (b mod 8) * G/8 + b/8 gives every XCD one contiguous 1/8 of the
+// rows and keeps its L2 working set (the neighbour rows gathered from T_j)
+// local.
+//
+// Reductions are deterministic: fixed per-lane order, fixed shuffle tree,
+// fixed cross-wavefront order, one partial per workgroup and vector, then a
+// fixed-order tree over workgroups in kpm_reduce.  No floating-point atomics.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace leo2d {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+// ---- element arithmetic ------------------------------------------------
+struct cf32 { float x, y; };
+
+template <typename X> struct acc_of;
+template <> struct acc_of<double> { using type = double; };
+template <> struct acc_of<double2> { using type = double2; };
+template <> struct acc_of<float2> { using type = float2; };
+template <> struct acc_of<float> { using type = float; };
+
+__device__ inline void zero(double& a) { a = 0.0; }
+__device__ inline void zero(double2& a) { a.x = 0.0; a.y = 0.0; }
+__device__ inline void zero(float2& a) { a.x = 0.0f; a.y = 0.0f; }
+__device__ inline void zero(float& a) { a = 0.0f; }
+
+// acc += h * x
+__device__ inline void mac(double& acc, double h, double x) { acc = fma(h, x, acc); }
+__device__ inline void mac(double2& acc, double h, double2 x)
+{
+    acc.x = fma(h, x.x, acc.x);
+    acc.y = fma(h, x.y, acc.y);
+}
+__device__ inline void mac(double2& acc, double2 h, double2 x)
+{
+    acc.x = fma(h.x, x.x, acc.x);
+    acc.x = fma(-h.y, x.y, acc.x);
+    acc.y = fma(h.x, x.y, acc.y);
+    acc.y = fma(h.y, x.x, acc.y);
+}
+__device__ inline void mac(float& acc, float h, float x) { acc = fmaf(h, x, acc); }
+__device__ inline void mac(float2& acc, float h, float2 x)
+{
+    acc.x = fmaf(h, x.x, acc.x);
+    acc.y = fmaf(h, x.y, acc.y);
+}
+__device__ inline void mac(float2& acc, float2 h, float2 x)
+{
+    acc.x = fmaf(h.x, x.x, acc.x);
+    acc.x = fmaf(-h.y, x.y, acc.x);
+    acc.y = fmaf(h.x, x.y, acc.y);
+    acc.y = fmaf(h.y, x.x, acc.y);
+}
+
+// out = alpha * acc - prev  (alpha is 1 or 2, exact)
+__device__ inline double comb(double alpha, double acc, double prev) { return alpha * acc - prev; }
+__device__ inline double2 comb(double alpha, double2 acc, double2 prev)
+{
+    double2 o;
+    o.x = alpha * acc.x - prev.x;
+    o.y = alpha * acc.y - prev.y;
+    return o;
+}
+__device__ inline float2 comb(double alpha, float2 acc, float2 prev)
+{
+    float al = (float)alpha;
+    float2 o;
+    o.x = al * acc.x - prev.x;
+    o.y = al * acc.y - prev.y;
+    return o;
+}
+__device__ inline float comb(double alpha, float acc, float prev) { return (float)alpha * acc - prev; }
+
+// FP64 moment terms
+__device__ inline double nrm2(double a) { return a * a; }
+__device__ inline double nrm2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
+__device__ inline double nrm2(float2 a)
+{
+    double x = a.x, y = a.y;
+    return fma(x, x, y * y);
+}
+__device__ inline double nrm2(float a)
+{
+    double x = a;
+    return x * x;
+}
+__device__ inline double redot(double a, double b) { return a * b; }
+__device__ inline double redot(double2 a, double2 b) { return fma(a.x, b.x, a.y * b.y); }
+__device__ inline double redot(float2 a, float2 b)
+{
+    double ax = a.x, ay = a.y, bx = b.x, by = b.y;
+    return fma(ax, bx, ay * by);
+}
+__device__ inline double redot(float a, float b) { return (double)a * (double)b; }
+__device__ inline double re(double a) { return a; }
+__device__ inline double re(double2 a) { return a.x; }
+__device__ inline double re(float2 a) { return a.x; }
+__device__ inline double re(float a) { return a; }
+
+// ---- the fused step --------------------------------------------------------
+// LPR: lanes per row (= min(R, 64)); VPL: vectors per lane (R / 64 when R > 64).
+template <typename HT, typename XT, int LPR, int VPL>
+__global__ __launch_bounds__(kThreads) void kpm_step_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int RPW = 64 / LPR;          // rows per wavefront
+    constexpr int RPI = RPW * kWaves;      // rows per workgroup iteration
+    const int R = LPR * VPL;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sub = lane / LPR;
+    const int r0 = lane % LPR;
+
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);   // XCD-contiguous chunk
+    const int n = a.n;
+    const int per = (n + G - 1) / G;
+    const int chunk = (per + RPI - 1) / RPI * RPI;
+    const int row_begin = lb * chunk;
+    const int row_end = min(n, row_begin + chunk);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const int32_t* __restrict__ col = a.col;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[VPL], d2[VPL], d3[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
+
+    for (int row0 = row_begin + wave * RPW; row0 < row_end; row0 += RPI) {
+        int row = row0 + sub;
+        if (RPW == 1) row = __builtin_amdgcn_readfirstlane(row);
+        if (row >= row_end) continue;
+        AT acc[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) zero(acc[v]);
+        int64_t q = rp[row];
+        const int64_t e = rp[row + 1];
+        for (; q + 4 <= e; q += 4) {
+            const int c0 = col[q], c1 = col[q + 1], c2 = col[q + 2], c3 = col[q + 3];
+            const HT h0 = val[q], h1 = val[q + 1], h2 = val[q + 2], h3 = val[q + 3];
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                const int r = r0 + v * LPR;
+                const XT x0 = cur[(size_t)c0 * R + r];
+                const XT x1 = cur[(size_t)c1 * R + r];
+                const XT x2 = cur[(size_t)c2 * R + r];
+                const XT x3 = cur[(size_t)c3 * R + r];
+                mac(acc[v], h0, x0);
+                mac(acc[v], h1, x1);
+                mac(acc[v], h2, x2);
+                mac(acc[v], h3, x3);
+            }
+        }
+        for (; q < e; ++q) {
+            const int c0 = col[q];
+            const HT h0 = val[q];
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) mac(acc[v], h0, cur[(size_t)c0 * R + r0 + v * LPR]);
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const size_t idx = (size_t)row * R + r0 + v * LPR;
+            const XT c = cur[idx];
+            XT p;
+            if (beta) p = prev[idx];
+            else zero(p);
+            const XT o = comb(alpha, acc[v], p);
+            out[idx] = o;
+            d1[v] += nrm2(c);
+            d2[v] += redot(o, c);
+            if (direct) d3[v] += redot(vin[idx], c);
+        }
+    }
+
+    // fold the row groups of a wavefront (lanes r0, r0+LPR, ...)
+#pragma unroll
+    for (int off = LPR; off < 64; off <<= 1) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            d1[v] += __shfl_xor(d1[v], off);
+            d2[v] += __shfl_xor(d2[v], off);
+            d3[v] += __shfl_xor(d3[v], off);
+        }
+    }
+    __shared__ double red[kWaves][LPR * VPL][3];
+    if (sub == 0) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            red[wave][r0 + v * LPR][0] = d1[v];
+            red[wave][r0 + v * LPR][1] = d2[v];
+            red[wave][r0 + v * LPR][2] = d3[v];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double s = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += red[w][r][k];
+        a.part[((size_t)lb * R + r) * 3 + k] = s;
+    }
+}
+
+template <typename HT, typename XT, int LPR, int VPL>
+hipError_t launch_typed(const StepArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((kpm_step_kernel<HT, XT, LPR, VPL>), dim3(a.grid), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename HT, typename XT>
+hipError_t launch_by_r(const StepArgs& a, hipStream_t s)
+{
+    switch (a.R) {
+    case 1: return launch_typed<HT, XT, 1, 1>(a, s);
+    case 2: return launch_typed<HT, XT, 2, 1>(a, s);
+    case 4: return launch_typed<HT, XT, 4, 1>(a, s);
+    case 8: return launch_typed<HT, XT, 8, 1>(a, s);
+    case 16: return launch_typed<HT, XT, 16, 1>(a, s);
+    case 32: return launch_typed<HT, XT, 32, 1>(a, s);
+    case 64: return launch_typed<HT, XT, 64, 1>(a, s);
+    case 128: return launch_typed<HT, XT, 64, 2>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---- reduction of partials into moments ----------------------------------
+__global__ __launch_bounds__(kThreads) void kpm_reduce_kernel(ReduceArgs a)
+{
+    const int r = blockIdx.x;
+    const int R = a.R;
+    __shared__ double s[3][kThreads];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int g = threadIdx.x; g < a.grid; g += kThreads) {
+        const double* p = a.part + ((size_t)g * R + r) * 3;
+        t0 += p[0];
+        t1 += p[1];
+        t2 += p[2];
+    }
+    s[0][threadIdx.x] = t0;
+    s[1][threadIdx.x] = t1;
+    s[2][threadIdx.x] = t2;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+            s[2][threadIdx.x] += s[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double d1 = s[0][0], d2 = s[1][0], d3 = s[2][0];
+    double* mu = a.mu + (size_t)r * a.M;
+    const int j = a.j, M = a.M;
+    auto load_re = [&](const void* buf, int row) -> double {
+        const size_t idx = (size_t)row * R + r;
+        switch (a.vec_type) {
+        case VEC_REAL_F64: return static_cast<const double*>(buf)[idx];
+        case VEC_CPX_F64: return static_cast<const double2*>(buf)[idx].x;
+        case VEC_CPX_F32: return (double)static_cast<const float2*>(buf)[idx].x;
+        default: return (double)static_cast<const float*>(buf)[idx];
+        }
+    };
+    if (a.scheme == 1) {
+        if (j == 0) { mu[0] = d1; mu[1] = d2; }
+        else if (2 * j + 1 < M) { mu[2 * j] = 2.0 * d1 - mu[0]; mu[2 * j + 1] = 2.0 * d2 - mu[1]; }
+        return;
+    }
+    if (j == 0) {
+        if (a.scheme == 0) { mu[0] = 1.0; mu[1] = load_re(a.out, a.targets[r]); }
+        else { mu[0] = d1; mu[1] = d2; }
+        return;
+    }
+    if (j >= 2 && j < M / 2) {
+        mu[j] = a.scheme == 0 ? load_re(a.cur, a.targets[r]) : d3;
+        if (j >= M / 4) {
+            mu[2 * j] = 2.0 * d1 - mu[0];
+            mu[2 * j + 1] = 2.0 * d2 - mu[1];
+        }
+    }
+}
+
+// ---- start vectors ---------------------------------------------------------
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ void init_random_phase_kernel(void* x, int vec_type, int n, int R, uint64_t seed, int r0)
+{
+    const size_t total = (size_t)n * R;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t i = t / R, r = t % R;
+        uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
+        h = mix64(h ^ ((r0 + r) * 0xD1B54A32D192ED03ULL));
+        h = mix64(h ^ (i * 0x8CB92BA72F3D8DD7ULL));
+        const double phi = (double)(h >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586;
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);
+        else static_cast<float2*>(x)[t] = make_float2((float)cs, (float)sn);
+    }
+}
+
+__global__ void init_targets_kernel(void* x, int vec_type, int n, int R, const int32_t* targets)
+{
+    const size_t total = (size_t)n * R;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / R), r = (int)(t % R);
+        const double v = (targets[r] == i) ? 1.0 : 0.0;
+        switch (vec_type) {
+        case VEC_REAL_F64: static_cast<double*>(x)[t] = v; break;
+        case VEC_CPX_F64: static_cast<double2*>(x)[t] = make_double2(v, 0.0); break;
+        case VEC_CPX_F32: static_cast<float2*>(x)[t] = make_float2((float)v, 0.0f); break;
+        default: static_cast<float*>(x)[t] = (float)v; break;
+        }
+    }
+}
+
+// src: host-style [R][n] real or complex (interleaved) FP64, already on device
+__global__ void init_from_host_layout_kernel(void* x, int vec_type, const double* src, int src_complex, int n, int R)
+{
+    const size_t total = (size_t)n * R;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = t / R, r = t % R;
+        double re_v, im_v;
+        if (src_complex) {
+            re_v = src[2 * (r * n + i)];
+            im_v = src[2 * (r * n + i) + 1];
+        } else {
+            re_v = src[r * n + i];
+            im_v = 0.0;
+        }
+        switch (vec_type) {
+        case VEC_REAL_F64: static_cast<double*>(x)[t] = re_v; break;
+        case VEC_CPX_F64: static_cast<double2*>(x)[t] = make_double2(re_v, im_v); break;
+        case VEC_CPX_F32: static_cast<float2*>(x)[t] = make_float2((float)re_v, (float)im_v); break;
+        default: static_cast<float*>(x)[t] = (float)re_v; break;
+        }
+    }
+}
+
+__global__ void convert_kernel(const double* in, float* out, int64_t count)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (float)in[t];
+}
+
+inline dim3 stride_grid(size_t total)
+{
+    size_t g = (total + kThreads - 1) / kThreads;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return dim3((unsigned)g);
+}
+
+}  // namespace
+
+hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStream_t s)
+{
+    if (a.grid % 8 != 0 || a.grid <= 0) return hipErrorInvalidValue;
+    if (mat_type == MAT_REAL_F64 && vec_type == VEC_REAL_F64) return launch_by_r<double, double>(a, s);
+    if (mat_type == MAT_REAL_F64 && vec_type == VEC_CPX_F64) return launch_by_r<double, double2>(a, s);
+    if (mat_type == MAT_CPX_F64 && vec_type == VEC_CPX_F64) return launch_by_r<double2, double2>(a, s);
+    if (mat_type == MAT_REAL_F32 && vec_type == VEC_CPX_F32) return launch_by_r<float, float2>(a, s);
+    if (mat_type == MAT_CPX_F32 && vec_type == VEC_CPX_F32) return launch_by_r<float2, float2>(a, s);
+    if (mat_type == MAT_REAL_F32 && vec_type == VEC_REAL_F32) return launch_by_r<float, float>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_kpm_reduce(const ReduceArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(kpm_reduce_kernel, dim3(a.R), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_targets(void* x, int vec_type, int n, int R, const int32_t* targets, hipStream_t s)
+{
+    hipLaunchKernelGGL(init_targets_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x, vec_type, n, R,
+                       targets);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_random_phase(void* x, int vec_type, int n, int R, uint64_t seed, int r0, hipStream_t s)
+{
+    if (vec_type != VEC_CPX_F64 && vec_type != VEC_CPX_F32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(init_random_phase_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x, vec_type, n,
+                       R, seed, r0);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_from_host_layout(void* x, int vec_type, const double* src, int src_complex, int n, int R,
+                                        hipStream_t s)
+{
+    hipLaunchKernelGGL(init_from_host_layout_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x, vec_type,
+                       src, src_complex, n, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s)
+{
+    hipLaunchKernelGGL(convert_kernel, stride_grid((size_t)count), dim3(kThreads), 0, s, in, out, count);
+    return hipGetLastError();
+}
+
+}  // namespace leo2d

@@ -1,0 +1,146 @@
+/*
+ * leo2d_oracle.h -- CPU FP64 restatement of LEO2D's KPM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (leo2d_amd) never links, imports or calls it.
+ *
+ * Every function restates one piece of the reference and cites it
+ * (paths relative to the reference tree).  Floating-point expressions keep the
+ * reference's operation order and are compiled without FMA contraction, so
+ * positions, pair lists and real Hamiltonian values reproduce the reference's
+ * arithmetic bit for bit on an x86-64 host.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   couplings      : against the reference's own graphene.cpp built into
+ *                    oracle/_ref, plus closed-form known answers.
+ *   sheet pairs    : against the reference's own Sheet::getIntraPairs built
+ *                    into oracle/_ref (flake and periodic decks).
+ *   inter pairs    : brute-force all-pairs scan restricted to the same window
+ *                    rule (Hstruct is not buildable here: it needs fftw3.h).
+ *   enumeration    : tests/tblg_test1.test target indices 60,61,175,176 and
+ *                    the commensurate-cell count 4(N^2+NM+M^2).
+ *   assembly       : Hermiticity, k=0 complex == real, dense consistency.
+ *   KPM            : dense eigendecomposition of small H, mu0/mu1 closed
+ *                    forms, doubling identity vs direct moments, and the
+ *                    reference's SpMatrix::vectorMultiply from oracle/_ref.
+ *   DoS            : O(M^2) direct DCT-III vs closed form, the Chebyshev-Gauss
+ *                    normalisation invariant verified on the golden rows of
+ *                    tests/tblg_test{1,2}.test.
+ *   random phases  : no reference counterpart (new input form); its integer
+ *                    hash is pinned against an independent Python restatement.
+ *   Gaussian trace vectors (std::default_random_engine): parity unpinned.
+ */
+#ifndef LEO2D_ORACLE_H
+#define LEO2D_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_SHEETS 8
+
+/* materials */
+#define ORC_MAT_GRAPHENE 0        /* src/materials/graphene.cpp */
+#define ORC_MAT_GRAPHENE_SIMPLE 1 /* NN t=-2.7 eV, t_perp = 0.48 exp(-(r-3.35)/0.45), r<6 A */
+
+typedef struct {
+    int mat;
+    int min_shape[2], max_shape[2];
+    double max_R;
+    int boundary_condition;
+    double supercell[2][2];       /* sheet-local frame, rows = A1, A2 */
+    int supercell_stride[2][2];
+    double angle;                 /* radians */
+    double height;
+    int global_shifts_on;
+    double global_shift[2];
+} orc_sheet_desc;
+
+typedef struct {
+    int num_sheets;
+    orc_sheet_desc sheet[ORC_MAX_SHEETS];
+    int boundary_condition;
+    int sc_search_size;
+    double supercell[2][2];       /* global supercell (sheet 0's), opts "supercell" */
+} orc_geom_desc;
+
+typedef struct orc_geom orc_geom;
+
+/* geometry ------------------------------------------------------------ */
+orc_geom* orc_geom_build(const orc_geom_desc* d);
+void orc_geom_free(orc_geom* g);
+int orc_geom_n(const orc_geom* g);
+int orc_geom_sheet_count(const orc_geom* g, int s);
+void orc_geom_arrays(const orc_geom* g, int32_t* index_to_grid, double* index_to_pos);
+int orc_grid_to_index(const orc_geom* g, int i, int j, int l, int s);
+int orc_target_center(const orc_geom* g, int s);
+
+/* couplings ------------------------------------------------------------ */
+double orc_intralayer(int mat, int orb_row, int orb_col, int v0, int v1);
+double orc_interlayer(int mat, int orb_row, int orb_col, double dx, double dy, double dz,
+                      double theta_row, double theta_col);
+
+/* pair lists: produced in reference order (per row: intra pairs of the
+ * sheet, then inter pairs), ready for assembly. */
+typedef struct orc_pairs orc_pairs;
+orc_pairs* orc_pairs_build(const orc_geom* g);
+void orc_pairs_free(orc_pairs* p);
+int64_t orc_pairs_count(const orc_pairs* p, int which /*0 intra, 1 inter*/);
+void orc_pairs_copy(const orc_pairs* p, int which, int32_t* ij /*[2P]*/, int32_t* sc /*[2P]*/,
+                    double* t /*[P] (intra only, may be NULL)*/);
+
+/* assembly -------------------------------------------------------------- */
+typedef struct {
+    double shift[ORC_MAX_SHEETS][2];   /* per sheet, fractional lattice units */
+    double energy_rescale, energy_shift;
+    int elec_on; double E;
+    int mag_on;  double B;
+    double k[2];
+    int complex_matrix;
+    int uniform_strain; double u[2][2];
+    int pruned;                         /* 0 reference-exact slots, 1 drop zero slots */
+} orc_job;
+
+typedef struct {
+    int n;
+    int64_t nnz;
+    int64_t* row_ptr;   /* [n+1] */
+    int32_t* col;       /* [nnz] */
+    double* val;        /* [nnz] real or [2 nnz] complex interleaved */
+    int is_complex;
+} orc_csr;
+
+void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos /*[3n]*/);
+int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_csr* out);
+void orc_csr_free(orc_csr* h);
+
+/* KPM ------------------------------------------------------------------- */
+#define ORC_START_TARGET 0      /* e_t, locality.cpp:5945 computeDosKPM */
+#define ORC_START_VECTORS 1     /* given start vectors, locality.cpp:6138 computeDosTraceKPM */
+
+int orc_kpm(int n, const int64_t* row_ptr, const int32_t* col, const double* val, int is_complex,
+            int M, int R, int start, const int32_t* targets,
+            const double* vecs, int vecs_complex, int doubling_all, int nthreads,
+            double* mu /*[R][M]*/);
+
+/* one faithful SpMV y = alpha*H*x + beta*y (spmatrix.cpp:346-360,395-410) */
+void orc_spmv(int n, const int64_t* row_ptr, const int32_t* col, const double* val, int is_complex,
+              const double* x, double* y, double alpha, double beta);
+
+/* random-phase start vectors: v[r][i] = exp(i*phi(seed, r, i)) */
+uint64_t orc_phase_hash(uint64_t seed, uint64_t r, uint64_t i);
+void orc_random_phase(int n, int R, uint64_t seed, int r0, double* v /*[R][2n]*/);
+
+/* DoS ------------------------------------------------------------------- */
+void orc_jackson(int M, double* g);
+void orc_energies(int M, double a, double b, double* E);
+void orc_dos(int M, const double* mu, double a, double b, double* rho);
+void orc_dos_trace(int M, int S, const double* mu, double a, double b, double scale, double* rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,121 @@
+// ref_harness.cpp -- C-ABI wrappers around the reference's own compiled sources
+// (graphene.cpp, sheet.cpp, spmatrix.cpp), built by `make -C oracle ref` into
+// oracle/_ref/libref.so.  TEST INFRASTRUCTURE ONLY: it lets tests pin the
+// oracle's restatement against the reference's code on this host.  No
+// reference source is copied here; this file only calls the reference's
+// public functions.
+#include <cstring>
+#include <vector>
+
+#include "geom/sdata.h"
+#include "geom/sheet.h"
+#include "materials/graphene.h"
+#include "matrix/spmatrix.h"
+#include "params/job_params.h"
+
+extern "C" {
+
+// Coupling::Intralayer::graphene (graphene.cpp:13)
+double ref_graphene_intra(int orb_row, int orb_col, int v0, int v1)
+{
+    std::array<int, 2> v = {{v0, v1}};
+    return Coupling::Intralayer::graphene(Graphene::orbital(orb_row), Graphene::orbital(orb_col), v);
+}
+
+// Coupling::Interlayer::C_to_C (graphene.cpp:91)
+double ref_graphene_inter(int orb_row, int orb_col, double x, double y, double z, double th_row, double th_col)
+{
+    std::array<double, 3> v = {{x, y, z}};
+    return Coupling::Interlayer::C_to_C(Graphene::orbital(orb_row), Graphene::orbital(orb_col), v, th_row, th_col);
+}
+
+// Sheet::getIntraPairs (sheet.cpp:624) for one graphene sheet.  Returns the
+// number of pairs; when the output pointers are non-null and cap suffices the
+// pair list is copied out (ij[2P], sc[2P], t[P]).
+long ref_sheet_intra_pairs(int min0, int min1, int max0, int max1, double max_R, int bc,
+                           const double* supercell /*[4]*/, const int* stride /*[4]*/, int sc_search_size,
+                           int global_shifts_on, const double* gshift /*[2]*/, int start_index,
+                           int* ij, int* sc, double* t, long cap, int* n_orbitals_out)
+{
+    std::vector<int> mn = {min0, min1, min0};
+    std::vector<int> mx = {max0, max1, max0};
+    Job_params opts;
+    opts.setParam("boundary_condition", bc);
+    opts.setParam("sc_search_size", sc_search_size);
+    opts.setParam("global_shifts_on", global_shifts_on);
+    std::vector<std::vector<double> > gs(1, std::vector<double>(3, 0.0));
+    gs[0][0] = gshift ? gshift[0] : 0.0;
+    gs[0][1] = gshift ? gshift[1] : 0.0;
+    opts.setParam("global_shifts", gs);
+    Sdata sd(Materials::Mat::Graphene, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
+    sd.sheet_index = 0;
+    sd.opts = opts;
+    if (bc == 1) {
+        sd.supercell = {{supercell[0], supercell[1]}, {supercell[2], supercell[3]}};
+        sd.supercell_stride = {{stride[0], stride[1]}, {stride[2], stride[3]}};
+    }
+    Sheet sheet(sd);
+    if (n_orbitals_out) *n_orbitals_out = sheet.getMaxIndex();
+    std::vector<int> ai, aj;
+    std::vector<double> at;
+    std::vector<std::vector<int> > scv;
+    sheet.getIntraPairs(ai, aj, at, scv, opts, start_index);
+    long np = (long)ai.size();
+    if (ij && np <= cap) {
+        for (long q = 0; q < np; ++q) {
+            ij[2 * q] = ai[q];
+            ij[2 * q + 1] = aj[q];
+            if (sc) { sc[2 * q] = scv[q][0]; sc[2 * q + 1] = scv[q][1]; }
+            if (t) t[q] = at[q];
+        }
+    }
+    return np;
+}
+
+// Sheet::posAtomIndex (sheet.cpp:458) for every orbital of the sheet above.
+int ref_sheet_positions(int min0, int min1, int max0, int max1, double max_R, int bc,
+                        const double* supercell, const int* stride, double* pos /*[3n]*/, int cap)
+{
+    std::vector<int> mn = {min0, min1, min0};
+    std::vector<int> mx = {max0, max1, max0};
+    Job_params opts;
+    opts.setParam("boundary_condition", bc);
+    Sdata sd(Materials::Mat::Graphene, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
+    sd.sheet_index = 0;
+    sd.opts = opts;
+    if (bc == 1) {
+        sd.supercell = {{supercell[0], supercell[1]}, {supercell[2], supercell[3]}};
+        sd.supercell_stride = {{stride[0], stride[1]}, {stride[2], stride[3]}};
+    }
+    Sheet sheet(sd);
+    int n = sheet.getMaxIndex();
+    if (pos && n <= cap)
+        for (int k = 0; k < n; ++k)
+            for (int d = 0; d < 3; ++d) pos[3 * k + d] = sheet.posAtomIndex(k, d);
+    return n;
+}
+
+// SpMatrix::vectorMultiply (spmatrix.cpp:270, 365): y = alpha*H*x + beta*y.
+void ref_spmv(int n, int nnz, const int* row_ptr, const int* col, const double* val, int is_complex,
+              const double* x, double* y, double alpha, double beta)
+{
+    SpMatrix H;
+    H.setup(nnz, n, n);
+    int* ci = H.allocColIndx();
+    int* rp = H.allocRowPtr();
+    std::memcpy(rp, row_ptr, sizeof(int) * (n + 1));
+    std::memcpy(ci, col, sizeof(int) * nnz);
+    if (!is_complex) {
+        double* v = H.allocRealVal();
+        std::memcpy(v, val, sizeof(double) * nnz);
+        H.vectorMultiply(const_cast<double*>(x), y, alpha, beta);
+    } else {
+        std::complex<double>* v = H.allocCpxVal();
+        std::memcpy(v, val, sizeof(double) * 2 * nnz);
+        H.vectorMultiply(reinterpret_cast<std::complex<double>*>(const_cast<double*>(x)),
+                         reinterpret_cast<std::complex<double>*>(y), std::complex<double>(alpha),
+                         std::complex<double>(beta));
+    }
+}
+
+}  // extern "C"

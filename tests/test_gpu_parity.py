@@ -1,0 +1,229 @@
+"""GPU parity through the C-ABI against the CPU oracle.
+
+Bars (stated per test):
+  * pair lists and CSR pattern: bit-exact (integer work);
+  * Hamiltonian values: |dv| <= 1e-12 |v| + 1e-14 max|H| (the GPU's atan2/cos/exp
+    differ from glibc by a few ulp; everything else is IEEE-exact);
+  * FP64 moments: |dmu| <= 1e-12 mu_0 (different summation order only);
+  * mixed precision (FP32 recursion): |dmu| <= 2e-4 mu_0, max |dDoS| reported;
+  * DoS: |drho| <= 1e-11 (1 + |rho|).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_path
+
+import leo2d_amd as L
+
+pytestmark = pytest.mark.gpu
+
+DECKS = os.path.join(ROOT, "tests", "decks")
+PARITY_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_gshift_strain.in",
+                "tblg_simple_flake.in", "tblg_c0_twin.in"]
+
+
+def deck_text(name):
+    p = golden_path(name)
+    return open(p).read() if os.path.exists(p) else open(os.path.join(DECKS, name)).read()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = L.Device(0)
+    yield d
+    d.close()
+
+
+def _oracle(orc, text):
+    od = orc.parse_deck(text)
+    orc.setup_supercell(od)
+    og = orc.Geometry(orc.geom_desc(od))
+    ojobs = orc.build_jobs(od, orc.target_list(od, og))
+    return od, og, ojobs
+
+
+def _setup(dev, text):
+    deck = L.Deck(text)
+    geom = L.Geometry(deck)
+    dev.set_geometry(geom)
+    counts = dev.pairs_build()
+    return deck, geom, counts
+
+
+@pytest.mark.parametrize("name", PARITY_DECKS)
+def test_pairs_bit_exact(dev, oracle_mod, name):
+    text = deck_text(name)
+    print("case", name, flush=True)
+    deck, geom, (ni, nx, ns) = _setup(dev, text)
+    od, og, _ = _oracle(oracle_mod, text)
+    for which, cnt in ((0, ni), (1, nx)):
+        ij, sc, t = dev.pairs_copy(which, cnt)
+        oij, osc, ot = og.pair_arrays(which)
+        assert np.array_equal(ij, oij), which
+        assert np.array_equal(sc, osc), which
+        if which == 0:
+            assert np.array_equal(t, ot)
+
+
+def _value_ok(v, ov):
+    tol = 1e-12 * np.abs(ov) + 1e-14 * max(1e-300, float(np.max(np.abs(ov))) if len(ov) else 1.0)
+    return np.all(np.abs(v - ov) <= tol)
+
+
+@pytest.mark.parametrize("name", PARITY_DECKS)
+@pytest.mark.parametrize("pattern", [L.PATTERN_REFERENCE, L.PATTERN_PRUNED])
+def test_assembly_parity(dev, oracle_mod, name, pattern):
+    text = deck_text(name)
+    print("case", name, pattern, flush=True)
+    deck, geom, _ = _setup(dev, text)
+    od, og, ojobs = _oracle(oracle_mod, text)
+    jobs = geom.jobs()
+    for q in sorted({0, len(jobs) - 1}):
+        dev.assemble(jobs[q], pattern)
+        rp, ci, v = dev.csr_copy()
+        orp, oci, ov = og.assemble(oracle_mod.make_job(od, ojobs[q], pruned=pattern))
+        assert np.array_equal(rp, orp)
+        assert np.array_equal(ci, oci)
+        assert np.iscomplexobj(v) == np.iscomplexobj(ov)
+        assert _value_ok(v, ov)
+
+
+@pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_gshift_strain.in"])
+def test_kpm_targets_reference_scheme(dev, oracle_mod, name):
+    text = deck_text(name)
+    print("case", name, flush=True)
+    deck, geom, _ = _setup(dev, text)
+    od, og, ojobs = _oracle(oracle_mod, text)
+    job = geom.jobs()[0]
+    dev.assemble(job, L.PATTERN_REFERENCE)
+    rp, ci, v = dev.csr_copy()
+    M = od.opts["poly_order"]
+    targets = list(job.targets[: job.num_targets])
+    mu = dev.kpm(M, targets=targets)
+    omu = oracle_mod.kpm(rp, ci, v, M, targets=targets)
+    assert np.max(np.abs(mu - omu)) <= 1e-12
+    # scheme variants and launch shapes give the same moments
+    mu2 = dev.kpm(M, targets=targets, scheme=L.SCHEME_DOUBLING)
+    omu2 = oracle_mod.kpm(rp, ci, v, M, targets=targets, doubling_all=True)
+    assert np.max(np.abs(mu2 - omu2)) <= 1e-12
+    mu3 = dev.kpm(M, targets=targets, grid=16, block=8)
+    assert np.max(np.abs(mu3 - mu)) <= 1e-13
+
+
+@pytest.mark.parametrize("R", [1, 3, 16, 64, 100])
+def test_kpm_random_phase(dev, oracle_mod, R):
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 64
+    seed = 0
+    mu = dev.kpm(M, random_phase=(R, seed), scheme=L.SCHEME_DOUBLING)
+    vecs = oracle_mod.random_phase(geom.n, R, seed)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    assert np.max(np.abs(mu[:, 0] - geom.n)) <= 1e-9
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+    # reference scheme keeps the start vectors: mu_j = Re<v|T_j v> directly
+    mur = dev.kpm(M, random_phase=(R, seed), scheme=L.SCHEME_REFERENCE)
+    omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+    assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+
+
+def test_kpm_real_vectors_trace(dev, oracle_mod):
+    text = deck_text("tblg_test1.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[1]
+    dev.assemble(job, L.PATTERN_REFERENCE)
+    rp, ci, v = dev.csr_copy()
+    rng = np.random.default_rng(2)
+    V = rng.standard_normal((5, geom.n))
+    mu = dev.kpm(200, vectors=V)
+    omu = oracle_mod.kpm(rp, ci, v, 200, vecs=V)
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+    Vc = V + 1j * rng.standard_normal((5, geom.n))
+    mu = dev.kpm(200, vectors=Vc)
+    omu = oracle_mod.kpm(rp, ci, v, 200, vecs=Vc)
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+
+
+def test_mixed_precision_against_fp64(dev, oracle_mod):
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    dev.assemble(job, L.PATTERN_PRUNED)
+    M = 500
+    mu64 = dev.kpm(M, random_phase=(16, 0), scheme=L.SCHEME_DOUBLING)
+    mu32 = dev.kpm(M, random_phase=(16, 0), scheme=L.SCHEME_DOUBLING, precision=L.PREC_MIXED)
+    n = geom.n
+    assert np.max(np.abs(mu32 - mu64)) <= 2e-4 * n
+    a, b = job.energy_rescale, job.energy_shift
+    _, r64 = dev.dos(mu64.mean(axis=0) / n, a, b)
+    _, r32 = dev.dos(mu32.mean(axis=0) / n, a, b)
+    print("mixed max|dDoS| states/eV/atom:", float(np.max(np.abs(r64 - r32))))
+    assert np.max(np.abs(r64 - r32)) < 1e-4
+
+
+def test_dos_parity(dev, oracle_mod):
+    rng = np.random.default_rng(4)
+    for M in (200, 512, 2048):
+        mu = rng.standard_normal((3, M)) * np.exp(-np.arange(M) / M)
+        mu[:, 0] = 1.0
+        E, rho = dev.dos(mu, 13.0, 0.25)
+        orho = oracle_mod.dos(mu, 13.0, 0.25)
+        assert np.max(np.abs(rho - orho) / (1 + np.abs(orho))) <= 1e-11
+        assert np.max(np.abs(E - oracle_mod.energies(M, 13.0, 0.25))) <= 1e-13
+
+
+def _cheb_numbers(text):
+    import re
+
+    return [float(x) for x in re.findall(r"-?\d+\.\d+", text)], re.sub(r"-?\d+\.\d+", "#", text)
+
+
+@pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in"])
+def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
+    text = deck_text(name)
+    prefix = str(tmp_path / "out")
+    dev.run_deck(text, prefix)
+    got = open(prefix + ".cheb").read()
+    od, og, jobs, results = oracle_mod.run_deck(text)
+    want = oracle_mod.cheb_text(od, results)
+    gn, gs = _cheb_numbers(got)
+    wn, ws = _cheb_numbers(want)
+    assert gs == ws                                   # identical layout and integers
+    assert np.max(np.abs(np.array(gn) - np.array(wn))) <= 1.0000001e-6   # 6-decimal rounding
+    if od.opts["matrix_save"] > 0:
+        rp, ci, v = og.assemble(oracle_mod.make_job(od, jobs[-1]))
+        assert os.path.exists(prefix + "_matrix.dat")
+        gl = open(prefix + "_matrix.dat").read().splitlines()
+        wl = oracle_mod.matrix_dat_text(rp, ci, v).splitlines()
+        assert len(gl) == len(wl)
+        for a, b in zip(gl, wl):
+            ga = [float(x) for x in a.split(",")]
+            wa = [float(x) for x in b.split(",")]
+            assert ga[:2] == wa[:2]
+            assert np.allclose(ga[2:], wa[2:], rtol=1e-5, atol=1e-6)
+
+
+def test_spectrum_and_nan_errors(dev):
+    n = 50
+    rp = np.arange(n + 1, dtype=np.int64)
+    ci = np.arange(n, dtype=np.int32)
+    v = np.linspace(-1.4, 1.4, n)          # spectrum outside [-1, 1]
+    dev.csr_upload(rp, ci, v)
+    with pytest.raises(L.Leo2dError) as e:
+        dev.kpm(64, targets=[0, 1])
+    assert e.value.code == L.ERR_SPECTRUM
+    v[3] = np.nan
+    with pytest.raises(L.Leo2dError) as e:
+        dev.csr_upload(rp, ci, v)
+    assert e.value.code == L.ERR_NAN
+    with pytest.raises(L.Leo2dError) as e:
+        dev.kpm(62, targets=[0])
+    assert e.value.code == L.ERR_ARG
+    with pytest.raises(L.Leo2dError) as e:
+        dev.kpm(64, targets=[n + 5])
+    assert e.value.code == L.ERR_ARG
