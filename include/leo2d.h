@@ -158,6 +158,9 @@ typedef struct {
     int32_t block;          /* vectors per device block (power of two <= 128): 0 = auto (<= 64) */
     int32_t grid;           /* workgroups per step launch (multiple of 8): 0 = auto */
     int32_t time_steps;     /* 1: time every fused step launch with HIP events */
+    int32_t natural_order;  /* 1: recurse in the reference row order; 0: spatial (Morton) order
+                               when the matrix came from leo2d_assemble (moments are unchanged
+                               up to summation order) */
 } leo2d_kpm_params;
 
 typedef struct {

@@ -81,6 +81,7 @@ class KpmParams(C.Structure):
         ("targets", C.POINTER(C.c_int32)), ("vectors", C.POINTER(C.c_double)),
         ("vectors_complex", C.c_int32), ("seed", C.c_uint64), ("precision", C.c_int32),
         ("scheme", C.c_int32), ("block", C.c_int32), ("grid", C.c_int32), ("time_steps", C.c_int32),
+        ("natural_order", C.c_int32),
     ]
 
 
@@ -282,7 +283,7 @@ class Device:
         return lo.value, hi.value
 
     def kpm(self, M, targets=None, vectors=None, random_phase=None, precision=PREC_FP64,
-            scheme=SCHEME_REFERENCE, block=0, grid=0, time_steps=False):
+            scheme=SCHEME_REFERENCE, block=0, grid=0, time_steps=False, natural_order=False):
         """Chebyshev moments mu[R][M] on the device Hamiltonian."""
         p = KpmParams()
         p.M = M
@@ -291,6 +292,7 @@ class Device:
         p.block = block
         p.grid = grid
         p.time_steps = int(time_steps)
+        p.natural_order = int(natural_order)
         keep = []
         if targets is not None:
             t = np.ascontiguousarray(targets, dtype=np.int32)

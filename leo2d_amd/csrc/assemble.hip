@@ -25,6 +25,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "assemble.h"
+#include "kernels.h"
 
 namespace leo2d {
 namespace {
@@ -442,6 +443,8 @@ hipError_t scan_counts(const int64_t* cnt, int64_t* out, int n, hipStream_t s)
 }
 
 }  // namespace
+
+hipError_t scan_exclusive_i64(const int64_t* cnt, int64_t* out, int n, hipStream_t s) { return scan_counts(cnt, out, n, s); }
 
 hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total, int64_t* pair_rp, hipStream_t s)
 {

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <random>
@@ -71,6 +72,13 @@ struct NanError : std::runtime_error {
     explicit NanError(const std::string& m) : std::runtime_error(m) {}
 };
 
+// step-kernel variant for R >= 64 (LEO2D_KPM_VARIANT overrides the default)
+int kpm_variant()
+{
+    const char* e = std::getenv("LEO2D_KPM_VARIANT");
+    return e ? std::atoi(e) : 1;
+}
+
 int next_pow2(int x)
 {
     int p = 1;
@@ -101,6 +109,12 @@ struct leo2d_handle {
     int is_complex = 0;
     DevBuf rp, col, val, val32, i2pos;
     bool val32_valid = false;
+    // spatial order used inside the recursion (perm[orig] = new row)
+    bool csr_geom_order = false;       // device H rows are geometry orbitals
+    std::vector<int32_t> perm_host;
+    DevBuf perm, iperm;
+    bool pcsr_valid = false, pval32_valid = false;
+    DevBuf prp, pcol_k, pval, pval32;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
 
@@ -108,7 +122,7 @@ struct leo2d_handle {
     {
         for (auto& b : grids) b.release();
         for (DevBuf* b : {&i2g, &i2p, &lpos, &cnt_intra, &cnt_tmp, &pair_rp, &pcol, &psc, &nslots, &slot_rp, &rp, &col,
-                          &val, &val32, &i2pos})
+                          &val, &val32, &i2pos, &perm, &iperm, &prp, &pcol_k, &pval, &pval32})
             b->release();
         for (auto e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -208,6 +222,67 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->is_complex = cpx;
     h->has_csr = true;
     h->val32_valid = false;
+    h->pcsr_valid = false;
+    h->pval32_valid = false;
+}
+
+// Morton (Z-order) rank of every orbital by in-plane position on a 1.42 A
+// grid, both sheets interleaved: rows that are neighbours in the lattice end
+// up close in memory, so a window of rows and the T_j rows it gathers fit in
+// one XCD's L2.
+std::vector<int32_t> spatial_order(const leo2d::Geom& g)
+{
+    const int n = g.n;
+    double xmin = 1e300, ymin = 1e300;
+    for (int i = 0; i < n; ++i) {
+        xmin = std::min(xmin, g.i2p[3 * (size_t)i]);
+        ymin = std::min(ymin, g.i2p[3 * (size_t)i + 1]);
+    }
+    const double cell = 1.42;
+    auto spread = [](uint64_t v) {
+        v &= 0xffffffffULL;
+        v = (v | (v << 16)) & 0x0000FFFF0000FFFFULL;
+        v = (v | (v << 8)) & 0x00FF00FF00FF00FFULL;
+        v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+        v = (v | (v << 2)) & 0x3333333333333333ULL;
+        v = (v | (v << 1)) & 0x5555555555555555ULL;
+        return v;
+    };
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; ++i) {
+        const uint64_t ix = (uint64_t)std::max(0.0, std::floor((g.i2p[3 * (size_t)i] - xmin) / cell));
+        const uint64_t iy = (uint64_t)std::max(0.0, std::floor((g.i2p[3 * (size_t)i + 1] - ymin) / cell));
+        key[i] = spread(ix) | (spread(iy) << 1);
+    }
+    std::vector<int32_t> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    std::vector<int32_t> perm(n);
+    for (int q = 0; q < n; ++q) perm[order[q]] = q;
+    return perm;
+}
+
+// H' = P H P^T on the device (cached until the Hamiltonian changes)
+void build_permuted(leo2d_handle* h)
+{
+    if (h->pcsr_valid) return;
+    const int n = h->n;
+    hipStream_t s = h->stream;
+    DevBuf len;
+    dalloc(len, sizeof(int64_t) * n, "permuted lengths");
+    dalloc(h->prp, sizeof(int64_t) * (n + 1), "permuted rows");
+    check(leo2d::launch_perm_len((const int64_t*)h->rp.p, (const int32_t*)h->iperm.p, n, (int64_t*)len.p, s), "perm");
+    check(leo2d::scan_exclusive_i64((const int64_t*)len.p, (int64_t*)h->prp.p, n, s), "perm scan");
+    dalloc(h->pcol_k, sizeof(int32_t) * (size_t)h->nnz, "permuted columns");
+    const int vb = h->is_complex ? 16 : 8;
+    dalloc(h->pval, (size_t)vb * h->nnz, "permuted values");
+    check(leo2d::launch_perm_fill((const int64_t*)h->rp.p, (const int32_t*)h->col.p, h->val.p, vb,
+                                  (const int32_t*)h->iperm.p, (const int32_t*)h->perm.p, n, (const int64_t*)h->prp.p,
+                                  (int32_t*)h->pcol_k.p, h->pval.p, s),
+          "perm fill");
+    check(hipStreamSynchronize(s), "perm");
+    h->pcsr_valid = true;
+    h->pval32_valid = false;
 }
 
 }  // namespace
@@ -417,6 +492,11 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
         d.i2p = static_cast<const double*>(h->i2p.p);
         d.lpos = static_cast<const double*>(h->lpos.p);
         h->gdev = d;
+        h->perm_host = spatial_order(g);
+        std::vector<int32_t> ip(g.n);
+        for (int i = 0; i < g.n; ++i) ip[h->perm_host[i]] = i;
+        to_device(h->perm, h->perm_host.data(), sizeof(int32_t) * g.n, s, "order upload");
+        to_device(h->iperm, ip.data(), sizeof(int32_t) * g.n, s, "order upload");
         check(hipStreamSynchronize(s), "geometry upload");
         h->has_geom = true;
         return LEO2D_OK;
@@ -580,6 +660,7 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
             set_csr_owned(h, n, kept, cpx);
         }
         check(hipStreamSynchronize(s), "assemble");
+        h->csr_geom_order = true;
         if (nnz_out) *nnz_out = h->nnz;
         return LEO2D_OK;
     });
@@ -604,6 +685,7 @@ int leo2d_csr_upload(leo2d_handle* h, int32_t n, int64_t nnz, const int64_t* row
         to_device(h->val, val, sizeof(double) * w * (size_t)nnz, s, "values");
         check(hipStreamSynchronize(s), "upload");
         set_csr_owned(h, n, nnz, is_complex ? 1 : 0);
+        h->csr_geom_order = false;
         return LEO2D_OK;
     });
 }
@@ -695,16 +777,31 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             vec_type = real_vec ? leo2d::VEC_REAL_F32 : leo2d::VEC_CPX_F32;
         }
         hipStream_t s = h->stream;
+        const bool spatial = !p->natural_order && h->csr_geom_order && h->has_geom && h->gdev.n == n;
+        const int32_t* iperm = nullptr;
+        const int64_t* drp = (const int64_t*)h->rp.p;
+        const int32_t* dcol = (const int32_t*)h->col.p;
         const void* dval = h->val.p;
+        DevBuf* v32 = &h->val32;
+        bool* v32_valid = &h->val32_valid;
+        if (spatial) {
+            build_permuted(h);
+            iperm = (const int32_t*)h->iperm.p;
+            drp = (const int64_t*)h->prp.p;
+            dcol = (const int32_t*)h->pcol_k.p;
+            dval = h->pval.p;
+            v32 = &h->pval32;
+            v32_valid = &h->pval32_valid;
+        }
         if (mixed) {
             const size_t count = (size_t)h->nnz * (h->is_complex ? 2 : 1);
-            if (!h->val32_valid) {
-                dalloc(h->val32, sizeof(float) * count, "fp32 matrix");
-                check(leo2d::launch_convert_f64_to_f32((const double*)h->val.p, (float*)h->val32.p, (int64_t)count, s),
+            if (!*v32_valid) {
+                dalloc(*v32, sizeof(float) * count, "fp32 matrix");
+                check(leo2d::launch_convert_f64_to_f32((const double*)dval, (float*)v32->p, (int64_t)count, s),
                       "fp32 matrix");
-                h->val32_valid = true;
+                *v32_valid = true;
             }
-            dval = h->val32.p;
+            dval = v32->p;
         }
         // scheme on the device (see kernels.h)
         int dev_scheme;
@@ -735,34 +832,38 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             check(hipMemsetAsync(dmu.p, 0, sizeof(double) * (size_t)M * B, s), "moments");
             if (p->start == LEO2D_START_TARGETS) {
                 std::vector<int32_t> tg(B);
-                for (int r = 0; r < B; ++r) tg[r] = p->targets[b0 + std::min(r, Rb - 1)];
+                for (int r = 0; r < B; ++r) {
+                    const int t = p->targets[b0 + std::min(r, Rb - 1)];
+                    tg[r] = spatial ? h->perm_host[t] : t;
+                }
                 check(hipMemcpyAsync(dtg.p, tg.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s), "targets");
                 check(hipStreamSynchronize(s), "targets");
                 check(leo2d::launch_init_targets(A.p, vec_type, n, B, (const int32_t*)dtg.p, s), "init");
             } else if (p->start == LEO2D_START_RANDOM_PHASE) {
                 if (real_vec) throw ArgError("random phases need complex vectors");
-                check(leo2d::launch_init_random_phase(A.p, vec_type, n, B, p->seed, b0, s), "init");
+                check(leo2d::launch_init_random_phase_perm(A.p, vec_type, n, B, p->seed, b0, iperm, s), "init");
             } else {
                 const int w = p->vectors_complex ? 2 : 1;
                 hv.assign((size_t)w * n * B, 0.0);
                 std::memcpy(hv.data(), p->vectors + (size_t)w * n * b0, sizeof(double) * (size_t)w * n * Rb);
                 dalloc(host_v, sizeof(double) * hv.size(), "vector upload");
                 check(hipMemcpyAsync(host_v.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, s), "upload");
-                check(leo2d::launch_init_from_host_layout(A.p, vec_type, (const double*)host_v.p, p->vectors_complex, n, B,
-                                                          s),
+                check(leo2d::launch_init_from_host_layout_perm(A.p, vec_type, (const double*)host_v.p,
+                                                               p->vectors_complex, n, B, iperm, s),
                       "init");
             }
             if (keep_v) check(hipMemcpyAsync(V.p, A.p, vb * (size_t)n * B, hipMemcpyDeviceToDevice, s), "copy start");
             check(hipStreamSynchronize(s), "init");
             leo2d::StepArgs sa{};
-            sa.row_ptr = (const int64_t*)h->rp.p;
-            sa.col = (const int32_t*)h->col.p;
+            sa.row_ptr = drp;
+            sa.col = dcol;
             sa.val = dval;
             sa.n = n;
             sa.R = B;
             sa.part = (double*)part.p;
             sa.grid = G;
             sa.v = keep_v ? V.p : nullptr;
+            sa.variant = kpm_variant();
             leo2d::ReduceArgs ra{};
             ra.part = (const double*)part.p;
             ra.grid = G;

@@ -33,6 +33,7 @@ struct StepArgs {
     int beta;
     double* part;   // [grid][R][3]
     int grid;       // number of workgroups (multiple of 8)
+    int variant;    // R >= 64: 0 = lane-per-vector loop, 1/2 = wavefront-per-row with 8/16 gathers in flight
 };
 hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStream_t s);
 
@@ -62,6 +63,16 @@ hipError_t launch_init_random_phase(void* x, int vec_type, int n, int R, uint64_
 hipError_t launch_init_from_host_layout(void* x, int vec_type, const double* src, int src_complex, int n, int R,
                                         hipStream_t s);
 hipError_t launch_convert_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s);
+
+// spatial permutation support (q = new row, i = iperm[q] = original row)
+hipError_t launch_init_random_phase_perm(void* x, int vec_type, int n, int R, uint64_t seed, int r0,
+                                        const int32_t* iperm, hipStream_t s);
+hipError_t launch_init_from_host_layout_perm(void* x, int vec_type, const double* src, int src_complex, int n, int R,
+                                             const int32_t* iperm, hipStream_t s);
+hipError_t launch_perm_len(const int64_t* rp, const int32_t* iperm, int n, int64_t* len, hipStream_t s);
+hipError_t launch_perm_fill(const int64_t* rp, const int32_t* col, const void* val, int val_bytes, const int32_t* iperm,
+                            const int32_t* perm, int n, const int64_t* prp, int32_t* pcol, void* pval, hipStream_t s);
+hipError_t scan_exclusive_i64(const int64_t* cnt, int64_t* out, int n, hipStream_t s);
 
 // Jackson-kernel DoS: rho[t][k] = DCT-III(g * mu[t]) / (pi sqrt(a^2 - (E_k - b)^2)) * scale
 hipError_t launch_dos(const double* mu, int T, int M, double a, double b, double scale, double* E, double* rho,

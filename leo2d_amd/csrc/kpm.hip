@@ -14,12 +14,13 @@
 // wavefront, one fully coalesced load).  Lanes of a wavefront map to vectors;
 // a wavefront owns 64/R rows when R < 64.
 //
-// Scheduling: a fixed grid of G workgroups (G multiple of 8).  Each workgroup
-// owns one contiguous chunk of rows and the workgroup -> chunk map is
-// XCD-aware: hardware dispatches workgroup b to XCD (b mod 8), so logical
-// chunk = (b mod 8) * G/8 + b/8 gives every XCD one contiguous 1/8 of the
-// rows and keeps its L2 working set (the neighbour rows gathered from T_j)
-// local.
+// Scheduling: a fixed grid of G workgroups (G multiple of 8) sweeps the rows
+// in strides of G * rows-per-workgroup.  Hardware dispatches workgroup b to
+// XCD (b mod 8); the logical slot (b mod 8) * G/8 + b/8 gives every XCD one
+// contiguous block of G/8 workgroup tiles per sweep, so at any moment each
+// XCD works on a compact window of rows whose gathered neighbours (T_j rows)
+// stay in its L2.  With the spatial (Morton) row order applied by the host,
+// that window is a compact 2D patch of the lattice.
 //
 // Reductions are deterministic: fixed per-lane order, fixed shuffle tree,
 // fixed cross-wavefront order, one partial per workgroup and vector, then a
@@ -137,12 +138,10 @@ __global__ __launch_bounds__(kThreads) void kpm_step_kernel(StepArgs a)
 
     const int G = gridDim.x;
     const int b = blockIdx.x;
-    const int lb = (b & 7) * (G >> 3) + (b >> 3);   // XCD-contiguous chunk
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);   // XCD-contiguous slot in each sweep
     const int n = a.n;
-    const int per = (n + G - 1) / G;
-    const int chunk = (per + RPI - 1) / RPI * RPI;
-    const int row_begin = lb * chunk;
-    const int row_end = min(n, row_begin + chunk);
+    const int row_end = n;
+    const int stride = G * RPI;
 
     const HT* __restrict__ val = static_cast<const HT*>(a.val);
     const int32_t* __restrict__ col = a.col;
@@ -159,7 +158,7 @@ __global__ __launch_bounds__(kThreads) void kpm_step_kernel(StepArgs a)
 #pragma unroll
     for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
 
-    for (int row0 = row_begin + wave * RPW; row0 < row_end; row0 += RPI) {
+    for (int row0 = lb * RPI + wave * RPW; row0 < row_end; row0 += stride) {
         int row = row0 + sub;
         if (RPW == 1) row = __builtin_amdgcn_readfirstlane(row);
         if (row >= row_end) continue;
@@ -234,10 +233,164 @@ __global__ __launch_bounds__(kThreads) void kpm_step_kernel(StepArgs a)
     }
 }
 
+// ---- wavefront-per-row step for R >= 64 --------------------------------------
+// A row's (column, value) entries are fetched with one coalesced load per 64
+// entries (lane k holds entry k), then broadcast lane by lane with readlane
+// into scalar registers: the gather address becomes scalar base + lane offset
+// and BATCH gathers are issued back to back with no dependent index load in
+// between.  The next row's entries are prefetched while the current row's
+// gathers are in flight.
+__device__ inline int rl(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline double rl(double v, int k)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float rl(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ inline double2 rl(double2 v, int k) { return make_double2(rl(v.x, k), rl(v.y, k)); }
+__device__ inline float2 rl(float2 v, int k) { return make_float2(rl(v.x, k), rl(v.y, k)); }
+
+__device__ inline int64_t uniform64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v & 0xffffffffLL));
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <typename HT>
+__device__ inline HT zero_h()
+{
+    HT z;
+    zero(z);
+    return z;
+}
+
+template <typename HT, typename XT, int VPL, int BATCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_wave_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    const int R = 64 * VPL;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    const int n = a.n;
+    const int stride = G * kWaves;
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const int32_t* __restrict__ col = a.col;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[VPL], d2[VPL], d3[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
+
+    int row = __builtin_amdgcn_readfirstlane(lb * kWaves + wave);
+    // prefetch the first row's entries
+    int64_t q0 = row < n ? rp[row] : 0, q1 = row < n ? rp[row + 1] : 0;
+    q0 = uniform64(q0);
+    q1 = uniform64(q1);
+    int my_c = (row < n && q0 + lane < q1) ? col[q0 + lane] : 0;
+    HT my_h = (row < n && q0 + lane < q1) ? val[q0 + lane] : zero_h<HT>();
+
+    for (; row < n; row += stride) {
+        const int next = row + stride;
+        int64_t nq0 = 0, nq1 = 0;
+        if (next < n) {
+            nq0 = rp[next];
+            nq1 = rp[next + 1];
+        }
+        AT acc[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) zero(acc[v]);
+        int64_t base = q0;
+        int c_reg = my_c;
+        HT h_reg = my_h;
+        for (;;) {
+            const int cnt = (int)min((int64_t)64, q1 - base);
+            for (int k = 0; k < cnt; k += BATCH) {
+                XT x[BATCH][VPL];
+#pragma unroll
+                for (int u = 0; u < BATCH; ++u) {
+                    const int kk = min(k + u, cnt - 1);
+                    const int c = rl(c_reg, kk);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) x[u][v] = cur[(size_t)c * R + lane + 64 * v];
+                }
+#pragma unroll
+                for (int u = 0; u < BATCH; ++u) {
+                    if (k + u < cnt) {
+                        const HT h = rl(h_reg, k + u);
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) mac(acc[v], h, x[u][v]);
+                    }
+                }
+            }
+            base += 64;
+            if (base >= q1) break;
+            c_reg = (base + lane < q1) ? col[base + lane] : 0;
+            h_reg = (base + lane < q1) ? val[base + lane] : zero_h<HT>();
+        }
+        // next row's entries (issued before this row's epilogue traffic)
+        nq0 = uniform64(nq0);
+        nq1 = uniform64(nq1);
+        if (next < n) {
+            my_c = (nq0 + lane < nq1) ? col[nq0 + lane] : 0;
+            my_h = (nq0 + lane < nq1) ? val[nq0 + lane] : zero_h<HT>();
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const size_t idx = (size_t)row * R + lane + 64 * v;
+            const XT c = cur[idx];
+            XT p;
+            if (beta) p = prev[idx];
+            else zero(p);
+            const XT o = comb(alpha, acc[v], p);
+            out[idx] = o;
+            d1[v] += nrm2(c);
+            d2[v] += redot(o, c);
+            if (direct) d3[v] += redot(vin[idx], c);
+        }
+        q0 = nq0;
+        q1 = nq1;
+    }
+
+    __shared__ double red[kWaves][64 * VPL][3];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+        red[wave][lane + 64 * v][0] = d1[v];
+        red[wave][lane + 64 * v][1] = d2[v];
+        red[wave][lane + 64 * v][2] = d3[v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double s = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += red[w][r][k];
+        a.part[((size_t)lb * R + r) * 3 + k] = s;
+    }
+}
+
 template <typename HT, typename XT, int LPR, int VPL>
 hipError_t launch_typed(const StepArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((kpm_step_kernel<HT, XT, LPR, VPL>), dim3(a.grid), dim3(kThreads), 0, s, a);
+    if (LPR == 64 && a.variant == 1)
+        hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8>), dim3(a.grid), dim3(kThreads), 0, s, a);
+    else if (LPR == 64 && a.variant == 2)
+        hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 16>), dim3(a.grid), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((kpm_step_kernel<HT, XT, LPR, VPL>), dim3(a.grid), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -382,6 +535,71 @@ __global__ void convert_kernel(const double* in, float* out, int64_t count)
         out[t] = (float)in[t];
 }
 
+// ---- symmetric permutation H' = P H P^T (row q of H' = row iperm[q] of H) --
+__global__ void perm_len_kernel(const int64_t* rp, const int32_t* iperm, int n, int64_t* len)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = iperm[q];
+    len[q] = rp[i + 1] - rp[i];
+}
+
+template <typename V>
+__global__ void perm_fill_kernel(const int64_t* rp, const int32_t* col, const V* val, const int32_t* iperm,
+                                 const int32_t* perm, int n, const int64_t* prp, int32_t* pcol, V* pval)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = iperm[q];
+    int64_t w = prp[q];
+    for (int64_t e = rp[i]; e < rp[i + 1]; ++e, ++w) {
+        pcol[w] = perm[col[e]];
+        pval[w] = val[e];
+    }
+}
+
+__global__ void init_random_phase_perm_kernel(void* x, int vec_type, int n, int R, uint64_t seed, int r0,
+                                              const int32_t* iperm)
+{
+    const size_t total = (size_t)n * R;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t q = t / R, r = t % R;
+        const uint64_t i = iperm ? (uint64_t)iperm[q] : q;
+        uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
+        h = mix64(h ^ ((r0 + r) * 0xD1B54A32D192ED03ULL));
+        h = mix64(h ^ (i * 0x8CB92BA72F3D8DD7ULL));
+        const double phi = (double)(h >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586;
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);
+        else static_cast<float2*>(x)[t] = make_float2((float)cs, (float)sn);
+    }
+}
+
+__global__ void init_from_host_layout_perm_kernel(void* x, int vec_type, const double* src, int src_complex, int n,
+                                                  int R, const int32_t* iperm)
+{
+    const size_t total = (size_t)n * R;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t q = t / R, r = t % R;
+        const size_t i = iperm ? (size_t)iperm[q] : q;
+        double re_v, im_v;
+        if (src_complex) {
+            re_v = src[2 * (r * n + i)];
+            im_v = src[2 * (r * n + i) + 1];
+        } else {
+            re_v = src[r * n + i];
+            im_v = 0.0;
+        }
+        switch (vec_type) {
+        case VEC_REAL_F64: static_cast<double*>(x)[t] = re_v; break;
+        case VEC_CPX_F64: static_cast<double2*>(x)[t] = make_double2(re_v, im_v); break;
+        case VEC_CPX_F32: static_cast<float2*>(x)[t] = make_float2((float)re_v, (float)im_v); break;
+        default: static_cast<float*>(x)[t] = (float)re_v; break;
+        }
+    }
+}
+
 inline dim3 stride_grid(size_t total)
 {
     size_t g = (total + kThreads - 1) / kThreads;
@@ -430,6 +648,44 @@ hipError_t launch_init_from_host_layout(void* x, int vec_type, const double* src
 {
     hipLaunchKernelGGL(init_from_host_layout_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x, vec_type,
                        src, src_complex, n, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_random_phase_perm(void* x, int vec_type, int n, int R, uint64_t seed, int r0,
+                                        const int32_t* iperm, hipStream_t s)
+{
+    if (vec_type != VEC_CPX_F64 && vec_type != VEC_CPX_F32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(init_random_phase_perm_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x, vec_type,
+                       n, R, seed, r0, iperm);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_from_host_layout_perm(void* x, int vec_type, const double* src, int src_complex, int n, int R,
+                                             const int32_t* iperm, hipStream_t s)
+{
+    hipLaunchKernelGGL(init_from_host_layout_perm_kernel, stride_grid((size_t)n * R), dim3(kThreads), 0, s, x,
+                       vec_type, src, src_complex, n, R, iperm);
+    return hipGetLastError();
+}
+
+hipError_t launch_perm_len(const int64_t* rp, const int32_t* iperm, int n, int64_t* len, hipStream_t s)
+{
+    hipLaunchKernelGGL(perm_len_kernel, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, rp, iperm, n, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_perm_fill(const int64_t* rp, const int32_t* col, const void* val, int val_bytes, const int32_t* iperm,
+                            const int32_t* perm, int n, const int64_t* prp, int32_t* pcol, void* pval, hipStream_t s)
+{
+    const dim3 g((n + kThreads - 1) / kThreads);
+    if (val_bytes == 16)
+        hipLaunchKernelGGL(perm_fill_kernel<double2>, g, dim3(kThreads), 0, s, rp, col, (const double2*)val, iperm, perm,
+                           n, prp, pcol, (double2*)pval);
+    else if (val_bytes == 8)
+        hipLaunchKernelGGL(perm_fill_kernel<double>, g, dim3(kThreads), 0, s, rp, col, (const double*)val, iperm, perm,
+                           n, prp, pcol, (double*)pval);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -227,3 +227,26 @@ def test_spectrum_and_nan_errors(dev):
     with pytest.raises(L.Leo2dError) as e:
         dev.kpm(64, targets=[n + 5])
     assert e.value.code == L.ERR_ARG
+
+
+@pytest.mark.parametrize("name", ["tblg_c0_twin.in", "tblg_sc_7_6_k.in"])
+def test_spatial_order_matches_natural_order(dev, name):
+    """The recursion runs on P H P^T (Morton order) by default; moments must
+    match the reference row order up to summation order."""
+    text = deck_text(name)
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    dev.assemble(job, L.PATTERN_PRUNED)
+    M = 128
+    a = dev.kpm(M, random_phase=(16, 3), scheme=L.SCHEME_DOUBLING)
+    b = dev.kpm(M, random_phase=(16, 3), scheme=L.SCHEME_DOUBLING, natural_order=True)
+    assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
+    t = list(job.targets[: job.num_targets])
+    a = dev.kpm(M, targets=t)
+    b = dev.kpm(M, targets=t, natural_order=True)
+    assert np.max(np.abs(a - b)) <= 1e-13
+    rng = np.random.default_rng(9)
+    V = rng.standard_normal((4, geom.n))
+    a = dev.kpm(M, vectors=V)
+    b = dev.kpm(M, vectors=V, natural_order=True)
+    assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
