@@ -170,6 +170,9 @@ typedef struct {
     int64_t nnz;            /* stored entries of H */
     int32_t n, R, M;
     int32_t grid;
+    int32_t kernel;   /* step-kernel variant launched for the last block (0 = lane per vector,
+                         1 = wavefront per row, 8 = wavefront per row, matrix via scalar loads) */
+    int32_t reserved;
 } leo2d_timing;
 
 /* mu: host double[R][M]. */

@@ -89,6 +89,7 @@ class Timing(C.Structure):
     _fields_ = [
         ("recursion_ms", C.c_double), ("step_ms_avg", C.c_double), ("steps", C.c_int64),
         ("nnz", C.c_int64), ("n", C.c_int32), ("R", C.c_int32), ("M", C.c_int32), ("grid", C.c_int32),
+        ("kernel", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -72,11 +72,22 @@ struct NanError : std::runtime_error {
     explicit NanError(const std::string& m) : std::runtime_error(m) {}
 };
 
-// step-kernel variant for R >= 64 (LEO2D_KPM_VARIANT overrides the default)
-int kpm_variant()
+// step-kernel variant for R >= 64 (LEO2D_KPM_VARIANT overrides the default):
+// the wavefront-per-row kernel with readlane broadcasts (1) for FP64, the same
+// with the matrix on the scalar path (8) for the FP32 recursion, whose 8-byte
+// complex values leave the vector-memory path to the gathers
+int kpm_variant(bool mixed, bool spatial)
 {
-    const char* e = std::getenv("LEO2D_KPM_VARIANT");
-    return e ? std::atoi(e) : 1;
+    if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) return std::atoi(e);
+    return mixed && spatial ? 8 : 1;
+}
+
+// row schedule of the wavefront-per-row kernels: 1 = one contiguous slice of
+// rows per XCD (LEO2D_KPM_SCHED overrides the default)
+int kpm_sched()
+{
+    const char* e = std::getenv("LEO2D_KPM_SCHED");
+    return e ? std::atoi(e) : 0;
 }
 
 int next_pow2(int x)
@@ -115,6 +126,12 @@ struct leo2d_handle {
     DevBuf perm, iperm;
     bool pcsr_valid = false, pval32_valid = false;
     DevBuf prp, pcol_k, pval, pval32;
+    // LDS tiles of the matrix the recursion runs on (built on first use)
+    struct Tiles {
+        bool valid = false, failed = false, spatial = false;
+        int ucap = 0, ntiles = 0;
+        DevBuf row0, u0, ucols, lcol;
+    } tiles;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
 
@@ -224,6 +241,74 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->val32_valid = false;
     h->pcsr_valid = false;
     h->pval32_valid = false;
+    h->tiles.valid = false;
+    h->tiles.failed = false;
+}
+
+// Greedy row tiles for the LDS-tiled step: consecutive rows (at most
+// kTileMaxRows) while the tile's distinct columns fit `ucap`.  Returns false
+// when a single row alone has more than `ucap` distinct columns.
+bool build_tiles(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, bool spatial)
+{
+    auto& T = h->tiles;
+    if (T.valid && T.ucap == ucap && T.spatial == spatial) return true;
+    if (T.failed && T.ucap == ucap && T.spatial == spatial) return false;
+    const int n = h->n;
+    hipStream_t s = h->stream;
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col((size_t)h->nnz);
+    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "tiles");
+    if (h->nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "tiles");
+    check(hipStreamSynchronize(s), "tiles");
+    std::vector<int32_t> row0, ucols;
+    std::vector<int64_t> u0;
+    std::vector<uint16_t> lcol((size_t)h->nnz);
+    std::vector<int32_t> stamp(n, -1), pos(n, 0);
+    std::vector<int32_t> set;
+    int r = 0, tile = 0;
+    bool ok = true;
+    while (r < n) {
+        const int start = r;
+        set.clear();
+        while (r < n && r - start < leo2d::kTileMaxRows) {
+            int fresh = 0;
+            for (int64_t q = rp[r]; q < rp[r + 1]; ++q)
+                if (stamp[col[q]] != tile) { stamp[col[q]] = tile; ++fresh; set.push_back(col[q]); }
+            if ((int)set.size() > ucap) {
+                // undo this row
+                for (int k = 0; k < fresh; ++k) { stamp[set.back()] = -1; set.pop_back(); }
+                break;
+            }
+            ++r;
+        }
+        if (r == start) { ok = false; break; }
+        std::sort(set.begin(), set.end());
+        for (size_t k = 0; k < set.size(); ++k) pos[set[k]] = (int32_t)k;
+        for (int rr = start; rr < r; ++rr)
+            for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) lcol[q] = (uint16_t)pos[col[q]];
+        row0.push_back(start);
+        u0.push_back((int64_t)ucols.size());
+        ucols.insert(ucols.end(), set.begin(), set.end());
+        ++tile;
+    }
+    T.ucap = ucap;
+    T.spatial = spatial;
+    if (!ok) {
+        T.failed = true;
+        T.valid = false;
+        return false;
+    }
+    row0.push_back(n);
+    u0.push_back((int64_t)ucols.size());
+    to_device(T.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "tile rows");
+    to_device(T.u0, u0.data(), sizeof(int64_t) * u0.size(), s, "tile offsets");
+    to_device(T.ucols, ucols.data(), sizeof(int32_t) * ucols.size(), s, "tile columns");
+    to_device(T.lcol, lcol.data(), sizeof(uint16_t) * lcol.size(), s, "tile local columns");
+    check(hipStreamSynchronize(s), "tiles");
+    T.ntiles = tile;
+    T.valid = true;
+    T.failed = false;
+    return true;
 }
 
 // Morton (Z-order) rank of every orbital by in-plane position on a 1.42 A
@@ -273,9 +358,14 @@ void build_permuted(leo2d_handle* h)
     dalloc(h->prp, sizeof(int64_t) * (n + 1), "permuted rows");
     check(leo2d::launch_perm_len((const int64_t*)h->rp.p, (const int32_t*)h->iperm.p, n, (int64_t*)len.p, s), "perm");
     check(leo2d::scan_exclusive_i64((const int64_t*)len.p, (int64_t*)h->prp.p, n, s), "perm scan");
-    dalloc(h->pcol_k, sizeof(int32_t) * (size_t)h->nnz, "permuted columns");
+    // kCsrPad trailing entries (column 0, value 0) let the scalar-path step
+    // kernel load whole chunks of a row without a bounds clamp
+    const size_t padded = (size_t)h->nnz + leo2d::kCsrPad;
+    dalloc(h->pcol_k, sizeof(int32_t) * padded, "permuted columns");
     const int vb = h->is_complex ? 16 : 8;
-    dalloc(h->pval, (size_t)vb * h->nnz, "permuted values");
+    dalloc(h->pval, (size_t)vb * padded, "permuted values");
+    check(hipMemsetAsync((int32_t*)h->pcol_k.p + h->nnz, 0, sizeof(int32_t) * leo2d::kCsrPad, s), "perm pad");
+    check(hipMemsetAsync((char*)h->pval.p + (size_t)vb * h->nnz, 0, (size_t)vb * leo2d::kCsrPad, s), "perm pad");
     check(leo2d::launch_perm_fill((const int64_t*)h->rp.p, (const int32_t*)h->col.p, h->val.p, vb,
                                   (const int32_t*)h->iperm.p, (const int32_t*)h->perm.p, n, (const int64_t*)h->prp.p,
                                   (int32_t*)h->pcol_k.p, h->pval.p, s),
@@ -796,7 +886,9 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         if (mixed) {
             const size_t count = (size_t)h->nnz * (h->is_complex ? 2 : 1);
             if (!*v32_valid) {
-                dalloc(*v32, sizeof(float) * count, "fp32 matrix");
+                const size_t pad = (size_t)leo2d::kCsrPad * (h->is_complex ? 2 : 1);
+                dalloc(*v32, sizeof(float) * (count + pad), "fp32 matrix");
+                check(hipMemsetAsync((float*)v32->p + count, 0, sizeof(float) * pad, s), "fp32 pad");
                 check(leo2d::launch_convert_f64_to_f32((const double*)dval, (float*)v32->p, (int64_t)count, s),
                       "fp32 matrix");
                 *v32_valid = true;
@@ -825,6 +917,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         dalloc(dtg, sizeof(int32_t) * B, "targets");
         std::vector<double> hv;
         double rec_ms = 0.0, step_ms = 0.0;
+        int kernel_used = -1;
         int64_t steps = 0;
         hipEvent_t e0 = event_at(h, 0), e1 = event_at(h, 1);
         for (int b0 = 0; b0 < R; b0 += B) {
@@ -863,10 +956,43 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             sa.part = (double*)part.p;
             sa.grid = G;
             sa.v = keep_v ? V.p : nullptr;
-            sa.variant = kpm_variant();
+            sa.variant = kpm_variant(mixed, spatial);
+            sa.xcd_rows = kpm_sched() == 1 ? (n + 7) / 8 : 0;
+            const bool tiled = sa.variant == 3 || sa.variant == 4;
+            if (tiled && (B % leo2d::kTileGroup != 0 ||
+                          !build_tiles(h, drp, dcol, leo2d::tile_ucap(vec_type), spatial)))
+                sa.variant = 1;
+            int lds_bytes = leo2d::kTileWaveLdsBytes;
+            if (const char* kb = std::getenv("LEO2D_TILE_LDS_KB")) lds_bytes = std::max(16, std::min(150, std::atoi(kb))) * 1024;
+            const int ucap5 = lds_bytes / (B * (int)leo2d::vec_bytes(vec_type));
+            if (sa.variant == 5 && ((B != 64 && B != 128) || ucap5 < 8 || !build_tiles(h, drp, dcol, ucap5, spatial)))
+                sa.variant = 1;
+            sa.tile_lds_bytes = lds_bytes;
+            if (sa.variant == 5 && p->grid <= 0) {
+                // one resident round: workgroups per CU from the LDS budget
+                int dev_id = 0, cus = 256;
+                if (hipGetDevice(&dev_id) == hipSuccess)
+                    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+                const int per_cu = std::max(1, (160 * 1024) / (lds_bytes + 4096));
+                sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));
+            }
+            if ((sa.variant == 8 || sa.variant == 9) && (!spatial || B % 64 != 0)) sa.variant = 1;
+            if (B < 64 && sa.variant != 3 && sa.variant != 4) sa.variant = 0;
+            kernel_used = sa.variant;
+            if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
+                // larger workgroups: keep the total number of resident wavefronts
+                sa.grid = std::max(8, (sa.variant == 6 ? G / 4 : G / 2) / 8 * 8);
+            }
+            if (sa.variant == 3 || sa.variant == 4 || sa.variant == 5) {
+                sa.ntiles = h->tiles.ntiles;
+                sa.tile_row0 = (const int32_t*)h->tiles.row0.p;
+                sa.tile_u0 = (const int64_t*)h->tiles.u0.p;
+                sa.ucols = (const int32_t*)h->tiles.ucols.p;
+                sa.lcol = (const uint16_t*)h->tiles.lcol.p;
+            }
             leo2d::ReduceArgs ra{};
             ra.part = (const double*)part.p;
-            ra.grid = G;
+            ra.grid = sa.grid;
             ra.R = B;
             ra.M = M;
             ra.scheme = dev_scheme;
@@ -915,6 +1041,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         h->timing.R = R;
         h->timing.M = M;
         h->timing.grid = G;
+        h->timing.kernel = kernel_used;
         // moments of a spectrum inside [-1, 1] satisfy |mu_n| <= mu_0
         for (int r = 0; r < R; ++r) {
             const double* m = mu + (size_t)r * M;

@@ -267,18 +267,51 @@ __device__ inline HT zero_h()
     return z;
 }
 
-template <typename HT, typename XT, int VPL, int BATCH>
-__global__ __launch_bounds__(kThreads) void kpm_step_wave_kernel(StepArgs a)
+// sum over the 4 entry slots (lanes e, e+16, e+32, e+48)
+__device__ inline void fold_slots(double& a)
+{
+    a += __shfl_xor(a, 16);
+    a += __shfl_xor(a, 32);
+}
+__device__ inline void fold_slots(float& a)
+{
+    a += __shfl_xor(a, 16);
+    a += __shfl_xor(a, 32);
+}
+__device__ inline void fold_slots(double2& a)
+{
+    fold_slots(a.x);
+    fold_slots(a.y);
+}
+__device__ inline void fold_slots(float2& a)
+{
+    fold_slots(a.x);
+    fold_slots(a.y);
+}
+
+template <typename HT, typename XT, int VPL, int BATCH, int NT = kThreads>
+__global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
 {
     using AT = typename acc_of<XT>::type;
+    constexpr int NW = NT / 64;
     const int R = 64 * VPL;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int G = gridDim.x;
     const int b = blockIdx.x;
     const int lb = (b & 7) * (G >> 3) + (b >> 3);
-    const int n = a.n;
-    const int stride = G * kWaves;
+    // rows of this workgroup: interleaved over the whole matrix, or (xcd_rows
+    // > 0) a grid-stride walk over its XCD's own contiguous slice of rows
+    int first, n, stride;
+    if (a.xcd_rows > 0) {
+        first = (b & 7) * a.xcd_rows + (b >> 3) * NW;
+        n = min(a.n, ((b & 7) + 1) * a.xcd_rows);
+        stride = (G >> 3) * NW;
+    } else {
+        first = lb * NW;
+        n = a.n;
+        stride = G * NW;
+    }
 
     const HT* __restrict__ val = static_cast<const HT*>(a.val);
     const int32_t* __restrict__ col = a.col;
@@ -295,7 +328,7 @@ __global__ __launch_bounds__(kThreads) void kpm_step_wave_kernel(StepArgs a)
 #pragma unroll
     for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
 
-    int row = __builtin_amdgcn_readfirstlane(lb * kWaves + wave);
+    int row = __builtin_amdgcn_readfirstlane(first + wave);
     // prefetch the first row's entries
     int64_t q0 = row < n ? rp[row] : 0, q1 = row < n ? rp[row + 1] : 0;
     q0 = uniform64(q0);
@@ -365,7 +398,591 @@ __global__ __launch_bounds__(kThreads) void kpm_step_wave_kernel(StepArgs a)
         q1 = nq1;
     }
 
-    __shared__ double red[kWaves][64 * VPL][3];
+    __shared__ double red[NW][64 * VPL][3];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+        red[wave][lane + 64 * v][0] = d1[v];
+        red[wave][lane + 64 * v][1] = d2[v];
+        red[wave][lane + 64 * v][2] = d3[v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += NT) {
+        const int r = t / 3, k = t % 3;
+        double s = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += red[w][r][k];
+        a.part[((size_t)lb * R + r) * 3 + k] = s;
+    }
+}
+
+// ---- LDS-tiled step (R a multiple of 16) --------------------------------------
+// One workgroup owns one tile of consecutive (Morton-ordered, hence spatially
+// compact) rows at a time.  For each group of 16 vectors it stages the tile's
+// distinct neighbour rows x[U][16] into LDS with coalesced 256-byte (FP64)
+// segments, then every row reads its nonzeros' x from LDS through the 16-bit
+// tile-local column index.  L2 traffic per row drops from nnz_row x R x 16 B
+// to (U / T) x R x 16 B (about 4-5 rows instead of 32 for bilayer graphene).
+template <typename HT, typename XT, int NG>
+__global__ __launch_bounds__(kThreads) void kpm_step_tiled_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int RG = kTileGroup;
+    constexpr int RPW = 64 / RG;            // rows per wavefront pass
+    constexpr int RPI = RPW * kWaves;       // rows per workgroup pass
+    constexpr int SU = 8;                   // staging gathers in flight per thread
+    constexpr int SLOTS = kThreads / RG;    // rows staged per workgroup pass
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int ucap = kTileLdsBytes / (RG * (int)sizeof(XT));
+    XT* xs = reinterpret_cast<XT*>(lds_raw);
+    int32_t* uc = reinterpret_cast<int32_t*>(lds_raw + kTileLdsBytes);   // tile column list
+    const int R = RG * NG;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sub = lane / RG;
+    const int e = lane % RG;
+    const int slot = threadIdx.x / RG;      // staging row slot
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const uint16_t* __restrict__ lcol = a.lcol;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[NG], d2[NG], d3[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) { d1[g] = 0.0; d2[g] = 0.0; d3[g] = 0.0; }
+
+    for (int tile = lb; tile < a.ntiles; tile += G) {
+        const int r_begin = a.tile_row0[tile], r_end = a.tile_row0[tile + 1];
+        const int64_t u_begin = a.tile_u0[tile];
+        const int U = min((int)(a.tile_u0[tile + 1] - u_begin), ucap);
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += kThreads) uc[u] = a.ucols[u_begin + u];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            __syncthreads();
+            for (int m0 = slot; m0 < U; m0 += SLOTS * SU) {
+                XT xv[SU];
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xv[m] = cur[(size_t)uc[u] * R + g * RG + e];
+                }
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xs[u * RG + e] = xv[m];
+                }
+            }
+            __syncthreads();
+            for (int row0 = r_begin + wave * RPW; row0 < r_end; row0 += RPI) {
+                const int row = row0 + sub;
+                if (row >= r_end) continue;
+                AT acc;
+                zero(acc);
+                int64_t q = rp[row];
+                const int64_t qe = rp[row + 1];
+                for (; q + 4 <= qe; q += 4) {
+                    const HT h0 = val[q], h1 = val[q + 1], h2 = val[q + 2], h3 = val[q + 3];
+                    const int l0 = lcol[q], l1 = lcol[q + 1], l2 = lcol[q + 2], l3 = lcol[q + 3];
+                    const XT x0 = xs[l0 * RG + e];
+                    const XT x1 = xs[l1 * RG + e];
+                    const XT x2 = xs[l2 * RG + e];
+                    const XT x3 = xs[l3 * RG + e];
+                    mac(acc, h0, x0);
+                    mac(acc, h1, x1);
+                    mac(acc, h2, x2);
+                    mac(acc, h3, x3);
+                }
+                for (; q < qe; ++q) mac(acc, val[q], xs[lcol[q] * RG + e]);
+                const size_t idx = (size_t)row * R + g * RG + e;
+                const XT c = cur[idx];
+                XT p;
+                if (beta) p = prev[idx];
+                else zero(p);
+                const XT o = comb(alpha, acc, p);
+                out[idx] = o;
+                d1[g] += nrm2(c);
+                d2[g] += redot(o, c);
+                if (direct) d3[g] += redot(vin[idx], c);
+            }
+        }
+    }
+
+    // fold the row slots of a wavefront, then the wavefronts (LDS reused)
+#pragma unroll
+    for (int off = RG; off < 64; off <<= 1) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            d1[g] += __shfl_xor(d1[g], off);
+            d2[g] += __shfl_xor(d2[g], off);
+            d3[g] += __shfl_xor(d3[g], off);
+        }
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds_raw);   // [kWaves][R][3]
+    if (sub == 0) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int r = g * RG + e;
+            red[(wave * R + r) * 3 + 0] = d1[g];
+            red[(wave * R + r) * 3 + 1] = d2[g];
+            red[(wave * R + r) * 3 + 2] = d3[g];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        double s = red[t];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += red[w * R * 3 + t];
+        a.part[(size_t)lb * R * 3 + t] = s;
+    }
+}
+
+// Variant 4: same tiles and LDS staging, but one row per wavefront pass: lane
+// = (entry slot k in 0..3) x (vector e in 0..15).  Lane group k owns entries
+// k, k+4, ..., k+28 of the row, loaded together up front (and prefetched for
+// the wavefront's next row), so a row costs one memory round trip; the four
+// partial sums are folded with two cross-lane shuffles.
+template <typename HT, typename XT, int NG>
+__global__ __launch_bounds__(kThreads) void kpm_step_tiled_row_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int RG = kTileGroup;
+    constexpr int KS = 64 / RG;             // entry slots per wavefront (4)
+    constexpr int EPL = 8;                  // entries per lane per chunk
+    constexpr int CH = KS * EPL;            // entries per chunk (32)
+    constexpr int SU = 8;
+    constexpr int SLOTS = kThreads / RG;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int ucap = kTileLdsBytes / (RG * (int)sizeof(XT));
+    XT* xs = reinterpret_cast<XT*>(lds_raw);
+    int32_t* uc = reinterpret_cast<int32_t*>(lds_raw + kTileLdsBytes);
+    const int R = RG * NG;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int k = lane / RG;
+    const int e = lane % RG;
+    const int slot = threadIdx.x / RG;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const uint16_t* __restrict__ lcol = a.lcol;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[NG], d2[NG], d3[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) { d1[g] = 0.0; d2[g] = 0.0; d3[g] = 0.0; }
+
+    for (int tile = lb; tile < a.ntiles; tile += G) {
+        const int r_begin = a.tile_row0[tile], r_end = a.tile_row0[tile + 1];
+        const int64_t u_begin = a.tile_u0[tile];
+        const int U = min((int)(a.tile_u0[tile + 1] - u_begin), ucap);
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += kThreads) uc[u] = a.ucols[u_begin + u];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            __syncthreads();
+            for (int m0 = slot; m0 < U; m0 += SLOTS * SU) {
+                XT xv[SU];
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xv[m] = cur[(size_t)uc[u] * R + g * RG + e];
+                }
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xs[u * RG + e] = xv[m];
+                }
+            }
+            __syncthreads();
+            int row = r_begin + wave;
+            HT hv[EPL];
+            int lv[EPL];
+            int64_t q0 = 0, q1 = 0;
+            if (row < r_end) {
+                q0 = rp[row];
+                q1 = rp[row + 1];
+#pragma unroll
+                for (int i = 0; i < EPL; ++i) {
+                    const int64_t q = q0 + k + KS * i;
+                    hv[i] = q < q1 ? val[q] : zero_h<HT>();
+                    lv[i] = q < q1 ? (int)lcol[q] : 0;
+                }
+            }
+            for (; row < r_end; row += kWaves) {
+                // prefetch the next row's first chunk
+                const int nrow = row + kWaves;
+                HT hn[EPL];
+                int ln[EPL];
+                int64_t n0 = 0, n1 = 0;
+                if (nrow < r_end) {
+                    n0 = rp[nrow];
+                    n1 = rp[nrow + 1];
+#pragma unroll
+                    for (int i = 0; i < EPL; ++i) {
+                        const int64_t q = n0 + k + KS * i;
+                        hn[i] = q < n1 ? val[q] : zero_h<HT>();
+                        ln[i] = q < n1 ? (int)lcol[q] : 0;
+                    }
+                }
+                AT acc;
+                zero(acc);
+                const int64_t cnt = q1 - q0;
+#pragma unroll
+                for (int i = 0; i < EPL; ++i)
+                    if (k + KS * i < cnt) mac(acc, hv[i], xs[lv[i] * RG + e]);
+                for (int64_t base = q0 + CH; base < q1; base += CH) {   // rows longer than one chunk
+#pragma unroll
+                    for (int i = 0; i < EPL; ++i) {
+                        const int64_t q = base + k + KS * i;
+                        if (q < q1) mac(acc, val[q], xs[(int)lcol[q] * RG + e]);
+                    }
+                }
+                // fold the four entry slots
+                fold_slots(acc);
+                if (k == 0) {
+                    const size_t idx = (size_t)row * R + g * RG + e;
+                    const XT c = cur[idx];
+                    XT p;
+                    if (beta) p = prev[idx];
+                    else zero(p);
+                    const XT o = comb(alpha, acc, p);
+                    out[idx] = o;
+                    d1[g] += nrm2(c);
+                    d2[g] += redot(o, c);
+                    if (direct) d3[g] += redot(vin[idx], c);
+                }
+#pragma unroll
+                for (int i = 0; i < EPL; ++i) { hv[i] = hn[i]; lv[i] = ln[i]; }
+                q0 = n0;
+                q1 = n1;
+            }
+        }
+    }
+
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds_raw);   // [kWaves][R][3]
+    if (k == 0) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int r = g * RG + e;
+            red[(wave * R + r) * 3 + 0] = d1[g];
+            red[(wave * R + r) * 3 + 1] = d2[g];
+            red[(wave * R + r) * 3 + 2] = d3[g];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        double s = red[t];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += red[w * R * 3 + t];
+        a.part[(size_t)lb * R * 3 + t] = s;
+    }
+}
+
+template <typename HT, typename XT, int NG>
+hipError_t launch_tiled_row(const StepArgs& a, hipStream_t s)
+{
+    const size_t shmem = (size_t)kTileLdsBytes + sizeof(int32_t) * (kTileLdsBytes / (kTileGroup * sizeof(XT)));
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)kpm_step_tiled_row_kernel<HT, XT, NG>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((kpm_step_tiled_row_kernel<HT, XT, NG>), dim3(a.grid), dim3(kThreads), shmem, s, a);
+    return hipGetLastError();
+}
+
+// Variant 5: tiles staged with all R vectors (R = 64 * VPL) and one row per
+// wavefront (lane = vector).  The row's (tile-local column, value) entries are
+// fetched with one coalesced load per 64 entries and broadcast with readlane,
+// so the values feed v_fma as scalar operands and x comes from LDS: the vector
+// memory pipe only carries the staging loads (about U/T rows per row) and the
+// row's own T_j, T_{j-1} and T_{j+1}.
+template <typename HT, typename XT, int VPL, int SU>
+__global__ __launch_bounds__(kThreads) void kpm_step_tiled_wave_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int R = 64 * VPL;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int ucap = a.tile_lds_bytes / (R * (int)sizeof(XT));
+    XT* xs = reinterpret_cast<XT*>(lds_raw);
+    int32_t* uc = reinterpret_cast<int32_t*>(lds_raw + a.tile_lds_bytes);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const uint16_t* __restrict__ lcol = a.lcol;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[VPL], d2[VPL], d3[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
+
+    for (int tile = lb; tile < a.ntiles; tile += G) {
+        const int r_begin = a.tile_row0[tile], r_end = a.tile_row0[tile + 1];
+        const int64_t u_begin = a.tile_u0[tile];
+        const int U = min((int)(a.tile_u0[tile + 1] - u_begin), ucap);
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += kThreads) uc[u] = a.ucols[u_begin + u];
+        __syncthreads();
+        // stage x[U][R]: wavefront w takes staged rows w, w+4, ...
+        for (int u0 = wave; u0 < U; u0 += kWaves * SU) {
+            XT xv[SU][VPL];
+#pragma unroll
+            for (int m = 0; m < SU; ++m) {
+                const int u = u0 + m * kWaves;
+                if (u < U) {
+                    const size_t base = (size_t)uc[u] * R + lane;
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) xv[m][v] = cur[base + 64 * v];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < SU; ++m) {
+                const int u = u0 + m * kWaves;
+                if (u < U) {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) xs[u * R + lane + 64 * v] = xv[m][v];
+                }
+            }
+        }
+        __syncthreads();
+        int row = r_begin + wave;
+        int64_t q0 = 0, q1 = 0;
+        int l_reg = 0;
+        HT h_reg = zero_h<HT>();
+        if (row < r_end) {
+            q0 = uniform64(rp[row]);
+            q1 = uniform64(rp[row + 1]);
+            if (q0 + lane < q1) {
+                l_reg = lcol[q0 + lane];
+                h_reg = val[q0 + lane];
+            }
+        }
+        for (; row < r_end; row += kWaves) {
+            const int nrow = row + kWaves;
+            int64_t n0 = 0, n1 = 0;
+            if (nrow < r_end) {
+                n0 = rp[nrow];
+                n1 = rp[nrow + 1];
+            }
+            AT acc[VPL];
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) zero(acc[v]);
+            int64_t base = q0;
+            for (;;) {
+                const int cnt = (int)min((int64_t)64, q1 - base);
+                int kk = 0;
+                for (; kk + 4 <= cnt; kk += 4) {
+                    const int c0 = rl(l_reg, kk), c1 = rl(l_reg, kk + 1), c2 = rl(l_reg, kk + 2), c3 = rl(l_reg, kk + 3);
+                    const HT h0 = rl(h_reg, kk), h1 = rl(h_reg, kk + 1), h2 = rl(h_reg, kk + 2), h3 = rl(h_reg, kk + 3);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        const XT x0 = xs[c0 * R + lane + 64 * v];
+                        const XT x1 = xs[c1 * R + lane + 64 * v];
+                        const XT x2 = xs[c2 * R + lane + 64 * v];
+                        const XT x3 = xs[c3 * R + lane + 64 * v];
+                        mac(acc[v], h0, x0);
+                        mac(acc[v], h1, x1);
+                        mac(acc[v], h2, x2);
+                        mac(acc[v], h3, x3);
+                    }
+                }
+                for (; kk < cnt; ++kk) {
+                    const int c0 = rl(l_reg, kk);
+                    const HT h0 = rl(h_reg, kk);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) mac(acc[v], h0, xs[c0 * R + lane + 64 * v]);
+                }
+                base += 64;
+                if (base >= q1) break;
+                l_reg = (base + lane < q1) ? (int)lcol[base + lane] : 0;
+                h_reg = (base + lane < q1) ? val[base + lane] : zero_h<HT>();
+            }
+            n0 = uniform64(n0);
+            n1 = uniform64(n1);
+            if (nrow < r_end) {
+                l_reg = (n0 + lane < n1) ? (int)lcol[n0 + lane] : 0;
+                h_reg = (n0 + lane < n1) ? val[n0 + lane] : zero_h<HT>();
+            }
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                const size_t idx = (size_t)row * R + lane + 64 * v;
+                const XT c = cur[idx];
+                XT p;
+                if (beta) p = prev[idx];
+                else zero(p);
+                const XT o = comb(alpha, acc[v], p);
+                out[idx] = o;
+                d1[v] += nrm2(c);
+                d2[v] += redot(o, c);
+                if (direct) d3[v] += redot(vin[idx], c);
+            }
+            q0 = n0;
+            q1 = n1;
+        }
+    }
+
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds_raw);   // [kWaves][R][3]
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+        const int r = lane + 64 * v;
+        red[(wave * R + r) * 3 + 0] = d1[v];
+        red[(wave * R + r) * 3 + 1] = d2[v];
+        red[(wave * R + r) * 3 + 2] = d3[v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        double s = red[t];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += red[w * R * 3 + t];
+        a.part[(size_t)lb * R * 3 + t] = s;
+    }
+}
+
+template <typename HT, typename XT, int VPL, int SU>
+hipError_t launch_tiled_wave_su(const StepArgs& a, hipStream_t s)
+{
+    const size_t shmem = (size_t)a.tile_lds_bytes + sizeof(int32_t) * (a.tile_lds_bytes / (64 * VPL * sizeof(XT)));
+    static int attr_bytes = 0;
+    if (attr_bytes < (int)shmem) {
+        hipError_t e = hipFuncSetAttribute((const void*)kpm_step_tiled_wave_kernel<HT, XT, VPL, SU>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+        attr_bytes = (int)shmem;
+    }
+    hipLaunchKernelGGL((kpm_step_tiled_wave_kernel<HT, XT, VPL, SU>), dim3(a.grid), dim3(kThreads), shmem, s, a);
+    return hipGetLastError();
+}
+
+// all of a tile's staged rows are loaded in one burst: SU >= ucap / 4 rows per wavefront
+template <typename HT, typename XT, int VPL>
+hipError_t launch_tiled_wave(const StepArgs& a, hipStream_t s)
+{
+    const int ucap = a.tile_lds_bytes / (64 * VPL * (int)sizeof(XT));
+    const int per_wave = (ucap + kWaves - 1) / kWaves;
+    if (per_wave <= 8) return launch_tiled_wave_su<HT, XT, VPL, 8>(a, s);
+    if (per_wave <= 18) return launch_tiled_wave_su<HT, XT, VPL, 18>(a, s);
+    if (per_wave <= 36) return launch_tiled_wave_su<HT, XT, VPL, 36>(a, s);
+    return launch_tiled_wave_su<HT, XT, VPL, 72>(a, s);
+}
+
+template <int B> struct alignas(4) ColChunk { int c[B]; };
+template <typename HT, int B> struct ValChunk { HT h[B]; };
+
+// Variant 8: wavefront per row, H through the scalar path.  The row index is
+// wave-uniform and every pointer is restrict-qualified (T_{j-1} and T_{j+1}
+// share one pointer), so the compiler fetches the row's column indices and
+// values with scalar loads into SGPRs: no VALU broadcasts, no vector-memory
+// traffic for H; the vector pipe carries only the x gathers (SGPR row base +
+// lane offset) and the row's own T_j / T_{j-1} / T_{j+1}.
+template <typename HT, typename XT, int VPL, int BATCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_smem_kernel(const int64_t* __restrict__ rp,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const HT* __restrict__ val,
+                                                                 const XT* __restrict__ cur,
+                                                                 XT* __restrict__ po,
+                                                                 const XT* __restrict__ vin, int nrows, double alpha,
+                                                                 int beta, double* __restrict__ part, int xcd_rows)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int R = 64 * VPL;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    int first, n, stride;
+    if (xcd_rows > 0) {
+        first = (b & 7) * xcd_rows + (b >> 3) * kWaves;
+        n = min(nrows, ((b & 7) + 1) * xcd_rows);
+        stride = (G >> 3) * kWaves;
+    } else {
+        first = lb * kWaves;
+        n = nrows;
+        stride = G * kWaves;
+    }
+    const bool direct = vin != nullptr;
+
+    double d1[VPL], d2[VPL], d3[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
+
+    for (int row = __builtin_amdgcn_readfirstlane(first + wave); row < n; row += stride) {
+        const int64_t q0 = rp[row], q1 = rp[row + 1];
+        AT acc[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) zero(acc[v]);
+        for (int64_t q = q0; q < q1; q += BATCH) {
+            // whole-chunk loads: the arrays carry BATCH padding entries
+            // (column 0, value 0) past nnz, so q + BATCH never leaves them
+            const ColChunk<BATCH> cc = *reinterpret_cast<const ColChunk<BATCH>*>(col + q);
+            const ValChunk<HT, BATCH> hc = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + q);
+            const int* c = cc.c;
+            const HT* h = hc.h;
+            XT x[BATCH][VPL];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const XT* xr = cur + (size_t)c[u] * R;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) x[u][v] = xr[lane + 64 * v];
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+                if (q + u < q1) {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) mac(acc[v], h[u], x[u][v]);
+                }
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const size_t idx = (size_t)row * R + lane + 64 * v;
+            const XT cv = cur[idx];
+            XT p;
+            if (beta) p = po[idx];
+            else zero(p);
+            const XT o = comb(alpha, acc[v], p);
+            po[idx] = o;
+            d1[v] += nrm2(cv);
+            d2[v] += redot(o, cv);
+            if (direct) d3[v] += redot(vin[idx], cv);
+        }
+    }
+
+    __shared__ double red[kWaves][R][3];
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
         red[wave][lane + 64 * v][0] = d1[v];
@@ -378,14 +995,68 @@ __global__ __launch_bounds__(kThreads) void kpm_step_wave_kernel(StepArgs a)
         double s = red[0][r][k];
 #pragma unroll
         for (int w = 1; w < kWaves; ++w) s += red[w][r][k];
-        a.part[((size_t)lb * R + r) * 3 + k] = s;
+        part[((size_t)lb * R + r) * 3 + k] = s;
     }
+}
+
+template <typename HT, typename XT, int NG>
+hipError_t launch_tiled(const StepArgs& a, hipStream_t s)
+{
+    const size_t shmem = (size_t)kTileLdsBytes + sizeof(int32_t) * (kTileLdsBytes / (kTileGroup * sizeof(XT)));
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)kpm_step_tiled_kernel<HT, XT, NG>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((kpm_step_tiled_kernel<HT, XT, NG>), dim3(a.grid), dim3(kThreads), shmem, s, a);
+    return hipGetLastError();
 }
 
 template <typename HT, typename XT, int LPR, int VPL>
 hipError_t launch_typed(const StepArgs& a, hipStream_t s)
 {
-    if (LPR == 64 && a.variant == 1)
+    if (a.variant == 5 && a.ntiles > 0) {
+        if (a.R == 64) return launch_tiled_wave<HT, XT, 1>(a, s);
+        if (a.R == 128) return launch_tiled_wave<HT, XT, 2>(a, s);
+    }
+    if (a.variant == 4 && a.ntiles > 0) {
+        switch (a.R) {
+        case 16: return launch_tiled_row<HT, XT, 1>(a, s);
+        case 32: return launch_tiled_row<HT, XT, 2>(a, s);
+        case 64: return launch_tiled_row<HT, XT, 4>(a, s);
+        case 128: return launch_tiled_row<HT, XT, 8>(a, s);
+        default: break;
+        }
+    }
+    if (a.variant == 3 && a.ntiles > 0) {
+        switch (a.R) {
+        case 16: return launch_tiled<HT, XT, 1>(a, s);
+        case 32: return launch_tiled<HT, XT, 2>(a, s);
+        case 64: return launch_tiled<HT, XT, 4>(a, s);
+        case 128: return launch_tiled<HT, XT, 8>(a, s);
+        default: break;
+        }
+    }
+    if (LPR == 64 && (a.variant == 8 || a.variant == 9) && a.prev == a.out) {
+        const XT* cur = static_cast<const XT*>(a.cur);
+        XT* po = static_cast<XT*>(a.out);
+        if (a.variant == 8)
+            hipLaunchKernelGGL((kpm_step_smem_kernel<HT, XT, VPL, 8>), dim3(a.grid), dim3(kThreads), 0, s, a.row_ptr,
+                               a.col, static_cast<const HT*>(a.val), cur, po, static_cast<const XT*>(a.v), a.n, a.alpha,
+                               a.beta, a.part, a.xcd_rows);
+        else
+            hipLaunchKernelGGL((kpm_step_smem_kernel<HT, XT, VPL, 16>), dim3(a.grid), dim3(kThreads), 0, s, a.row_ptr,
+                               a.col, static_cast<const HT*>(a.val), cur, po, static_cast<const XT*>(a.v), a.n, a.alpha,
+                               a.beta, a.part, a.xcd_rows);
+        return hipGetLastError();
+    }
+    if (LPR == 64 && a.variant == 6)
+        hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8, 1024>), dim3(a.grid), dim3(1024), 0, s, a);
+    else if (LPR == 64 && a.variant == 7)
+        hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8, 512>), dim3(a.grid), dim3(512), 0, s, a);
+    else if (LPR == 64 && (a.variant == 1 || a.variant >= 8))
         hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8>), dim3(a.grid), dim3(kThreads), 0, s, a);
     else if (LPR == 64 && a.variant == 2)
         hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 16>), dim3(a.grid), dim3(kThreads), 0, s, a);

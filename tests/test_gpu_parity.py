@@ -250,3 +250,49 @@ def test_spatial_order_matches_natural_order(dev, name):
     a = dev.kpm(M, vectors=V)
     b = dev.kpm(M, vectors=V, natural_order=True)
     assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
+
+
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9"])
+@pytest.mark.parametrize("R", [4, 16, 64, 128])
+def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
+    """Every step-kernel variant (lane-per-vector, wavefront-per-row with 8/16
+    gathers in flight, LDS-tiled) gives the oracle's moments."""
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", variant)
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = 0.013, -0.007
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 48
+    vecs = oracle_mod.random_phase(geom.n, R, 5)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+        mu = dev.kpm(M, random_phase=(R, 5), scheme=L.SCHEME_DOUBLING, precision=prec)
+        assert np.max(np.abs(mu - omu)) <= tol * geom.n, (variant, R, prec)
+    mur = dev.kpm(M, random_phase=(R, 5), scheme=L.SCHEME_REFERENCE)
+    omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+    assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+
+
+@pytest.mark.parametrize("variant", ["1", "2", "8", "9"])
+@pytest.mark.parametrize("R", [64, 128])
+def test_xcd_slice_schedule(dev, oracle_mod, monkeypatch, variant, R):
+    """The one-slice-per-XCD row schedule covers every row exactly once: the
+    moments equal the oracle's, including for n not a multiple of 8 x 4."""
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", variant)
+    monkeypatch.setenv("LEO2D_KPM_SCHED", "1")
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = -0.011, 0.004
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 40
+    vecs = oracle_mod.random_phase(geom.n, R, 9)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    for grid in (0, 8, 24):
+        mu = dev.kpm(M, random_phase=(R, 9), scheme=L.SCHEME_DOUBLING, grid=grid)
+        assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n, (variant, R, grid)
