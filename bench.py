@@ -194,14 +194,7 @@ def main():
         "assembly": {"geometry_host_s": t_geom, "pair_search_s": t_pairs, "fill_ms": t_fill2 * 1e3,
                      "first_fill_ms": t_fill * 1e3},
     }
-    prof = os.path.join(ROOT, "profiles", "kpm_step_c1_traffic.json")
-    if os.path.exists(prof):
-        try:
-            pj = json.load(open(prof))
-            if pj.get("nnz") == nnz and pj.get("R") == R and pj.get("kernel") == out["roofline"]["kernel"]:
-                out["roofline"]["traffic"] = pj["bytes_per_launch"]
-        except Exception:
-            pass
+    out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"], nnz, R)
 
     if not args.no_mixed:
         mu32, times32, tm32 = run(L.PREC_MIXED, args.steps, 1, True)
@@ -213,6 +206,7 @@ def main():
             "value": nnz * R * (M // 2) / t32, "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t_step / t32,
             "max_abs_dDoS_states_per_eV_atom": ddos, "dtype": "complex64 recursion, f64 moment sums",
             "launch_us_avg": tm32.step_ms_avg * 1e3, "kernel": kernel_name(tm32.kernel, "float2"),
+            "traffic": measured_traffic(kernel_name(tm32.kernel, "float2"), nnz, R),
             "achieved_GBs": sb32 / (tm32.step_ms_avg * 1e-3) / 1e9,
         }
         log(f"mixed: {t32 * 1e3:.1f} ms/step ({t_step / t32:.2f}x), max|dDoS| = {ddos:.2e} states/eV/atom")
@@ -222,6 +216,21 @@ def main():
 
     dev.close()
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, nnz, R):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
+    calibration), or None when no pass covers this kernel and shape."""
+    path = os.path.join(ROOT, "profiles", "kpm_step_c1_traffic.json")
+    try:
+        pj = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if pj.get("nnz") != nnz or pj.get("R") != R:
+        return None
+    k = pj.get("kernels", {}).get(kernel)
+    return k["bytes_per_launch"] if k else None
 
 
 def kernel_name(variant, t):
@@ -242,7 +251,7 @@ def cpu_baseline(dev, nnz, n, mu_gpu, log):
     rp, ci, v = dev.csr_copy()
     threads = min(16, os.cpu_count() or 1)
     R_s = threads
-    M_s = 96
+    M_s = 768   # 384 steps: about 10-15 s of CPU work on 16 threads
     vecs = orc.random_phase(n, R_s, 0)
     t0 = time.perf_counter()
     mu = orc.kpm(rp, ci, v, M_s, vecs=vecs, doubling_all=True, nthreads=threads)
