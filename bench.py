@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--workload", choices=["c1", "c2"], default="c1",
+                    help="c1: 1.05 deg 8x8 tBLG (headline); c2: 32 uneven cells x 8 k-points batched")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -94,10 +96,14 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design (DESIGN.md), not measured here.", file=sys.stderr)
+              "how the path would shard is a design, not measured here.", file=sys.stderr)
         sys.exit(2)
 
     import leo2d_amd as L
+
+    if args.workload == "c2":
+        run_c2(args)
+        return
 
     M, R = args.M, args.R
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
@@ -216,6 +222,89 @@ def main():
 
     dev.close()
     print(json.dumps(out), flush=True)
+
+
+def run_c2(args):
+    """Batched uneven solves: 32 commensurate tBLG cells (i+1, i) with i drawn
+    uniformly from 9..31 (seed 1), 8 Bloch k-points per cell drawn uniformly in
+    the moire Brillouin zone, 16 random-phase vectors per k, M = 1024, all 256
+    jobs stacked and advanced by one hipGraph-captured recursion with the
+    autotuned launch shape.  The same jobs run one by one through the
+    single-job entry point give the unbatched rate beside it."""
+    import leo2d_amd as L
+    from leo2d_amd import workloads as W
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    M, R, cells, kpts = 1024, 16, 32, 8
+    rng = np.random.default_rng(1)
+    idx = rng.integers(9, 32, size=cells)
+    kfr = rng.random((cells, kpts, 2))
+    dev = L.Device(0)
+    t0 = time.perf_counter()
+    nnz_jobs = []
+    for c, i in enumerate(idx):
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(int(i) + 1, int(i), M=M)))
+        dev.set_geometry(geom)
+        dev.pairs_build()
+        for k in range(kpts):
+            _, _, _, nnz = W.assemble_rescaled(dev, geom, kfr[c, k])
+            dev.batch_append()
+            nnz_jobs.append(nnz)
+    t_asm = time.perf_counter() - t0
+    jobs, rows, nnz = dev.batch_info()
+    log(f"c2: {jobs} jobs, {rows} rows (cells i = {sorted(int(x) for x in idx)}), nnz {nnz}, assembly {t_asm:.2f}s")
+    updates = nnz * R * (M // 2)
+
+    out_prec = {}
+    for prec, name in ((L.PREC_FP64, "fp64"), (L.PREC_MIXED, "mixed")):
+        dev.batch_kpm(M, R, seed=0, precision=prec, tune=True)        # tune + capture
+        tt = dev.batch_timing()
+        for _ in range(args.warmup):
+            dev.batch_kpm(M, R, seed=0, precision=prec, tune=True)
+        times, gpu_ms = [], []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            mu = dev.batch_kpm(M, R, seed=0, precision=prec, tune=True)
+            times.append(time.perf_counter() - t1)
+            gpu_ms.append(dev.batch_timing().recursion_ms)
+        t = float(np.mean(times))
+        tb = dev.batch_timing()
+        out_prec[name] = {"value": updates / t, "ms_per_step": t * 1e3, "graph_ms": float(np.mean(gpu_ms)),
+                          "mode": tb.mode, "tile_rows": tb.tile_rows, "grid": tb.grid,
+                          "tune_ms": tt.tune_ms, "candidates": tt.candidates, "mu": mu}
+        log(f"c2 {name}: {t * 1e3:.1f} ms per batch ({updates / t:.3e} nnz.vec/s), graph {np.mean(gpu_ms):.1f} ms, "
+            f"tuned mode={tb.mode} tile_rows={tb.tile_rows} grid={tb.grid} ({tt.candidates} shapes, {tt.tune_ms:.0f} ms)")
+
+    # the same jobs one at a time through leo2d_kpm_moments (no batching, no graph)
+    t0 = time.perf_counter()
+    done = 0
+    for c, i in enumerate(idx):
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(int(i) + 1, int(i), M=M)))
+        dev.set_geometry(geom)
+        dev.pairs_build()
+        for k in range(kpts):
+            W.assemble_rescaled(dev, geom, kfr[c, k])
+            t1 = time.perf_counter()
+            dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING)
+            done += time.perf_counter() - t1
+    seq = updates / done
+    log(f"c2 one job at a time: {done * 1e3:.1f} ms of KPM ({seq:.3e} nnz.vec/s)")
+    dev.close()
+    mu64, mu32 = out_prec["fp64"].pop("mu"), out_prec["mixed"].pop("mu")
+    dmu = float(np.max(np.abs(mu64 - mu32)) / np.max(np.abs(mu64[:, :, 0])))
+    f64 = out_prec["fp64"]
+    print(json.dumps({
+        "metric": METRIC, "value": f64["value"], "unit": "nnz·vec updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": f64["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128",
+        "data": "synthetic: 32 commensurate tBLG cells built in-process, random-phase start vectors (seed 0)",
+        "config": {"workload": "batched uneven solves: 32 tBLG cells (i+1, i), i ~ U{9..31} (seed 1), 8 k-points "
+                               "each, R=16 per k, M=1024, pruned GrapheneSimple H(k), one hipGraph per batch",
+                   "jobs": jobs, "rows": rows, "nnz": nnz, "cells_i": [int(x) for x in idx]},
+        "batched": f64, "mixed": out_prec["mixed"], "max_rel_dmu_mixed": dmu,
+        "one_job_at_a_time": {"value": seq, "kpm_ms": done * 1e3, "speedup_batched": f64["value"] / seq},
+        "assembly_s": t_asm,
+    }), flush=True)
 
 
 def measured_traffic(kernel, nnz, R):

@@ -179,6 +179,51 @@ typedef struct {
 int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu);
 int leo2d_last_timing(const leo2d_handle* h, leo2d_timing* t);
 
+/* ---- batched jobs: many small KPM jobs in one recursion -----------------
+ * The reference farms every (cell, shift, k-point) job to its own MPI worker
+ * (Locality::rootChebSolve / workerChebSolve, locality.cpp:1271-3575), each
+ * running computeDosKPM's recursion (locality.cpp:5945-6136) on its own
+ * matrix.  Here the jobs' Hamiltonians are stacked block-diagonally on the
+ * device and advanced together: one launch per Chebyshev step for all jobs,
+ * the whole recursion optionally captured once in a hipGraph and replayed.
+ * Each job gets R random-phase vectors (vector r of job p is global vector
+ * p*R + r of the LEO2D_START_RANDOM_PHASE generator, orbital index local to
+ * the job) and its moments by the doubling identity from the first step. */
+int leo2d_batch_clear(leo2d_handle* h);
+/* Append the current device Hamiltonian (from leo2d_assemble or
+ * leo2d_csr_upload, in its own row order) as the next job; *job (may be NULL)
+ * receives its index.  All jobs of a batch must be all real or all complex. */
+int leo2d_batch_append(leo2d_handle* h, int32_t* job);
+int leo2d_batch_info(const leo2d_handle* h, int32_t* jobs, int64_t* rows, int64_t* nnz);
+
+#define LEO2D_BATCH_ROW_GROUPS 0    /* 64/R rows per wavefront */
+#define LEO2D_BATCH_ENTRY_SLOTS 1   /* one row per wavefront, 64/R entries in flight */
+
+typedef struct {
+    int32_t M;            /* moments per vector, multiple of 4 */
+    int32_t R;            /* vectors per job: 4, 8, 16 or 32 */
+    uint64_t seed;
+    int32_t precision;    /* LEO2D_PREC_FP64 or LEO2D_PREC_MIXED */
+    int32_t tune;         /* 1: time the candidate launch shapes on this batch and keep the fastest */
+    int32_t graph;        /* 1: capture the whole recursion in a hipGraph and replay it */
+    int32_t mode;         /* LEO2D_BATCH_* when tune = 0 */
+    int32_t tile_rows;    /* rows per tile when tune = 0 (0 = 128) */
+    int32_t grid;         /* workgroups per step when tune = 0 (0 = auto, else a multiple of 8) */
+} leo2d_batch_params;
+
+typedef struct {
+    double recursion_ms;  /* HIP events around the recursion (graph launch or eager steps) */
+    double tune_ms;       /* autotuning time (0 when the cached choice was used) */
+    int32_t mode, tile_rows, grid, graph;   /* launch shape used */
+    int32_t candidates;   /* shapes timed by the autotuner */
+    int32_t jobs, ntiles;
+    int64_t rows, nnz;
+} leo2d_batch_timing;
+
+/* mu: host double[jobs][R][M]. */
+int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu);
+int leo2d_batch_last_timing(const leo2d_handle* h, leo2d_batch_timing* t);
+
 /* ---- DoS: Param_tools::densityTransform (param_tools.cpp:661-705) --------
  * rho[t][k] = scale * DCT-III(g . mu[t])_k / (pi sqrt(a^2 - (E_k - b)^2)),
  * E_k = b + a cos((k + 1/2) pi / M).  mu, E (may be NULL), rho: host. */

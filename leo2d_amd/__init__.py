@@ -37,6 +37,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 SCHEME_REFERENCE = 0
 SCHEME_DOUBLING = 1
+BATCH_ROW_GROUPS = 0
+BATCH_ENTRY_SLOTS = 1
 
 # every symbol include/leo2d.h declares
 EXPORTS = [
@@ -46,7 +48,8 @@ EXPORTS = [
     "leo2d_last_error", "leo2d_set_geometry", "leo2d_pairs_build", "leo2d_pairs_copy",
     "leo2d_assemble", "leo2d_csr_upload", "leo2d_csr_info", "leo2d_csr_copy",
     "leo2d_spectral_bounds", "leo2d_kpm_moments", "leo2d_last_timing", "leo2d_dos",
-    "leo2d_run_deck",
+    "leo2d_run_deck", "leo2d_batch_clear", "leo2d_batch_append", "leo2d_batch_info", "leo2d_batch_kpm",
+    "leo2d_batch_last_timing",
 ]
 
 
@@ -93,6 +96,22 @@ class Timing(C.Structure):
     ]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [
+        ("M", C.c_int32), ("R", C.c_int32), ("seed", C.c_uint64), ("precision", C.c_int32),
+        ("tune", C.c_int32), ("graph", C.c_int32), ("mode", C.c_int32), ("tile_rows", C.c_int32),
+        ("grid", C.c_int32),
+    ]
+
+
+class BatchTiming(C.Structure):
+    _fields_ = [
+        ("recursion_ms", C.c_double), ("tune_ms", C.c_double), ("mode", C.c_int32), ("tile_rows", C.c_int32),
+        ("grid", C.c_int32), ("graph", C.c_int32), ("candidates", C.c_int32), ("jobs", C.c_int32),
+        ("ntiles", C.c_int32), ("rows", C.c_int64), ("nnz", C.c_int64),
+    ]
+
+
 _lib = None
 
 
@@ -134,6 +153,11 @@ def lib() -> C.CDLL:
         L.leo2d_last_timing.argtypes = [vp, C.POINTER(Timing)]
         L.leo2d_dos.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, vp, vp]
         L.leo2d_run_deck.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int32]
+        L.leo2d_batch_clear.argtypes = [vp]
+        L.leo2d_batch_append.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.leo2d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.leo2d_batch_kpm.argtypes = [vp, C.POINTER(BatchParams), vp]
+        L.leo2d_batch_last_timing.argtypes = [vp, C.POINTER(BatchTiming)]
         _lib = L
     return _lib
 
@@ -322,6 +346,38 @@ class Device:
     def timing(self) -> Timing:
         t = Timing()
         lib().leo2d_last_timing(self.h, C.byref(t))
+        return t
+
+    # ---- batched jobs ----
+    def batch_clear(self):
+        self._check(lib().leo2d_batch_clear(self.h))
+
+    def batch_append(self) -> int:
+        """Append the current device Hamiltonian as the next job of the batch."""
+        j = C.c_int32()
+        self._check(lib().leo2d_batch_append(self.h, C.byref(j)))
+        return j.value
+
+    def batch_info(self):
+        jobs, rows, nnz = C.c_int32(), C.c_int64(), C.c_int64()
+        self._check(lib().leo2d_batch_info(self.h, C.byref(jobs), C.byref(rows), C.byref(nnz)))
+        return jobs.value, rows.value, nnz.value
+
+    def batch_kpm(self, M, R, seed=0, precision=PREC_FP64, tune=False, graph=True, mode=BATCH_ROW_GROUPS,
+                  tile_rows=0, grid=0):
+        """Moments mu[jobs][R][M] of every job in the batch (random-phase starts)."""
+        p = BatchParams()
+        p.M, p.R, p.seed, p.precision = M, R, seed, precision
+        p.tune, p.graph, p.mode, p.tile_rows, p.grid = int(tune), int(graph), mode, tile_rows, grid
+        jobs = self.batch_info()[0]
+        mu = np.zeros((jobs, R, M))
+        self.last_rc = lib().leo2d_batch_kpm(self.h, C.byref(p), _p(mu))
+        self._check(self.last_rc)
+        return mu
+
+    def batch_timing(self) -> BatchTiming:
+        t = BatchTiming()
+        lib().leo2d_batch_last_timing(self.h, C.byref(t))
         return t
 
     def dos(self, mu, a, b, scale=1.0):

@@ -30,9 +30,30 @@ struct leo2d_geom {
 
 namespace {
 
+// owning device allocation (freed on release() or destruction; movable, not copyable)
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     void release()
     {
         if (p) (void)hipFree(p);
@@ -134,6 +155,49 @@ struct leo2d_handle {
     } tiles;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
+    // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
+    struct Batch {
+        struct Pending {
+            DevBuf rp, col, val;
+            int n = 0;
+            int64_t nnz = 0;
+        };
+        std::vector<Pending> pending;          // appended, not yet stacked
+        int is_complex = -1;
+        int64_t rows = 0, nnz = 0;             // stacked so far
+        std::vector<int64_t> job_row0{0};      // [jobs + 1]
+        DevBuf rp, col, val, val32, drow0;
+        bool val32_valid = false;
+        int tile_rows = 0, ntiles = 0;          // current tiling
+        std::vector<int32_t> tile_row0, job_tile0;
+        DevBuf dtile_row0, djob_tile0;
+        // autotuned launch shape, per (R, precision)
+        struct Tuned {
+            int R, precision, mode, tile_rows, grid;
+        };
+        std::vector<Tuned> tuned;
+        // captured recursion (valid while the key matches)
+        hipGraphExec_t exec = nullptr;
+        struct GraphKey {
+            int M, R, precision, mode, tile_rows, grid;
+            uint64_t seed;
+            const void *A, *B, *part, *mu, *val;
+            bool operator==(const GraphKey& o) const
+            {
+                return M == o.M && R == o.R && precision == o.precision && mode == o.mode && tile_rows == o.tile_rows &&
+                       grid == o.grid && seed == o.seed && A == o.A && B == o.B && part == o.part && mu == o.mu &&
+                       val == o.val;
+            }
+        } key{};
+        DevBuf A, B, part, mu;
+        leo2d_batch_timing timing{};
+        int jobs() const { return (int)job_row0.size() - 1 + (int)pending.size(); }
+        void drop_graph()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+    } batch;
 
     ~leo2d_handle()
     {
@@ -141,6 +205,7 @@ struct leo2d_handle {
         for (DevBuf* b : {&i2g, &i2p, &lpos, &cnt_intra, &cnt_tmp, &pair_rp, &pcol, &psc, &nslots, &slot_rp, &rp, &col,
                           &val, &val32, &i2pos, &perm, &iperm, &prp, &pcol_k, &pval, &pval32})
             b->release();
+        batch.drop_graph();
         for (auto e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -375,6 +440,82 @@ void build_permuted(leo2d_handle* h)
     h->pval32_valid = false;
 }
 
+
+// ---- batched jobs --------------------------------------------------------------
+// Move every pending job into the stacked arrays (columns and row pointers
+// offset by the rows / entries already stacked).
+void batch_stack(leo2d_handle* h)
+{
+    auto& B = h->batch;
+    if (B.pending.empty()) return;
+    hipStream_t s = h->stream;
+    int64_t rows = B.rows, nnz = B.nnz;
+    for (auto& q : B.pending) {
+        rows += q.n;
+        nnz += q.nnz;
+    }
+    if (rows > 0x7fffffffLL) throw ArgError("batch exceeds 2^31 rows");
+    const size_t vb = B.is_complex ? 16 : 8;
+    DevBuf rp, col, val;
+    dalloc(rp, sizeof(int64_t) * (size_t)(rows + 1), "batch row pointers");
+    dalloc(col, sizeof(int32_t) * (size_t)nnz, "batch columns");
+    dalloc(val, vb * (size_t)nnz, "batch values");
+    if (B.rows > 0) {
+        check(hipMemcpyAsync(rp.p, B.rp.p, sizeof(int64_t) * (size_t)(B.rows + 1), hipMemcpyDeviceToDevice, s), "batch");
+        check(hipMemcpyAsync(col.p, B.col.p, sizeof(int32_t) * (size_t)B.nnz, hipMemcpyDeviceToDevice, s), "batch");
+        check(hipMemcpyAsync(val.p, B.val.p, vb * (size_t)B.nnz, hipMemcpyDeviceToDevice, s), "batch");
+    }
+    for (auto& q : B.pending) {
+        check(leo2d::launch_batch_offset((const int32_t*)q.col.p, q.nnz, (int32_t)B.rows, (int32_t*)col.p + B.nnz,
+                                         (const int64_t*)q.rp.p, q.n, B.nnz, (int64_t*)rp.p + B.rows, s),
+              "batch offsets");
+        if (q.nnz)
+            check(hipMemcpyAsync((char*)val.p + vb * (size_t)B.nnz, q.val.p, vb * (size_t)q.nnz, hipMemcpyDeviceToDevice, s),
+                  "batch values");
+        B.rows += q.n;
+        B.nnz += q.nnz;
+        B.job_row0.push_back(B.rows);
+    }
+    check(hipStreamSynchronize(s), "batch stack");
+    B.pending.clear();
+    B.rp = std::move(rp);
+    B.col = std::move(col);
+    B.val = std::move(val);
+    B.val32_valid = false;
+    to_device(B.drow0, B.job_row0.data(), sizeof(int64_t) * B.job_row0.size(), s, "batch job rows");
+    check(hipStreamSynchronize(s), "batch job rows");
+    B.tile_rows = 0;
+    B.drop_graph();
+}
+
+// Tiles of at most T rows that never cross a job boundary.
+void batch_tiles(leo2d_handle* h, int T)
+{
+    auto& B = h->batch;
+    if (B.tile_rows == T) return;
+    // the old tile arrays may still be read by queued steps or a captured graph
+    check(hipStreamSynchronize(h->stream), "batch tiles");
+    B.drop_graph();
+    const int P = (int)B.job_row0.size() - 1;
+    B.tile_row0.clear();
+    B.job_tile0.assign(1, 0);
+    for (int p = 0; p < P; ++p) {
+        for (int64_t r = B.job_row0[p]; r < B.job_row0[p + 1]; r += T) B.tile_row0.push_back((int32_t)r);
+        B.job_tile0.push_back((int32_t)B.tile_row0.size());
+    }
+    B.tile_row0.push_back((int32_t)B.rows);
+    B.ntiles = (int)B.tile_row0.size() - 1;
+    to_device(B.dtile_row0, B.tile_row0.data(), sizeof(int32_t) * B.tile_row0.size(), h->stream, "batch tiles");
+    to_device(B.djob_tile0, B.job_tile0.data(), sizeof(int32_t) * B.job_tile0.size(), h->stream, "batch tiles");
+    check(hipStreamSynchronize(h->stream), "batch tiles");
+    B.tile_rows = T;
+}
+
+int batch_default_grid(int ntiles)
+{
+    return std::min(2048, std::max(8, (ntiles + 7) / 8 * 8));
+}
+
 }  // namespace
 
 extern "C" {
@@ -537,7 +678,8 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
         h->has_csr = false;
         hipStream_t s = h->stream;
         for (auto& b : h->grids) b.release();
-        h->grids.assign(g.num_sheets, DevBuf());
+        h->grids.clear();
+        h->grids.resize(g.num_sheets);
         leo2d::GeomDev d{};
         d.n = g.n;
         d.num_sheets = g.num_sheets;
@@ -720,10 +862,8 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
         if (pattern == LEO2D_PATTERN_REFERENCE) {
             dalloc(h->rp, sizeof(int64_t) * (n + 1), "row pointers");
             check(hipMemcpyAsync(h->rp.p, h->slot_rp.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s), "rp");
-            h->col = scol;
-            h->val = sval;
-            scol.p = nullptr;
-            sval.p = nullptr;
+            h->col = std::move(scol);
+            h->val = std::move(sval);
             skeep.release();
             set_csr_owned(h, n, h->n_slots, cpx);
         } else {
@@ -1182,6 +1322,276 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
         f << leo2d::cheb_text(geo.deck, jobs, rho_all, M);
         return LEO2D_OK;
     });
+}
+
+// ---- batched jobs ------------------------------------------------------------
+int leo2d_batch_clear(leo2d_handle* h)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        check(hipStreamSynchronize(h->stream), "batch clear");
+        h->batch.drop_graph();
+        h->batch = leo2d_handle::Batch();
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_batch_append(leo2d_handle* h, int32_t* job)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr) throw StateError("no Hamiltonian on the device");
+        auto& B = h->batch;
+        if (B.is_complex >= 0 && B.is_complex != h->is_complex)
+            throw ArgError("a batch holds either real or complex Hamiltonians, not both");
+        leo2d_handle::Batch::Pending q;
+        q.n = h->n;
+        q.nnz = h->nnz;
+        const size_t vb = h->is_complex ? 16 : 8;
+        hipStream_t s = h->stream;
+        dalloc(q.rp, sizeof(int64_t) * (size_t)(q.n + 1), "batch job rows");
+        dalloc(q.col, sizeof(int32_t) * (size_t)q.nnz, "batch job columns");
+        dalloc(q.val, vb * (size_t)q.nnz, "batch job values");
+        check(hipMemcpyAsync(q.rp.p, h->rp.p, sizeof(int64_t) * (size_t)(q.n + 1), hipMemcpyDeviceToDevice, s), "append");
+        if (q.nnz) {
+            check(hipMemcpyAsync(q.col.p, h->col.p, sizeof(int32_t) * (size_t)q.nnz, hipMemcpyDeviceToDevice, s), "append");
+            check(hipMemcpyAsync(q.val.p, h->val.p, vb * (size_t)q.nnz, hipMemcpyDeviceToDevice, s), "append");
+        }
+        check(hipStreamSynchronize(s), "append");
+        B.is_complex = h->is_complex;
+        B.pending.push_back(std::move(q));
+        if (job) *job = B.jobs() - 1;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_batch_info(const leo2d_handle* h, int32_t* jobs, int64_t* rows, int64_t* nnz)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    const auto& B = h->batch;
+    int64_t r = B.rows, z = B.nnz;
+    for (const auto& q : B.pending) {
+        r += q.n;
+        z += q.nnz;
+    }
+    if (jobs) *jobs = B.jobs();
+    if (rows) *rows = r;
+    if (nnz) *nnz = z;
+    return LEO2D_OK;
+}
+
+int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
+{
+    if (!h || !p || !mu) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        auto& B = h->batch;
+        if (B.jobs() == 0) throw StateError("the batch holds no jobs");
+        const int M = p->M, R = p->R;
+        if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4");
+        if (R != 4 && R != 8 && R != 16 && R != 32) throw ArgError("R must be 4, 8, 16 or 32");
+        if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
+        if (p->mode != LEO2D_BATCH_ROW_GROUPS && p->mode != LEO2D_BATCH_ENTRY_SLOTS) throw ArgError("unknown mode");
+        constexpr int kMinTile = 16;
+        if (p->tile_rows != 0 && p->tile_rows < kMinTile) throw ArgError("tile_rows must be 0 or at least 16");
+        if (p->grid < 0 || p->grid % 8 != 0) throw ArgError("grid must be 0 or a positive multiple of 8");
+        batch_stack(h);
+        hipStream_t s = h->stream;
+        const int P = (int)B.job_row0.size() - 1;
+        const bool mixed = p->precision == LEO2D_PREC_MIXED;
+        const int mat_type = B.is_complex ? (mixed ? leo2d::MAT_CPX_F32 : leo2d::MAT_CPX_F64)
+                                          : (mixed ? leo2d::MAT_REAL_F32 : leo2d::MAT_REAL_F64);
+        const int vec_type = mixed ? leo2d::VEC_CPX_F32 : leo2d::VEC_CPX_F64;
+        const void* dval = B.val.p;
+        if (mixed) {
+            if (!B.val32_valid) {
+                const size_t count = (size_t)B.nnz * (B.is_complex ? 2 : 1);
+                dalloc(B.val32, sizeof(float) * count, "batch fp32 values");
+                check(leo2d::launch_convert_f64_to_f32((const double*)B.val.p, (float*)B.val32.p, (int64_t)count, s),
+                      "batch fp32 values");
+                B.val32_valid = true;
+                B.drop_graph();
+            }
+            dval = B.val32.p;
+        }
+        // device buffers, kept between calls so a captured graph stays valid
+        const size_t vbytes = leo2d::vec_bytes(vec_type) * (size_t)B.rows * R;
+        auto ensure = [&](DevBuf& b, size_t bytes, const char* what) {
+            if (b.bytes != std::max<size_t>(bytes, 16)) {
+                B.drop_graph();
+                dalloc(b, bytes, what);
+            }
+        };
+        int64_t max_tiles = 0;   // tiles of the shortest allowed height bound every tiling's count
+        for (int q = 0; q < P; ++q) max_tiles += (B.job_row0[q + 1] - B.job_row0[q] + kMinTile - 1) / kMinTile;
+        ensure(B.A, vbytes, "batch vectors");
+        ensure(B.B, vbytes, "batch vectors");
+        ensure(B.part, sizeof(double) * 2 * (size_t)std::max<int64_t>(max_tiles, 1) * R, "batch partials");
+        ensure(B.mu, sizeof(double) * (size_t)P * R * M, "batch moments");
+
+        leo2d::BatchStepArgs sa{};
+        sa.row_ptr = (const int64_t*)B.rp.p;
+        sa.col = (const int32_t*)B.col.p;
+        sa.val = dval;
+        sa.R = R;
+        sa.part = (double*)B.part.p;
+        auto set_shape = [&](int mode, int T, int grid) {
+            batch_tiles(h, T);
+            sa.mode = mode;
+            sa.tile_row0 = (const int32_t*)B.dtile_row0.p;
+            sa.ntiles = B.ntiles;
+            sa.grid = grid > 0 ? grid : batch_default_grid(B.ntiles);
+        };
+        auto step = [&](void* cur, void* out, int j) {
+            sa.cur = cur;
+            sa.out = out;
+            sa.alpha = j == 0 ? 1.0 : 2.0;
+            sa.beta = j == 0 ? 0 : 1;
+            check(leo2d::launch_batch_step(mat_type, vec_type, sa, s), "batch step");
+        };
+        hipEvent_t e0 = event_at(h, 0), e1 = event_at(h, 1);
+
+        // launch shape: given, cached from an earlier tuning, or tuned now
+        int mode = p->mode, T = p->tile_rows > 0 ? p->tile_rows : 128, grid = p->grid;
+        double tune_ms = 0.0;
+        int candidates = 0;
+        if (p->tune) {
+            const leo2d_handle::Batch::Tuned* hit = nullptr;
+            for (const auto& t : B.tuned)
+                if (t.R == R && t.precision == p->precision) hit = &t;
+            if (hit) {
+                mode = hit->mode;
+                T = hit->tile_rows;
+                grid = hit->grid;
+            } else {
+                // time a few Chebyshev steps for every candidate shape on this batch
+                check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, s),
+                      "batch init");
+                set_shape(0, 128, 0);
+                step(B.A.p, B.B.p, 0);
+                check(hipEventRecord(e0, s), "event");
+                float best = 1e30f;
+                for (int tt : {64, 128, 256, 512}) {
+                    for (int md : {0, 1}) {
+                        int last = -1;
+                        for (int per_cu : {2, 4, 8}) {
+                            const int g = std::min(batch_default_grid((int)((B.rows + tt - 1) / tt) + P), 256 * per_cu);
+                            if (g == last) continue;
+                            last = g;
+                            set_shape(md, tt, g);
+                            step(B.A.p, B.B.p, 1);
+                            check(hipEventRecord(e1, s), "event");
+                            for (int k = 0; k < 4; ++k) step(k % 2 ? B.B.p : B.A.p, k % 2 ? B.A.p : B.B.p, 1);
+                            hipEvent_t e2 = event_at(h, 2);
+                            check(hipEventRecord(e2, s), "event");
+                            check(hipEventSynchronize(e2), "tune");
+                            float ms = 0.f;
+                            check(hipEventElapsedTime(&ms, e1, e2), "event");
+                            ++candidates;
+                            if (ms < best) {
+                                best = ms;
+                                mode = md;
+                                T = tt;
+                                grid = sa.grid;
+                            }
+                        }
+                    }
+                }
+                hipEvent_t e3 = event_at(h, 3);
+                check(hipEventRecord(e3, s), "event");
+                check(hipEventSynchronize(e3), "tune");
+                float tms = 0.f;
+                check(hipEventElapsedTime(&tms, e0, e3), "event");
+                tune_ms = tms;
+                B.tuned.push_back({R, p->precision, mode, T, grid});
+            }
+        }
+        set_shape(mode, T, grid);
+        grid = sa.grid;
+
+        auto record = [&]() {
+            check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, s),
+                  "batch init");
+            for (int j = 0; j < M / 2; ++j) {
+                void* cur = (j % 2 == 0) ? B.A.p : B.B.p;
+                void* other = (j % 2 == 0) ? B.B.p : B.A.p;
+                step(cur, other, j);
+                check(leo2d::launch_batch_reduce((const double*)B.part.p, (const int32_t*)B.djob_tile0.p, P, R, M, j,
+                                                 (double*)B.mu.p, s),
+                      "batch reduce");
+            }
+        };
+        if (p->graph) {
+            const leo2d_handle::Batch::GraphKey key{M, R, p->precision, mode, T, grid, p->seed,
+                                                     B.A.p, B.B.p, B.part.p, B.mu.p, dval};
+            if (!B.exec || !(key == B.key)) {
+                B.drop_graph();
+                hipGraph_t graph = nullptr;
+                check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
+                try {
+                    record();
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                check(hipStreamEndCapture(s, &graph), "graph capture");
+                const hipError_t ie = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    B.exec = nullptr;
+                    check(ie, "graph instantiate");
+                }
+                B.key = key;
+            }
+            check(hipEventRecord(e0, s), "event");
+            check(hipGraphLaunch(B.exec, s), "graph launch");
+            check(hipEventRecord(e1, s), "event");
+        } else {
+            check(hipEventRecord(e0, s), "event");
+            record();
+            check(hipEventRecord(e1, s), "event");
+        }
+        check(hipStreamSynchronize(s), "batch recursion");
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, e0, e1), "event");
+        check(hipMemcpy(mu, B.mu.p, sizeof(double) * (size_t)P * R * M, hipMemcpyDeviceToHost), "batch moments");
+
+        auto& t = B.timing;
+        t.recursion_ms = ms;
+        t.tune_ms = tune_ms;
+        t.mode = mode;
+        t.tile_rows = T;
+        t.grid = grid;
+        t.graph = p->graph ? 1 : 0;
+        t.candidates = candidates;
+        t.jobs = P;
+        t.ntiles = B.ntiles;
+        t.rows = B.rows;
+        t.nnz = B.nnz;
+        for (int q = 0; q < P * R; ++q) {
+            const double* m = mu + (size_t)q * M;
+            const double lim = std::fabs(m[0]) * (1.0 + 1e-6) + 1e-12;
+            for (int j = 0; j < M; ++j) {
+                if (!std::isfinite(m[j])) {
+                    h->err = "non-finite moment (job " + std::to_string(q / R) + ", vector " + std::to_string(q % R) + ")";
+                    return LEO2D_ERR_NAN;
+                }
+                if (std::fabs(m[j]) > lim) {
+                    h->err = "job " + std::to_string(q / R) + ": |mu_" + std::to_string(j) +
+                             "| > mu_0: rescaled spectrum leaves [-1, 1]";
+                    return LEO2D_ERR_SPECTRUM;
+                }
+            }
+        }
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_batch_last_timing(const leo2d_handle* h, leo2d_batch_timing* t)
+{
+    if (!h || !t) return LEO2D_ERR_ARG;
+    *t = h->batch.timing;
+    return LEO2D_OK;
 }
 
 }  // extern "C"

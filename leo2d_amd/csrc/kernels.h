@@ -80,6 +80,35 @@ struct ReduceArgs {
 };
 hipError_t launch_kpm_reduce(const ReduceArgs& a, hipStream_t s);
 
+// ---- batched recursion over many stacked Hamiltonians (batch.hip) ----------
+// One step for every job of a block-diagonal stack: out = alpha*H*cur - (beta ?
+// out : 0) with per-tile partials part[t][r][0] = sum |cur|^2,
+// part[t][r][1] = sum Re(conj(out) cur) over the rows of tile t.
+struct BatchStepArgs {
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const void* val;
+    const void* cur;
+    void* out;                 // T_{j-1} in, T_{j+1} out
+    double alpha;
+    int beta;
+    int R;                     // vectors per job: 4, 8, 16 or 32
+    const int32_t* tile_row0;  // [ntiles + 1]; tiles never cross a job boundary
+    int ntiles;
+    double* part;              // [ntiles][R][2]
+    int grid;                  // workgroups (multiple of 8)
+    int mode;                  // 0 = row groups, 1 = entry slots
+};
+hipError_t launch_batch_step(int mat_type, int vec_type, const BatchStepArgs& a, hipStream_t s);
+// mu[p][r][M] of job p from its tiles [job_tile0[p], job_tile0[p+1]) (doubling identity)
+hipError_t launch_batch_reduce(const double* part, const int32_t* job_tile0, int P, int R, int M, int j, double* mu,
+                               hipStream_t s);
+// random-phase start vectors: vector r of job p = global vector p*R + r
+hipError_t launch_batch_init(void* x, int vec_type, const int64_t* job_row0, int P, int64_t nrows, int R, uint64_t seed,
+                             hipStream_t s);
+hipError_t launch_batch_offset(const int32_t* col_in, int64_t nnz, int32_t row_off, int32_t* col_out,
+                               const int64_t* rp_in, int n, int64_t nnz_off, int64_t* rp_out, hipStream_t s);
+
 // start vectors
 hipError_t launch_init_targets(void* x, int vec_type, int n, int R, const int32_t* targets, hipStream_t s);
 hipError_t launch_init_random_phase(void* x, int vec_type, int n, int R, uint64_t seed, int r0, hipStream_t s);

@@ -28,99 +28,10 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "kpm_common.h"
 
 namespace leo2d {
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-
-// ---- element arithmetic ------------------------------------------------
-struct cf32 { float x, y; };
-
-template <typename X> struct acc_of;
-template <> struct acc_of<double> { using type = double; };
-template <> struct acc_of<double2> { using type = double2; };
-template <> struct acc_of<float2> { using type = float2; };
-template <> struct acc_of<float> { using type = float; };
-
-__device__ inline void zero(double& a) { a = 0.0; }
-__device__ inline void zero(double2& a) { a.x = 0.0; a.y = 0.0; }
-__device__ inline void zero(float2& a) { a.x = 0.0f; a.y = 0.0f; }
-__device__ inline void zero(float& a) { a = 0.0f; }
-
-// acc += h * x
-__device__ inline void mac(double& acc, double h, double x) { acc = fma(h, x, acc); }
-__device__ inline void mac(double2& acc, double h, double2 x)
-{
-    acc.x = fma(h, x.x, acc.x);
-    acc.y = fma(h, x.y, acc.y);
-}
-__device__ inline void mac(double2& acc, double2 h, double2 x)
-{
-    acc.x = fma(h.x, x.x, acc.x);
-    acc.x = fma(-h.y, x.y, acc.x);
-    acc.y = fma(h.x, x.y, acc.y);
-    acc.y = fma(h.y, x.x, acc.y);
-}
-__device__ inline void mac(float& acc, float h, float x) { acc = fmaf(h, x, acc); }
-__device__ inline void mac(float2& acc, float h, float2 x)
-{
-    acc.x = fmaf(h, x.x, acc.x);
-    acc.y = fmaf(h, x.y, acc.y);
-}
-__device__ inline void mac(float2& acc, float2 h, float2 x)
-{
-    acc.x = fmaf(h.x, x.x, acc.x);
-    acc.x = fmaf(-h.y, x.y, acc.x);
-    acc.y = fmaf(h.x, x.y, acc.y);
-    acc.y = fmaf(h.y, x.x, acc.y);
-}
-
-// out = alpha * acc - prev  (alpha is 1 or 2, exact)
-__device__ inline double comb(double alpha, double acc, double prev) { return alpha * acc - prev; }
-__device__ inline double2 comb(double alpha, double2 acc, double2 prev)
-{
-    double2 o;
-    o.x = alpha * acc.x - prev.x;
-    o.y = alpha * acc.y - prev.y;
-    return o;
-}
-__device__ inline float2 comb(double alpha, float2 acc, float2 prev)
-{
-    float al = (float)alpha;
-    float2 o;
-    o.x = al * acc.x - prev.x;
-    o.y = al * acc.y - prev.y;
-    return o;
-}
-__device__ inline float comb(double alpha, float acc, float prev) { return (float)alpha * acc - prev; }
-
-// FP64 moment terms
-__device__ inline double nrm2(double a) { return a * a; }
-__device__ inline double nrm2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
-__device__ inline double nrm2(float2 a)
-{
-    double x = a.x, y = a.y;
-    return fma(x, x, y * y);
-}
-__device__ inline double nrm2(float a)
-{
-    double x = a;
-    return x * x;
-}
-__device__ inline double redot(double a, double b) { return a * b; }
-__device__ inline double redot(double2 a, double2 b) { return fma(a.x, b.x, a.y * b.y); }
-__device__ inline double redot(float2 a, float2 b)
-{
-    double ax = a.x, ay = a.y, bx = b.x, by = b.y;
-    return fma(ax, bx, ay * by);
-}
-__device__ inline double redot(float a, float b) { return (double)a * (double)b; }
-__device__ inline double re(double a) { return a; }
-__device__ inline double re(double2 a) { return a.x; }
-__device__ inline double re(float2 a) { return a.x; }
-__device__ inline double re(float a) { return a; }
 
 // ---- the fused step --------------------------------------------------------
 // LPR: lanes per row (= min(R, 64)); VPL: vectors per lane (R / 64 when R > 64).
@@ -1139,22 +1050,13 @@ __global__ __launch_bounds__(kThreads) void kpm_reduce_kernel(ReduceArgs a)
 }
 
 // ---- start vectors ---------------------------------------------------------
-__device__ inline uint64_t mix64(uint64_t z)
-{
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
 
 __global__ void init_random_phase_kernel(void* x, int vec_type, int n, int R, uint64_t seed, int r0)
 {
     const size_t total = (size_t)n * R;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const uint64_t i = t / R, r = t % R;
-        uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
-        h = mix64(h ^ ((r0 + r) * 0xD1B54A32D192ED03ULL));
-        h = mix64(h ^ (i * 0x8CB92BA72F3D8DD7ULL));
-        const double phi = (double)(h >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586;
+        const double phi = random_phase_angle(seed, r0 + r, i);
         double sn, cs;
         sincos(phi, &sn, &cs);
         if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);
@@ -1236,10 +1138,7 @@ __global__ void init_random_phase_perm_kernel(void* x, int vec_type, int n, int 
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const uint64_t q = t / R, r = t % R;
         const uint64_t i = iperm ? (uint64_t)iperm[q] : q;
-        uint64_t h = mix64(seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL);
-        h = mix64(h ^ ((r0 + r) * 0xD1B54A32D192ED03ULL));
-        h = mix64(h ^ (i * 0x8CB92BA72F3D8DD7ULL));
-        const double phi = (double)(h >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586;
+        const double phi = random_phase_angle(seed, r0 + r, i);
         double sn, cs;
         sincos(phi, &sn, &cs);
         if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);

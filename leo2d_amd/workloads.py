@@ -1,0 +1,79 @@
+"""Decks and job preparation for commensurate twisted-bilayer-graphene cells.
+
+These are the synthetic workloads the KPM path is measured and tested on:
+an (m, n) commensurate tBLG cell (Locality::setupSupercell type 1,
+src/locality.cpp:131-315) optionally tiled, with the GrapheneSimple coupling
+model (nearest-neighbour t = -2.7 eV, t_perp = 0.48 exp(-(r - 3.35)/0.45) eV
+for r < 6 A), a complex Bloch Hamiltonian H(k) and a Gershgorin rescale with
+1 % padding.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import leo2d_amd as L
+
+
+def cell_orbitals(i: int) -> int:
+    """Orbitals of the (i+1, i) commensurate cell: 4 (3 i^2 + 3 i + 1)."""
+    return 4 * (3 * i * i + 3 * i + 1)
+
+
+def tblg_cell_deck(m: int, n: int, tiles: int = 1, M: int = 1024, name: str = "tblg_cell") -> str:
+    """Periodic two-sheet deck of the (m, n) cell tiled tiles x tiles
+    (SUPERCELL_M_N = tiles*m tiles*n).  The orbital grid [-S, S) with
+    S = 2 max(m, n) + min(m, n) + 1 (in tiled units) covers the cell with the
+    centre cell (0, 0) inside it."""
+    mm, nn = tiles * m, tiles * n
+    S = 2 * max(mm, nn) + min(mm, nn) + 1
+    return f"""JOB_NAME = {name}
+MAXSIZE = {S}
+MINSIZE = -{S}
+BOUNDARY_CONDITION = 1
+SUPERCELL_M_N = {mm} {nn}
+NUM_SHEETS = 2
+START_SHEET 1
+MATERIAL = GrapheneSimple
+HEIGHT = 0
+ANGLE = 0
+END_SHEET 1
+START_SHEET 2
+MATERIAL = GrapheneSimple
+HEIGHT = 3.35
+ANGLE = 0
+END_SHEET 2
+SOLVER_TYPE = SQ
+NUM_TARGET_SHEETS = 1
+TARGET_SHEETS = 1
+NUM_SHIFT_SHEETS = 1
+SHIFT_SHEETS = 2
+NSHIFTS = 1
+ENERGY_RESCALE = 1
+ENERGY_SHIFT = 0.0
+POLY_ORDER = {M}
+"""
+
+
+def reciprocal(geom) -> np.ndarray:
+    """Reciprocal vectors b1, b2 (rows, 1/A) of the geometry's supercell."""
+    a = np.array(geom.info.supercell)
+    A3 = np.array([[a[0, 0], a[0, 1], 0.0], [a[1, 0], a[1, 1], 0.0], [0.0, 0.0, 1.0]])
+    return (2 * np.pi * np.linalg.inv(A3).T)[:2, :2]
+
+
+def assemble_rescaled(dev, geom, k_frac, pattern=L.PATTERN_PRUNED, pad: float = 1.01):
+    """Fill H(k) for k = k_frac[0] b1 + k_frac[1] b2 and rescale it into [-1, 1]:
+    a = pad (hi - lo)/2, b = (hi + lo)/2 from the Gershgorin interval of the
+    unscaled matrix.  Returns (job, a, b, nnz)."""
+    job = geom.jobs()[0]
+    k = k_frac[0] * reciprocal(geom)[0] + k_frac[1] * reciprocal(geom)[1]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = float(k[0]), float(k[1])
+    job.energy_rescale, job.energy_shift = 1.0, 0.0
+    dev.assemble(job, pattern)
+    lo, hi = dev.spectral_bounds()
+    a = pad * (hi - lo) / 2.0
+    b = (hi + lo) / 2.0
+    job.energy_rescale, job.energy_shift = a, -b
+    nnz = dev.assemble(job, pattern)
+    return job, a, b, nnz

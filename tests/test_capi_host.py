@@ -82,6 +82,16 @@ def test_commensurate_cell_count():
     assert g.n == 4 * (24 * 24 + 24 * 28 + 28 * 28) == 8128
 
 
+@pytest.mark.parametrize("i,tiles", [(2, 1), (3, 1), (9, 1), (6, 2)])
+def test_workload_cell_decks(i, tiles):
+    """The generated (i+1, i) cell decks hold 4 (3 i^2 + 3 i + 1) tiles^2 orbitals
+    (the commensurate-cell count), split evenly over the two sheets."""
+    from leo2d_amd import workloads as W
+    g = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, tiles=tiles)))
+    assert g.n == W.cell_orbitals(i) * tiles * tiles
+    assert g.info.sheet_count[0] == g.info.sheet_count[1] == g.n // 2
+
+
 @pytest.mark.parametrize("edit,needle", [
     (("POLY_ORDER = 200", "POLY_ORDER = 202"), "divisible 4"),
     (("MATERIAL = Graphene", "MATERIAL = Unobtainium"), "material"),

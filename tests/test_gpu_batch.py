@@ -1,0 +1,140 @@
+"""Batched jobs (many stacked Hamiltonians in one recursion) against the oracle.
+
+Each job is one (cell, k-point) Hamiltonian with its own R random-phase vectors
+(global vector index p*R + r).  Bars: FP64 moments |dmu| <= 1e-12 mu_0 per job
+(summation order only); FP32 recursion |dmu| <= 2e-5 mu_0; graph replays equal
+the eager launches bit for bit (same kernels, same order).
+"""
+import numpy as np
+import pytest
+
+import leo2d_amd as L
+from leo2d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CELLS = (2, 3, 5)          # (i+1, i) cells: 76, 148, 364 orbitals
+KFRAC = ((0.31, 0.12), (0.77, 0.58))
+R, M, SEED = 16, 64, 7
+
+
+@pytest.fixture(scope="module")
+def batch(oracle_mod):
+    dev = L.Device(0)
+    jobs = []
+    for i in CELLS:
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=M)))
+        assert geom.n == W.cell_orbitals(i)
+        dev.set_geometry(geom)
+        dev.pairs_build()
+        for kf in KFRAC:
+            W.assemble_rescaled(dev, geom, kf)
+            rp, ci, v = dev.csr_copy()
+            p = dev.batch_append()
+            vecs = oracle_mod.random_phase(geom.n, R, SEED, r0=p * R)
+            omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+            jobs.append((geom.n, omu))
+    yield dev, jobs
+    dev.close()
+
+
+@pytest.mark.parametrize("mode", [L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS])
+@pytest.mark.parametrize("tile_rows", [64, 100, 256])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_fp64_matches_oracle(batch, mode, tile_rows, graph):
+    dev, jobs = batch
+    print("case mode", mode, "tile_rows", tile_rows, "graph", graph, flush=True)
+    mu = dev.batch_kpm(M, R, seed=SEED, mode=mode, tile_rows=tile_rows, graph=graph)
+    assert mu.shape == (len(jobs), R, M)
+    for p, (n, omu) in enumerate(jobs):
+        assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (p, mode, tile_rows)
+    t = dev.batch_timing()
+    assert t.jobs == len(jobs) and t.rows == sum(n for n, _ in jobs)
+    assert t.mode == mode and t.tile_rows == tile_rows and t.graph == int(graph)
+
+
+@pytest.mark.parametrize("Rv", [4, 8, 32])
+def test_batch_other_block_widths(oracle_mod, Rv):
+    """R = 4, 8, 32 vectors per job on two jobs of different size."""
+    dev = L.Device(0)
+    try:
+        want = []
+        for i in (2, 4):
+            geom = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=32)))
+            dev.set_geometry(geom)
+            dev.pairs_build()
+            W.assemble_rescaled(dev, geom, (0.2, 0.45))
+            rp, ci, v = dev.csr_copy()
+            p = dev.batch_append()
+            vecs = oracle_mod.random_phase(geom.n, Rv, 11, r0=p * Rv)
+            want.append((geom.n, oracle_mod.kpm(rp, ci, v, 32, vecs=vecs, doubling_all=True, nthreads=8)))
+        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS):
+            mu = dev.batch_kpm(32, Rv, seed=11, mode=mode, graph=False)
+            for p, (n, omu) in enumerate(want):
+                assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (Rv, mode, p)
+    finally:
+        dev.close()
+
+
+def test_batch_mixed_precision(batch):
+    dev, jobs = batch
+    mu = dev.batch_kpm(M, R, seed=SEED, precision=L.PREC_MIXED, graph=True)
+    for p, (n, omu) in enumerate(jobs):
+        assert np.max(np.abs(mu[p] - omu)) <= 2e-5 * n, p
+
+
+def test_batch_autotune_and_cache(batch):
+    dev, jobs = batch
+    mu = dev.batch_kpm(M, R, seed=SEED, tune=True)
+    t = dev.batch_timing()
+    assert t.candidates > 0 and t.tune_ms > 0
+    for p, (n, omu) in enumerate(jobs):
+        assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n
+    mu2 = dev.batch_kpm(M, R, seed=SEED, tune=True)
+    t2 = dev.batch_timing()
+    assert t2.candidates == 0 and (t2.mode, t2.tile_rows, t2.grid) == (t.mode, t.tile_rows, t.grid)
+    assert np.array_equal(mu, mu2)
+
+
+def test_batch_graph_replays_match_eager(batch):
+    """The captured recursion is replayed several times with other device work
+    in between; every replay equals the eager launches bit for bit."""
+    dev, jobs = batch
+    eager = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_ENTRY_SLOTS, tile_rows=128, graph=False)
+    for rep in range(4):
+        g = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_ENTRY_SLOTS, tile_rows=128, graph=True)
+        assert np.array_equal(g, eager), rep
+        dev.kpm(16, random_phase=(4, rep), scheme=L.SCHEME_DOUBLING)   # unrelated eager work on the same stream
+
+
+def test_batch_append_after_run_and_clear(oracle_mod):
+    dev = L.Device(0)
+    try:
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(4, 3, M=32)))
+        dev.set_geometry(geom)
+        dev.pairs_build()
+        W.assemble_rescaled(dev, geom, (0.1, 0.9))
+        rp, ci, v = dev.csr_copy()
+        assert dev.batch_append() == 0
+        dev.batch_kpm(32, 8, seed=2, graph=True)
+        assert dev.batch_append() == 1                  # the same H again, stacked after the first run
+        mu = dev.batch_kpm(32, 8, seed=2, graph=True)
+        for p in range(2):
+            omu = oracle_mod.kpm(rp, ci, v, 32, vecs=oracle_mod.random_phase(geom.n, 8, 2, r0=p * 8),
+                                 doubling_all=True, nthreads=8)
+            assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * geom.n
+        dev.batch_clear()
+        assert dev.batch_info() == (0, 0, 0)
+        with pytest.raises(L.Leo2dError):
+            dev.batch_kpm(32, 8)
+    finally:
+        dev.close()
+
+
+def test_batch_argument_errors(batch):
+    dev, _ = batch
+    for kw in ({"R": 3}, {"R": 64}, {"M": 30}, {"grid": 12}, {"tile_rows": 8}):
+        args = {"M": M, "R": R}
+        args.update(kw)
+        with pytest.raises(L.Leo2dError):
+            dev.batch_kpm(args.pop("M"), args.pop("R"), **args)
