@@ -186,9 +186,11 @@ int leo2d_last_timing(const leo2d_handle* h, leo2d_timing* t);
  * matrix.  Here the jobs' Hamiltonians are stacked block-diagonally on the
  * device and advanced together: one launch per Chebyshev step for all jobs,
  * the whole recursion optionally captured once in a hipGraph and replayed.
- * Each job gets R random-phase vectors (vector r of job p is global vector
- * p*R + r of the LEO2D_START_RANDOM_PHASE generator, orbital index local to
- * the job) and its moments by the doubling identity from the first step. */
+ * Each job gets R start vectors: random phases (vector r of job p is global
+ * vector p*R + r of the LEO2D_START_RANDOM_PHASE generator, orbital index
+ * local to the job) or unit vectors on R of its orbitals (local DoS, as
+ * computeDosKPM's targets), and its moments by the doubling identity from the
+ * first step. */
 int leo2d_batch_clear(leo2d_handle* h);
 /* Append the current device Hamiltonian (from leo2d_assemble or
  * leo2d_csr_upload, in its own row order) as the next job; *job (may be NULL)
@@ -202,6 +204,8 @@ int leo2d_batch_info(const leo2d_handle* h, int32_t* jobs, int64_t* rows, int64_
 typedef struct {
     int32_t M;            /* moments per vector, multiple of 4 */
     int32_t R;            /* vectors per job: 4, 8, 16 or 32 */
+    int32_t start;        /* LEO2D_START_RANDOM_PHASE, or LEO2D_START_TARGETS (local DoS of R orbitals per job) */
+    const int32_t* targets;   /* [jobs][R] orbital indices inside each job, for LEO2D_START_TARGETS */
     uint64_t seed;
     int32_t precision;    /* LEO2D_PREC_FP64 or LEO2D_PREC_MIXED */
     int32_t tune;         /* 1: time the candidate launch shapes on this batch and keep the fastest */
@@ -235,6 +239,51 @@ int leo2d_dos(leo2d_handle* h, const double* mu, int32_t T, int32_t M, double a,
  * when MATRIX_SAVE > 0, <out_prefix>_matrix.dat (locality.cpp:4366-4383,
  * 5213-5228).  out_prefix NULL = the deck's JOB_NAME. */
 int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefix, int32_t pattern);
+
+/* ---- learned surrogate: local stacking configuration -> local moments -----
+ * No counterpart in the reference.  A multilayer perceptron (Fourier features
+ * -> hidden GELU layers -> outputs) with bfloat16 weights and activations and
+ * FP32 accumulation on the matrix cores, trained on the GPU (Adam, FP32 master
+ * weights) on local moments that leo2d_batch_kpm computes.  A configuration
+ * is (b1, b2, exx, eyy, exy, theta): the shift of sheet 2 in fractional
+ * lattice units, the symmetric strain tensor and the twist in degrees. */
+#define LEO2D_MLP_FEATURES 64
+#define LEO2D_CONFIG_DIM 6
+
+typedef struct leo2d_mlp leo2d_mlp;
+typedef struct {
+    int32_t in;        /* LEO2D_MLP_FEATURES */
+    int32_t hidden;    /* units per hidden layer */
+    int32_t layers;    /* hidden layers */
+    int32_t out;       /* outputs (moments x sublattices) */
+    uint64_t seed;     /* Glorot-uniform initialisation */
+} leo2d_mlp_desc;
+
+typedef struct {
+    int32_t epochs, batch;
+    float lr, beta1, beta2, eps;   /* Adam */
+    uint64_t seed;                 /* shuffling */
+} leo2d_mlp_train_params;
+
+/* feat[n][64] from cfg[n][6] (host) */
+int leo2d_mlp_features(const double* cfg, int64_t n, float* feat);
+int leo2d_mlp_create(int32_t device, const leo2d_mlp_desc* d, leo2d_mlp** out);
+void leo2d_mlp_destroy(leo2d_mlp* m);
+const char* leo2d_mlp_last_error(const leo2d_mlp* m);
+/* Flat parameter vector: per layer l, W_l[d_{l+1}][d_l] row-major then b_l[d_{l+1}]. */
+int64_t leo2d_mlp_num_params(const leo2d_mlp* m);
+int leo2d_mlp_get_params(leo2d_mlp* m, float* p);
+int leo2d_mlp_set_params(leo2d_mlp* m, const float* p);   /* also resets the optimiser */
+/* y[n][out] = network(x[n][in]) (host buffers) */
+int leo2d_mlp_forward(leo2d_mlp* m, const float* x, int64_t n, float* y);
+/* Mean squared error over the n x out outputs and its gradient (flat, as the parameters). */
+int leo2d_mlp_gradients(leo2d_mlp* m, const float* x, const float* t, int32_t n, double* loss, float* grad);
+/* Adam over shuffled mini-batches; epoch_loss[e] = mean squared error of epoch e (may be NULL). */
+int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, const leo2d_mlp_train_params* p,
+                    double* epoch_loss);
+/* Ensemble of count networks of one shape on one device: mean[n][out] (may be NULL) and
+ * spread[n] = max over outputs of the ensemble standard deviation. */
+int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, float* mean, float* spread);
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,12 @@ EXPORTS = [
     "leo2d_assemble", "leo2d_csr_upload", "leo2d_csr_info", "leo2d_csr_copy",
     "leo2d_spectral_bounds", "leo2d_kpm_moments", "leo2d_last_timing", "leo2d_dos",
     "leo2d_run_deck", "leo2d_batch_clear", "leo2d_batch_append", "leo2d_batch_info", "leo2d_batch_kpm",
-    "leo2d_batch_last_timing",
+    "leo2d_batch_last_timing", "leo2d_mlp_features", "leo2d_mlp_create", "leo2d_mlp_destroy",
+    "leo2d_mlp_last_error", "leo2d_mlp_num_params", "leo2d_mlp_get_params", "leo2d_mlp_set_params",
+    "leo2d_mlp_forward", "leo2d_mlp_gradients", "leo2d_mlp_train", "leo2d_mlp_ensemble",
 ]
+MLP_FEATURES = 64
+CONFIG_DIM = 6
 
 
 class Leo2dError(RuntimeError):
@@ -98,7 +102,8 @@ class Timing(C.Structure):
 
 class BatchParams(C.Structure):
     _fields_ = [
-        ("M", C.c_int32), ("R", C.c_int32), ("seed", C.c_uint64), ("precision", C.c_int32),
+        ("M", C.c_int32), ("R", C.c_int32), ("start", C.c_int32), ("targets", C.POINTER(C.c_int32)),
+        ("seed", C.c_uint64), ("precision", C.c_int32),
         ("tune", C.c_int32), ("graph", C.c_int32), ("mode", C.c_int32), ("tile_rows", C.c_int32),
         ("grid", C.c_int32),
     ]
@@ -110,6 +115,16 @@ class BatchTiming(C.Structure):
         ("grid", C.c_int32), ("graph", C.c_int32), ("candidates", C.c_int32), ("jobs", C.c_int32),
         ("ntiles", C.c_int32), ("rows", C.c_int64), ("nnz", C.c_int64),
     ]
+
+
+class MlpDesc(C.Structure):
+    _fields_ = [("in_", C.c_int32), ("hidden", C.c_int32), ("layers", C.c_int32), ("out", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class MlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
 
 
 _lib = None
@@ -153,6 +168,20 @@ def lib() -> C.CDLL:
         L.leo2d_last_timing.argtypes = [vp, C.POINTER(Timing)]
         L.leo2d_dos.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, vp, vp]
         L.leo2d_run_deck.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int32]
+        L.leo2d_mlp_features.argtypes = [vp, C.c_int64, vp]
+        L.leo2d_mlp_create.argtypes = [C.c_int32, C.POINTER(MlpDesc), C.POINTER(vp)]
+        L.leo2d_mlp_destroy.argtypes = [vp]
+        L.leo2d_mlp_destroy.restype = None
+        L.leo2d_mlp_last_error.argtypes = [vp]
+        L.leo2d_mlp_last_error.restype = C.c_char_p
+        L.leo2d_mlp_num_params.argtypes = [vp]
+        L.leo2d_mlp_num_params.restype = C.c_int64
+        L.leo2d_mlp_get_params.argtypes = [vp, vp]
+        L.leo2d_mlp_set_params.argtypes = [vp, vp]
+        L.leo2d_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
+        L.leo2d_mlp_gradients.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]
+        L.leo2d_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(MlpTrainParams), vp]
+        L.leo2d_mlp_ensemble.argtypes = [C.POINTER(vp), C.c_int32, vp, C.c_int64, vp, vp]
         L.leo2d_batch_clear.argtypes = [vp]
         L.leo2d_batch_append.argtypes = [vp, C.POINTER(C.c_int32)]
         L.leo2d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -364,10 +393,16 @@ class Device:
         return jobs.value, rows.value, nnz.value
 
     def batch_kpm(self, M, R, seed=0, precision=PREC_FP64, tune=False, graph=True, mode=BATCH_ROW_GROUPS,
-                  tile_rows=0, grid=0):
-        """Moments mu[jobs][R][M] of every job in the batch (random-phase starts)."""
+                  tile_rows=0, grid=0, targets=None):
+        """Moments mu[jobs][R][M] of every job in the batch: random-phase starts,
+        or unit vectors on targets[jobs][R] (orbital indices inside each job)."""
         p = BatchParams()
         p.M, p.R, p.seed, p.precision = M, R, seed, precision
+        p.start = START_RANDOM_PHASE
+        if targets is not None:
+            tg = np.ascontiguousarray(targets, dtype=np.int32)
+            p.start = START_TARGETS
+            p.targets = tg.ctypes.data_as(C.POINTER(C.c_int32))
         p.tune, p.graph, p.mode, p.tile_rows, p.grid = int(tune), int(graph), mode, tile_rows, grid
         jobs = self.batch_info()[0]
         mu = np.zeros((jobs, R, M))
@@ -390,3 +425,88 @@ class Device:
 
     def run_deck(self, text: str, out_prefix: str, pattern: int = PATTERN_REFERENCE):
         self._check(lib().leo2d_run_deck(self.h, text.encode(), out_prefix.encode(), pattern))
+
+
+# ---- learned surrogate ------------------------------------------------------
+def mlp_features(cfg) -> np.ndarray:
+    """[n][64] Fourier features of configurations [n][6] = (b1, b2, exx, eyy, exy, theta_deg)."""
+    c = np.ascontiguousarray(np.atleast_2d(cfg), dtype=np.float64)
+    f = np.zeros((c.shape[0], MLP_FEATURES), dtype=np.float32)
+    rc = lib().leo2d_mlp_features(_p(c), c.shape[0], _p(f))
+    if rc != OK:
+        raise Leo2dError(rc, "leo2d_mlp_features: invalid configuration")
+    return f
+
+
+class Mlp:
+    """One surrogate network on a device (BF16 MFMA training and inference)."""
+
+    def __init__(self, device=0, in_=MLP_FEATURES, hidden=256, layers=3, out=1024, seed=0):
+        d = MlpDesc()
+        d.in_, d.hidden, d.layers, d.out, d.seed = in_, hidden, layers, out, seed
+        h = C.c_void_p()
+        rc = lib().leo2d_mlp_create(device, C.byref(d), C.byref(h))
+        if rc != OK:
+            raise Leo2dError(rc, "leo2d_mlp_create failed")
+        self.h = h
+        self.dims = [in_] + [hidden] * layers + [out]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().leo2d_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc != OK:
+            raise Leo2dError(rc, lib().leo2d_mlp_last_error(self.h).decode())
+
+    @property
+    def num_params(self) -> int:
+        return lib().leo2d_mlp_num_params(self.h)
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, dtype=np.float32)
+        self._check(lib().leo2d_mlp_get_params(self.h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        self._check(lib().leo2d_mlp_set_params(self.h, _p(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.dims[-1]), dtype=np.float32)
+        self._check(lib().leo2d_mlp_forward(self.h, _p(x), x.shape[0], _p(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = C.c_double()
+        self._check(lib().leo2d_mlp_gradients(self.h, _p(x), _p(t), x.shape[0], C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, x, t, epochs=10, batch=256, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        p = MlpTrainParams()
+        p.epochs, p.batch, p.lr, p.beta1, p.beta2, p.eps, p.seed = epochs, batch, lr, beta1, beta2, eps, seed
+        losses = np.zeros(max(epochs, 1))
+        self._check(lib().leo2d_mlp_train(self.h, _p(x), _p(t), x.shape[0], C.byref(p), _p(losses)))
+        return losses[:epochs]
+
+
+def mlp_ensemble(nets, x, with_mean=True):
+    """(mean [n][out] or None, spread [n]) of an ensemble of networks of one shape."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    arr = (C.c_void_p * len(nets))(*[m.h for m in nets])
+    mean = np.zeros((x.shape[0], nets[0].dims[-1]), dtype=np.float32) if with_mean else None
+    spread = np.zeros(x.shape[0], dtype=np.float32)
+    rc = lib().leo2d_mlp_ensemble(arr, len(nets), _p(x), x.shape[0], _p(mean), _p(spread))
+    if rc != OK:
+        raise Leo2dError(rc, lib().leo2d_mlp_last_error(nets[0].h).decode())
+    return mean, spread

@@ -159,10 +159,11 @@ __global__ void batch_reduce_kernel(const double* part, const int32_t* job_tile0
     else m[2 * j + k] = 2.0 * s - m[k];
 }
 
-// random-phase start vectors of every job: vector r of job p is global vector
-// p * R + r, orbital i is the row's index inside its job
+// start vectors of every job: random phases (vector r of job p is global vector
+// p * R + r, orbital i is the row's index inside its job) or, with targets,
+// the unit vector on orbital targets[p * R + r] of job p
 __global__ void batch_init_kernel(void* x, int vec_type, const int64_t* job_row0, int P, int64_t nrows, int R,
-                                  uint64_t seed)
+                                  uint64_t seed, const int32_t* targets)
 {
     const size_t total = (size_t)nrows * R;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -175,9 +176,14 @@ __global__ void batch_init_kernel(void* x, int vec_type, const int64_t* job_row0
             else hi = mid - 1;
         }
         const uint64_t i = (uint64_t)(row - job_row0[lo]);
-        const double phi = random_phase_angle(seed, (uint64_t)lo * R + r, i);
         double sn, cs;
-        sincos(phi, &sn, &cs);
+        if (targets) {
+            cs = (int64_t)targets[(size_t)lo * R + r] == (int64_t)i ? 1.0 : 0.0;
+            sn = 0.0;
+        } else {
+            const double phi = random_phase_angle(seed, (uint64_t)lo * R + r, i);
+            sincos(phi, &sn, &cs);
+        }
         if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);
         else static_cast<float2*>(x)[t] = make_float2((float)cs, (float)sn);
     }
@@ -238,13 +244,13 @@ hipError_t launch_batch_reduce(const double* part, const int32_t* job_tile0, int
 }
 
 hipError_t launch_batch_init(void* x, int vec_type, const int64_t* job_row0, int P, int64_t nrows, int R, uint64_t seed,
-                             hipStream_t s)
+                             const int32_t* targets, hipStream_t s)
 {
     if (vec_type != VEC_CPX_F64 && vec_type != VEC_CPX_F32) return hipErrorInvalidValue;
     const size_t total = (size_t)nrows * R;
     const int blocks = (int)std::min<size_t>(8192, (total + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(batch_init_kernel, dim3(std::max(blocks, 1)), dim3(kThreads), 0, s, x, vec_type, job_row0, P,
-                       nrows, R, seed);
+                       nrows, R, seed, targets);
     return hipGetLastError();
 }
 

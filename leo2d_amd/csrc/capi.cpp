@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "assemble.h"
+#include "devbuf.h"
 #include "host.h"
 #include "kernels.h"
 
@@ -30,58 +31,10 @@ struct leo2d_geom {
 
 namespace {
 
-// owning device allocation (freed on release() or destruction; movable, not copyable)
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
-    {
-        o.p = nullptr;
-        o.bytes = 0;
-    }
-    DevBuf& operator=(DevBuf&& o) noexcept
-    {
-        if (this != &o) {
-            release();
-            p = o.p;
-            bytes = o.bytes;
-            o.p = nullptr;
-            o.bytes = 0;
-        }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-struct HipError : std::runtime_error {
-    int code;
-    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
-};
-
-void check(hipError_t e, const char* what)
-{
-    if (e != hipSuccess) {
-        const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? LEO2D_ERR_ALLOC : LEO2D_ERR_HIP;
-        throw HipError(std::string(what) + ": " + hipGetErrorString(e), code);
-    }
-}
-
-void dalloc(DevBuf& b, size_t bytes, const char* what)
-{
-    b.release();
-    if (bytes == 0) bytes = 16;
-    check(hipMalloc(&b.p, bytes), what);
-    b.bytes = bytes;
-}
+using leo2d::DevBuf;
+using leo2d::HipError;
+using leo2d::check;
+using leo2d::dalloc;
 
 struct ArgError : std::runtime_error {
     explicit ArgError(const std::string& m) : std::runtime_error(m) {}
@@ -181,15 +134,15 @@ struct leo2d_handle {
         struct GraphKey {
             int M, R, precision, mode, tile_rows, grid;
             uint64_t seed;
-            const void *A, *B, *part, *mu, *val;
+            const void *A, *B, *part, *mu, *val, *targets;
             bool operator==(const GraphKey& o) const
             {
                 return M == o.M && R == o.R && precision == o.precision && mode == o.mode && tile_rows == o.tile_rows &&
                        grid == o.grid && seed == o.seed && A == o.A && B == o.B && part == o.part && mu == o.mu &&
-                       val == o.val;
+                       val == o.val && targets == o.targets;
             }
         } key{};
-        DevBuf A, B, part, mu;
+        DevBuf A, B, part, mu, tgt;
         leo2d_batch_timing timing{};
         int jobs() const { return (int)job_row0.size() - 1 + (int)pending.size(); }
         void drop_graph()
@@ -1394,10 +1347,28 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         constexpr int kMinTile = 16;
         if (p->tile_rows != 0 && p->tile_rows < kMinTile) throw ArgError("tile_rows must be 0 or at least 16");
         if (p->grid < 0 || p->grid % 8 != 0) throw ArgError("grid must be 0 or a positive multiple of 8");
+        if (p->start != LEO2D_START_RANDOM_PHASE && p->start != LEO2D_START_TARGETS) throw ArgError("unknown start");
+        if (p->start == LEO2D_START_TARGETS && !p->targets) throw ArgError("LEO2D_START_TARGETS needs targets");
         batch_stack(h);
         hipStream_t s = h->stream;
         const int P = (int)B.job_row0.size() - 1;
         const bool mixed = p->precision == LEO2D_PREC_MIXED;
+        const int32_t* dtargets = nullptr;
+        if (p->start == LEO2D_START_TARGETS) {
+            for (int q = 0; q < P; ++q)
+                for (int r = 0; r < R; ++r) {
+                    const int32_t t = p->targets[(size_t)q * R + r];
+                    if (t < 0 || t >= B.job_row0[q + 1] - B.job_row0[q])
+                        throw ArgError("target " + std::to_string(t) + " outside job " + std::to_string(q));
+                }
+            if (B.tgt.bytes != std::max<size_t>(sizeof(int32_t) * (size_t)P * R, 16)) {
+                B.drop_graph();
+                dalloc(B.tgt, sizeof(int32_t) * (size_t)P * R, "batch targets");
+            }
+            check(hipMemcpyAsync(B.tgt.p, p->targets, sizeof(int32_t) * (size_t)P * R, hipMemcpyHostToDevice, s),
+                  "batch targets");
+            dtargets = (const int32_t*)B.tgt.p;
+        }
         const int mat_type = B.is_complex ? (mixed ? leo2d::MAT_CPX_F32 : leo2d::MAT_CPX_F64)
                                           : (mixed ? leo2d::MAT_REAL_F32 : leo2d::MAT_REAL_F64);
         const int vec_type = mixed ? leo2d::VEC_CPX_F32 : leo2d::VEC_CPX_F64;
@@ -1464,7 +1435,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 grid = hit->grid;
             } else {
                 // time a few Chebyshev steps for every candidate shape on this batch
-                check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, s),
+                check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, dtargets, s),
                       "batch init");
                 set_shape(0, 128, 0);
                 step(B.A.p, B.B.p, 0);
@@ -1509,7 +1480,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         grid = sa.grid;
 
         auto record = [&]() {
-            check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, s),
+            check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, dtargets, s),
                   "batch init");
             for (int j = 0; j < M / 2; ++j) {
                 void* cur = (j % 2 == 0) ? B.A.p : B.B.p;
@@ -1522,7 +1493,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         };
         if (p->graph) {
             const leo2d_handle::Batch::GraphKey key{M, R, p->precision, mode, T, grid, p->seed,
-                                                     B.A.p, B.B.p, B.part.p, B.mu.p, dval};
+                                                     B.A.p, B.B.p, B.part.p, B.mu.p, dval, dtargets};
             if (!B.exec || !(key == B.key)) {
                 B.drop_graph();
                 hipGraph_t graph = nullptr;

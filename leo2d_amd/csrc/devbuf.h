@@ -1,0 +1,67 @@
+// devbuf.h -- owning device buffers and HIP error checks shared by the
+// host-side implementation files of libleo2d.  Internal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/leo2d.h"
+
+namespace leo2d {
+
+// owning device allocation (freed on release() or destruction; movable, not copyable)
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? LEO2D_ERR_ALLOC : LEO2D_ERR_HIP;
+        throw HipError(std::string(what) + ": " + hipGetErrorString(e), code);
+    }
+}
+
+inline void dalloc(DevBuf& b, size_t bytes, const char* what)
+{
+    b.release();
+    if (bytes == 0) bytes = 16;
+    check(hipMalloc(&b.p, bytes), what);
+    b.bytes = bytes;
+}
+
+}  // namespace leo2d

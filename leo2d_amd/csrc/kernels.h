@@ -103,9 +103,10 @@ hipError_t launch_batch_step(int mat_type, int vec_type, const BatchStepArgs& a,
 // mu[p][r][M] of job p from its tiles [job_tile0[p], job_tile0[p+1]) (doubling identity)
 hipError_t launch_batch_reduce(const double* part, const int32_t* job_tile0, int P, int R, int M, int j, double* mu,
                                hipStream_t s);
-// random-phase start vectors: vector r of job p = global vector p*R + r
+// start vectors: random phases (vector r of job p = global vector p*R + r), or,
+// when targets != nullptr, the unit vector on orbital targets[p*R + r] of job p
 hipError_t launch_batch_init(void* x, int vec_type, const int64_t* job_row0, int P, int64_t nrows, int R, uint64_t seed,
-                             hipStream_t s);
+                             const int32_t* targets, hipStream_t s);
 hipError_t launch_batch_offset(const int32_t* col_in, int64_t nnz, int32_t row_off, int32_t* col_out,
                                const int64_t* rp_in, int n, int64_t nnz_off, int64_t* rp_out, hipStream_t s);
 

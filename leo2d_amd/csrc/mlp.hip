@@ -1,0 +1,298 @@
+// mlp.hip -- BF16 MFMA kernels of the learned local-moment surrogate.
+//
+// The surrogate is a small multilayer perceptron (64 Fourier features of a
+// local stacking configuration -> hidden GELU layers -> local Chebyshev
+// moments).  Weights and activations are bfloat16, every product accumulates
+// in FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16), and the optimiser
+// keeps FP32 master weights.  There is no counterpart in the reference; the
+// moments it learns come from the KPM recursion of this library.
+//
+// One GEMM kernel serves every product of training and inference through
+// strides: C[m][n] = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak] and
+// B(k, n) = B[k*sbk + n*sbn], so X W^T (forward), dZ^T X (weight gradient)
+// and dZ W (input gradient) need no transposed copies.  The epilogue fuses
+// bias, GELU, its derivative and the bf16 store of the next layer's operand.
+//
+// Tiling: a 256-thread workgroup owns a 64 x 64 tile of C; its 4 wavefronts
+// each own 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.  K advances in steps of 32
+// through LDS (A tile 64 x 32, B tile 32 x 64, bf16), two MFMA k-steps each.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "mlp.h"
+
+namespace leo2d {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kT = 256;
+constexpr int kTM = 64, kTN = 64, kTK = 32;
+
+__device__ inline uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// GELU, tanh form: 0.5 z (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2/pi)
+__device__ inline float gelu(float z)
+{
+    const float c = 0.7978845608028654f;
+    return 0.5f * z * (1.0f + tanhf(c * (z + 0.044715f * z * z * z)));
+}
+__device__ inline float gelu_grad(float z)
+{
+    const float c = 0.7978845608028654f;
+    const float u = c * (z + 0.044715f * z * z * z);
+    const float t = tanhf(u);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+template <int EPI>
+__global__ __launch_bounds__(kT) void gemm_bf16_kernel(GemmArgs g)
+{
+    __shared__ uint16_t As[kTM][kTK + 4];
+    __shared__ uint16_t Bs[kTK][kTN + 4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        // stage A (64 x 32) and B (32 x 64): 8 elements per thread each, zero outside the matrix
+        for (int e = tid; e < kTM * kTK; e += kT) {
+            const int mm = e / kTK, kk = e % kTK;
+            const int m = m0 + mm, k = k0 + kk;
+            As[mm][kk] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : (uint16_t)0;
+        }
+        for (int e = tid; e < kTK * kTN; e += kT) {
+            const int kk = e / kTN, nn = e % kTN;
+            const int k = k0 + kk, n = n0 + nn;
+            Bs[kk][nn] = (k < g.K && n < g.N) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : (uint16_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kTK; ks += 16) {
+            bf16x4 af[2], bfr[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wm * 32 + i * 16 + (lane & 15);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) af[i][t] = (short)As[row][ks + 4 * (lane >> 4) + t];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int cc = wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bfr[j][t] = (short)Bs[ks + 4 * (lane >> 4) + t][cc];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: lane holds C[row0 + t][col] for t = 0..3
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + t;
+                if (row >= g.M) continue;
+                const int64_t o = (int64_t)row * g.scm + col;
+                const float v = acc[i][j][t];
+                if (EPI == EPI_STORE_F32) {
+                    g.C32[o] = v;
+                } else if (EPI == EPI_BIAS_F32) {
+                    g.C32[o] = v + g.bias[col];
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + g.bias[col];
+                    g.Z[o] = z;
+                    g.C16[o] = f2bf(gelu(z));
+                } else {   // EPI_GELU_BWD: dZ = (dZ_next W) * gelu'(Z)
+                    g.C16[o] = f2bf(v * gelu_grad(g.Z[o]));
+                }
+            }
+        }
+}
+
+// dY = scale (Y - T) as bf16 and per-row squared-error sums (FP64, fixed order)
+__global__ __launch_bounds__(kT) void mse_grad_kernel(const float* Y, const float* T, int rows, int cols, float scale,
+                                                      uint16_t* dY, double* row_loss)
+{
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    __shared__ double s[kT];
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < cols; c += kT) {
+        const int64_t o = (int64_t)r * cols + c;
+        const float d = Y[o] - T[o];
+        dY[o] = f2bf(scale * d);
+        acc += (double)d * (double)d;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) row_loss[r] = s[0];
+}
+
+// db[n] = sum_m dZ[m][n] (bf16 operand values, FP32 sum in row order)
+__global__ __launch_bounds__(kT) void colsum_kernel(const uint16_t* dZ, int rows, int cols, float* db)
+{
+    const int c = blockIdx.x * kT + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r) s += bf2f(dZ[(int64_t)r * cols + c]);
+    db[c] = s;
+}
+
+// rows idx[b] of X (FP32) -> bf16 batch; rows idx[b] of T -> FP32 batch
+__global__ __launch_bounds__(kT) void gather_kernel(const float* X, int in, const float* T, int out, const int64_t* idx,
+                                                    int rows, uint16_t* Xb, float* Tb)
+{
+    const int b = blockIdx.x;
+    if (b >= rows) return;
+    const int64_t src = idx[b];
+    for (int c = threadIdx.x; c < in; c += kT) Xb[(int64_t)b * in + c] = f2bf(X[src * in + c]);
+    if (T)
+        for (int c = threadIdx.x; c < out; c += kT) Tb[(int64_t)b * out + c] = T[src * out + c];
+}
+
+__global__ __launch_bounds__(kT) void cast_bf16_kernel(const float* x, int64_t n, uint16_t* y)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) y[i] = f2bf(x[i]);
+}
+
+// Adam with bias correction; writes the bf16 copy used by the GEMMs
+__global__ __launch_bounds__(kT) void adam_kernel(float* p, const float* grad, float* m, float* v, int64_t n, float lr,
+                                                  float b1, float b2, float eps, float c1, float c2, uint16_t* p16)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+        const float gr = grad[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gr;
+        const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+        m[i] = mi;
+        v[i] = vi;
+        const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        const float pn = p[i] - upd;
+        p[i] = pn;
+        if (p16) p16[i] = f2bf(pn);
+    }
+}
+
+// per row: mean over the E ensemble members and the largest standard deviation over the outputs
+__global__ __launch_bounds__(kT) void ensemble_stats_kernel(const float* Ys, int E, int64_t estride, int rows, int cols,
+                                                            float* mean, float* spread)
+{
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    __shared__ float s[kT];
+    float worst = 0.f;
+    for (int c = threadIdx.x; c < cols; c += kT) {
+        const int64_t o = (int64_t)r * cols + c;
+        float mu = 0.f;
+        for (int e = 0; e < E; ++e) mu += Ys[e * estride + o];
+        mu /= (float)E;
+        float var = 0.f;
+        for (int e = 0; e < E; ++e) {
+            const float d = Ys[e * estride + o] - mu;
+            var += d * d;
+        }
+        mean[o] = mu;
+        worst = fmaxf(worst, sqrtf(var / (float)E));
+    }
+    s[threadIdx.x] = worst;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) spread[r] = s[0];
+}
+
+int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + kT - 1) / kT)); }
+
+}  // namespace
+
+hipError_t launch_gemm_bf16(const GemmArgs& g, int epi, hipStream_t s)
+{
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
+    const dim3 grid((g.N + kTN - 1) / kTN, (g.M + kTM - 1) / kTM);
+    switch (epi) {
+    case EPI_STORE_F32: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_STORE_F32>, grid, dim3(kT), 0, s, g); break;
+    case EPI_BIAS_F32: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_BIAS_F32>, grid, dim3(kT), 0, s, g); break;
+    case EPI_BIAS_GELU: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_BIAS_GELU>, grid, dim3(kT), 0, s, g); break;
+    case EPI_GELU_BWD: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_GELU_BWD>, grid, dim3(kT), 0, s, g); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mse_grad(const float* Y, const float* T, int rows, int cols, float scale, uint16_t* dY,
+                           double* row_loss, hipStream_t s)
+{
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(rows), dim3(kT), 0, s, Y, T, rows, cols, scale, dY, row_loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum(const uint16_t* dZ, int rows, int cols, float* db, hipStream_t s)
+{
+    hipLaunchKernelGGL(colsum_kernel, dim3((cols + kT - 1) / kT), dim3(kT), 0, s, dZ, rows, cols, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const float* X, int in, const float* T, int out, const int64_t* idx, int rows, uint16_t* Xb,
+                         float* Tb, hipStream_t s)
+{
+    hipLaunchKernelGGL(gather_kernel, dim3(rows), dim3(kT), 0, s, X, in, T, out, idx, rows, Xb, Tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_cast_bf16(const float* x, int64_t n, uint16_t* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(cast_bf16_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, x, n, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int rows, int cols, float* mean,
+                                 float* spread, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ensemble_stats_kernel, dim3(rows), dim3(kT), 0, s, Ys, E, estride, rows, cols, mean, spread);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                       float eps, int step, uint16_t* p16, hipStream_t s)
+{
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, p, grad, m, v, n, lr, b1, b2, eps, c1, c2,
+                       p16);
+    return hipGetLastError();
+}
+
+}  // namespace leo2d

@@ -109,3 +109,16 @@ def test_truncated_deck_is_an_error():
     text = deck_text("tblg_test1.in")
     with pytest.raises(L.Leo2dError):
         L.Geometry(L.Deck(text[: text.index("START_SHEET 2")]))
+
+
+def test_mlp_features_match_oracle():
+    """leo2d_mlp_features (host code of the library) equals the oracle's features bit for bit."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from oracle import mlp_oracle as O
+    rng = np.random.default_rng(0)
+    cfg = np.column_stack([rng.random(300), rng.random(300), rng.normal(0, 0.003, (300, 3)),
+                           rng.uniform(0.3, 3, 300)])
+    assert np.array_equal(L.mlp_features(cfg), O.features(cfg))
+    with pytest.raises(L.Leo2dError):
+        L.mlp_features(np.array([[0.1, 0.2, 0, 0, 0, 0.0]]))    # twist must be positive

@@ -138,3 +138,33 @@ def test_batch_argument_errors(batch):
         args.update(kw)
         with pytest.raises(L.Leo2dError):
             dev.batch_kpm(args.pop("M"), args.pop("R"), **args)
+
+
+def test_batch_target_starts(batch, oracle_mod):
+    """Local-DoS starts: unit vectors on R orbitals of each job."""
+    dev, jobs = batch
+    rng = np.random.default_rng(3)
+    Rt = 4
+    targets = np.array([rng.choice(n, size=Rt, replace=False) for n, _ in jobs], dtype=np.int32)
+    mu = dev.batch_kpm(M, Rt, targets=targets, graph=True)
+    # the oracle needs each job's matrix: rebuild them in the fixture's order
+    p = 0
+    for i in CELLS:
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=M)))
+        d2 = L.Device(0)
+        try:
+            d2.set_geometry(geom)
+            d2.pairs_build()
+            for kf in KFRAC:
+                W.assemble_rescaled(d2, geom, kf)
+                rp, ci, v = d2.csr_copy()
+                omu = oracle_mod.kpm(rp, ci, v, M, targets=targets[p], doubling_all=True, nthreads=8)
+                assert np.max(np.abs(mu[p] - omu)) <= 1e-12, p
+                assert np.allclose(mu[p][:, 0], 1.0)
+                p += 1
+        finally:
+            d2.close()
+    with pytest.raises(L.Leo2dError):
+        bad = targets.copy()
+        bad[0, 0] = jobs[0][0]          # one past the first job's last orbital
+        dev.batch_kpm(M, Rt, targets=bad)

@@ -374,3 +374,69 @@ def test_deck_errors(oracle_mod):
         orc.parse_deck(base.replace("POLY_ORDER = 200", "POLY_ORDER = 202"))
     with pytest.raises(orc.DeckError):
         orc.parse_deck(base.replace("MATERIAL = Graphene", "MATERIAL = Unobtainium"))
+
+
+# ---- learned surrogate oracle --------------------------------------------------
+def _mlp_oracle():
+    from oracle import mlp_oracle
+    return mlp_oracle
+
+
+def test_mlp_oracle_gelu_closed_forms():
+    O = _mlp_oracle()
+    assert O.gelu(0.0) == 0.0 and O.gelu_grad(0.0) == 0.5
+    assert abs(O.gelu(12.0) - 12.0) < 1e-12 and abs(O.gelu(-12.0)) < 1e-12
+    z = np.linspace(-6, 6, 121)
+    h = 1e-6
+    fd = (O.gelu(z + h) - O.gelu(z - h)) / (2 * h)
+    assert np.max(np.abs(fd - O.gelu_grad(z))) < 1e-8
+    # the tanh form stays within 1e-3 of the exact (erf) GELU
+    from scipy.special import erf
+    assert np.max(np.abs(O.gelu(z) - 0.5 * z * (1 + erf(z / np.sqrt(2))))) < 1e-3
+
+
+def test_mlp_oracle_bf16_rounding():
+    O = _mlp_oracle()
+    # exactly representable values survive; halfway cases go to the even neighbour
+    assert O.bf16(np.float32(1.0)) == 1.0 and O.bf16(np.float32(-3.5)) == -3.5
+    one_ulp = 2.0 ** -7                            # bf16 spacing in [1, 2)
+    assert O.bf16(np.float32(1.0 + one_ulp / 2)) == 1.0                 # tie -> even (1.0)
+    assert O.bf16(np.float32(1.0 + 1.5 * one_ulp)) == 1.0 + 2 * one_ulp  # tie -> even
+    assert O.bf16(np.float32(1.0 + 0.6 * one_ulp)) == 1.0 + one_ulp
+    x = np.random.default_rng(1).normal(size=1000).astype(np.float32)
+    r = O.bf16(x)
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8)
+
+
+def test_mlp_oracle_gradient_against_finite_differences():
+    """Float64 loss gradient of the oracle network against central differences."""
+    O = _mlp_oracle()
+    rng = np.random.default_rng(2)
+    dims = [6, 5, 7, 3]
+    nparam = sum(dims[l + 1] * dims[l] + dims[l + 1] for l in range(len(dims) - 1))
+    p = rng.normal(0, 0.5, nparam)
+    X = rng.normal(size=(9, 6))
+    T = rng.normal(size=(9, 3))
+    _, g = O.loss_and_grad(p, dims, X, T, bf16_points=False)
+    h = 1e-6
+    fd = np.zeros(nparam)
+    for i in range(nparam):
+        e = np.zeros(nparam)
+        e[i] = h
+        lp, _ = O.loss_and_grad(p + e, dims, X, T, bf16_points=False)
+        lm, _ = O.loss_and_grad(p - e, dims, X, T, bf16_points=False)
+        fd[i] = (lp - lm) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 1e-8 * max(1.0, np.max(np.abs(g)))
+
+
+def test_mlp_features_periodic_in_shift():
+    """Features of a shift are invariant under lattice translations b -> b + (1, 0), (0, 1)."""
+    O = _mlp_oracle()
+    rng = np.random.default_rng(4)
+    cfg = np.column_stack([rng.random(20), rng.random(20), rng.normal(0, 0.003, (20, 3)), rng.uniform(0.8, 3, 20)])
+    f0 = O.features(cfg)
+    for d in ((1, 0), (0, 1), (-2, 3)):
+        c2 = cfg.copy()
+        c2[:, 0] += d[0]
+        c2[:, 1] += d[1]
+        assert np.max(np.abs(O.features(c2) - f0)) < 1e-5
