@@ -9,6 +9,8 @@ for r < 6 A), a complex Bloch Hamiltonian H(k) and a Gershgorin rescale with
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 import leo2d_amd as L
@@ -19,13 +21,16 @@ def cell_orbitals(i: int) -> int:
     return 4 * (3 * i * i + 3 * i + 1)
 
 
-def tblg_cell_deck(m: int, n: int, tiles: int = 1, M: int = 1024, name: str = "tblg_cell") -> str:
+def tblg_cell_deck(m: int, n: int, tiles: int = 1, M: int = 1024, name: str = "tblg_cell",
+                   target_sheets=(1,)) -> str:
     """Periodic two-sheet deck of the (m, n) cell tiled tiles x tiles
     (SUPERCELL_M_N = tiles*m tiles*n).  The orbital grid [-S, S) with
     S = 2 max(m, n) + min(m, n) + 1 (in tiled units) covers the cell with the
-    centre cell (0, 0) inside it."""
+    centre cell (0, 0) inside it.  The targets are the centre-cell orbitals of
+    target_sheets."""
     mm, nn = tiles * m, tiles * n
     S = 2 * max(mm, nn) + min(mm, nn) + 1
+    ts = " ".join(str(t) for t in target_sheets)
     return f"""JOB_NAME = {name}
 MAXSIZE = {S}
 MINSIZE = -{S}
@@ -43,12 +48,43 @@ HEIGHT = 3.35
 ANGLE = 0
 END_SHEET 2
 SOLVER_TYPE = SQ
+NUM_TARGET_SHEETS = {len(target_sheets)}
+TARGET_SHEETS = {ts}
+NUM_SHIFT_SHEETS = 1
+SHIFT_SHEETS = 2
+NSHIFTS = 1
+ENERGY_RESCALE = 1
+ENERGY_SHIFT = 0.0
+POLY_ORDER = {M}
+"""
+
+
+def tblg_flake_deck(theta_deg: float, radius: float, M: int = 1024, name: str = "tblg_flake") -> str:
+    """Open (disk) bilayer flake of the given radius in A (MAXSIZE also sets the
+    disk radius, app/src/main.cpp:103), sheet 2 twisted by theta_deg."""
+    S = int(math.ceil(radius))
+    return f"""JOB_NAME = {name}
+MAXSIZE = {S}
+MINSIZE = -{S}
+BOUNDARY_CONDITION = 0
+NUM_SHEETS = 2
+START_SHEET 1
+MATERIAL = GrapheneSimple
+HEIGHT = 0
+ANGLE = 0
+END_SHEET 1
+START_SHEET 2
+MATERIAL = GrapheneSimple
+HEIGHT = 3.35
+ANGLE = {theta_deg!r}
+END_SHEET 2
+SOLVER_TYPE = SQ
 NUM_TARGET_SHEETS = 1
 TARGET_SHEETS = 1
 NUM_SHIFT_SHEETS = 1
 SHIFT_SHEETS = 2
 NSHIFTS = 1
-ENERGY_RESCALE = 1
+ENERGY_RESCALE = 10.5
 ENERGY_SHIFT = 0.0
 POLY_ORDER = {M}
 """
