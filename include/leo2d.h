@@ -92,6 +92,12 @@ typedef struct {
     double u[2][2];
     int32_t num_targets;
     int32_t targets[LEO2D_MAX_TARGETS];
+    /* per-sheet strain (open boundaries only; no counterpart in the reference):
+     * r -> (I + sheet_strain[s]) r on sheet s after the shifts, before the
+     * uniform strain.  Deck key HETEROSTRAIN = eps nu sets uniaxial strain
+     * [[eps, 0], [0, -nu eps]] on the last sheet. */
+    int32_t sheet_strain_on;
+    double sheet_strain[LEO2D_MAX_SHEETS][2][2];
 } leo2d_job;
 
 int leo2d_jobs_count(const leo2d_deck* d, const leo2d_geom* g, int32_t* count);
@@ -282,8 +288,11 @@ int leo2d_mlp_gradients(leo2d_mlp* m, const float* x, const float* t, int32_t n,
 int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, const leo2d_mlp_train_params* p,
                     double* epoch_loss);
 /* Ensemble of count networks of one shape on one device: mean[n][out] (may be NULL) and
- * spread[n] = max over outputs of the ensemble standard deviation. */
-int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, float* mean, float* spread);
+ * spread[n][blocks]: for each of `blocks` equal slices of the outputs (one per
+ * sublattice, say), the root mean square over the slice of the ensemble
+ * standard deviation. */
+int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, int32_t blocks, float* mean,
+                       float* spread);
 
 #ifdef __cplusplus
 }

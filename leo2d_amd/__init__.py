@@ -79,6 +79,7 @@ class Job(C.Structure):
         ("elec_on", C.c_int32), ("mag_on", C.c_int32), ("E", C.c_double), ("B", C.c_double),
         ("complex_matrix", C.c_int32), ("uniform_strain", C.c_int32), ("u", (C.c_double * 2) * 2),
         ("num_targets", C.c_int32), ("targets", C.c_int32 * MAX_TARGETS),
+        ("sheet_strain_on", C.c_int32), ("sheet_strain", ((C.c_double * 2) * 2) * MAX_SHEETS),
     ]
 
 
@@ -181,7 +182,7 @@ def lib() -> C.CDLL:
         L.leo2d_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
         L.leo2d_mlp_gradients.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]
         L.leo2d_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(MlpTrainParams), vp]
-        L.leo2d_mlp_ensemble.argtypes = [C.POINTER(vp), C.c_int32, vp, C.c_int64, vp, vp]
+        L.leo2d_mlp_ensemble.argtypes = [C.POINTER(vp), C.c_int32, vp, C.c_int64, C.c_int32, vp, vp]
         L.leo2d_batch_clear.argtypes = [vp]
         L.leo2d_batch_append.argtypes = [vp, C.POINTER(C.c_int32)]
         L.leo2d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -500,13 +501,15 @@ class Mlp:
         return losses[:epochs]
 
 
-def mlp_ensemble(nets, x, with_mean=True):
-    """(mean [n][out] or None, spread [n]) of an ensemble of networks of one shape."""
+def mlp_ensemble(nets, x, with_mean=True, blocks=1):
+    """(mean [n][out] or None, spread [n] (blocks = 1) or [n][blocks]) of an
+    ensemble of networks of one shape; spread = RMS over each block of the
+    outputs of the ensemble standard deviation."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     arr = (C.c_void_p * len(nets))(*[m.h for m in nets])
     mean = np.zeros((x.shape[0], nets[0].dims[-1]), dtype=np.float32) if with_mean else None
-    spread = np.zeros(x.shape[0], dtype=np.float32)
-    rc = lib().leo2d_mlp_ensemble(arr, len(nets), _p(x), x.shape[0], _p(mean), _p(spread))
+    spread = np.zeros((x.shape[0], blocks), dtype=np.float32)
+    rc = lib().leo2d_mlp_ensemble(arr, len(nets), _p(x), x.shape[0], blocks, _p(mean), _p(spread))
     if rc != OK:
         raise Leo2dError(rc, lib().leo2d_mlp_last_error(nets[0].h).decode())
-    return mean, spread
+    return mean, (spread[:, 0] if blocks == 1 else spread)

@@ -44,6 +44,8 @@ struct JobDev {
     int complex_matrix;
     int uniform_strain;
     double u[2][2];
+    int sheet_strain_on;
+    double ss[LEO2D_MAX_SHEETS][2][2];   // per-sheet strain tensors
 };
 
 hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total, int64_t* pair_rp, hipStream_t s);

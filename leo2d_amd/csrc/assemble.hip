@@ -34,6 +34,7 @@ constexpr int kThreads = 256;
 constexpr double kPI = 3.14159265358979323846;
 constexpr double kPI_6 = 0.523598775598298873077;
 constexpr double kA = 2.4768;
+constexpr double kAcc = 1.429981146728865;   // a / sqrt(3)
 
 inline dim3 rows_grid(int n) { return dim3((unsigned)((n + kThreads - 1) / kThreads)); }
 
@@ -46,7 +47,7 @@ __device__ inline double intralayer(int mat, int orb_row, int orb_col, int v0, i
     if (orb_col == 0 && orb_row == 1) h0 -= 1;
     else if (orb_col == 1 && orb_row == 0) h0 += 1;
     const int r = h0 * (h0 + h1) + h1 * h1;
-    if (mat == MAT_GRAPHENE_SIMPLE) return r == 1 ? -2.7 : 0.0;
+    if (mat != MAT_GRAPHENE) return r == 1 ? -2.7 : 0.0;
     switch (r) {
     case 0: return 0.3504;
     case 1: return -2.8922;
@@ -107,7 +108,7 @@ __device__ double c_to_c(int orb_row, int orb_col, double v0, double v1, double 
 
 __device__ inline double interlayer(int mat, int o1, int o2, double dx, double dy, double dz, double th1, double th2)
 {
-    if (mat == MAT_GRAPHENE_SIMPLE) {
+    if (mat != MAT_GRAPHENE) {
         // t_perp(r) = 0.48 exp(-(r - 3.35)/0.45) eV for 3D distance r < 6 A
         const double r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 < 36.0)) return 0.0;
@@ -120,7 +121,7 @@ __device__ inline double interlayer(int mat, int o1, int o2, double dx, double d
 // support of a coupling (pruned pattern): decided by exact comparisons only
 __device__ inline bool inter_support(int mat, double dx, double dy, double dz)
 {
-    if (mat == MAT_GRAPHENE_SIMPLE) return dx * dx + dy * dy + dz * dz < 36.0;
+    if (mat != MAT_GRAPHENE) return dx * dx + dy * dy + dz * dz < 36.0;
     if (iabs_trunc(dz) > 5) return false;
     return dx * dx + dy * dy < 64.0;
 }
@@ -299,6 +300,12 @@ __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j
     const double sh0 = j.shift[s][0], sh1 = j.shift[s][1];
     double x = g.i2p[3 * (size_t)i + 0] + sh0 * a[0] + sh1 * a[2];
     double y = g.i2p[3 * (size_t)i + 1] + sh0 * a[1] + sh1 * a[3];
+    if (j.sheet_strain_on == 1) {
+        // per-sheet strain (open boundaries only), before the uniform strain
+        const double tx = x, ty = y;
+        x = (1.0 + j.ss[s][0][0]) * tx + j.ss[s][0][1] * ty;
+        y = j.ss[s][1][0] * tx + (1.0 + j.ss[s][1][1]) * ty;
+    }
     if (j.uniform_strain == 1) {
         const double tx = x, ty = y;
         x = (1.0 + j.u[0][0]) * tx + j.u[0][1] * ty;
@@ -348,7 +355,11 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
                 gd0 = gd0 - sc_i * sd.st[0][0] - sc_j * sd.st[1][0];
                 gd1 = gd1 - sc_i * sd.st[0][1] - sc_j * sd.st[1][1];
             }
-            const double raw_t = intralayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1);
+            double raw_t = intralayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1);
+            if (mat == MAT_GRAPHENE_SIMPLE_STRAIN && raw_t != 0) {
+                const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+                raw_t = raw_t * exp(-kStrainBeta * (sqrt(dx * dx + dy * dy + dz * dz) / kAcc - 1.0));
+            }
             const bool diag = CPX ? (new_k == k_i && sc_i == 0 && sc_j == 0) : (new_k == k_i);
             if (diag) {
                 if (j.elec_on == 1) t = (raw_t + eshift + z1 * j.E) / resc;

@@ -247,6 +247,10 @@ leo2d::JobDev job_dev(const leo2d::Geom& g, const leo2d_job& j)
     d.uniform_strain = j.uniform_strain;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) d.u[a][b] = j.u[a][b];
+    d.sheet_strain_on = j.sheet_strain_on;
+    for (int s = 0; s < LEO2D_MAX_SHEETS; ++s)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) d.ss[s][a][b] = j.sheet_strain[s][a][b];
     return d;
 }
 
@@ -774,7 +778,7 @@ int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, d
                     static const double tab[17] = {0.3504, -2.8922, 0, 0.2425, -0.2656, 0, 0, 0.0235, 0, 0.0524,
                                                    0, 0, -0.0209, -0.0148, 0, 0, -0.0211};
                     double v = (rr >= 0 && rr <= 16) ? tab[rr] : 0.0;
-                    if (sd.mat == leo2d::MAT_GRAPHENE_SIMPLE) v = rr == 1 ? -2.7 : 0.0;
+                    if (sd.mat != leo2d::MAT_GRAPHENE) v = rr == 1 ? -2.7 : 0.0;
                     t[w] = v;
                 }
             }
@@ -790,6 +794,7 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
         if (!h->has_geom) throw StateError("leo2d_assemble before leo2d_set_geometry");
         if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
             throw ArgError("energy_rescale must be finite and non-zero");
+        if (job->sheet_strain_on && h->geom.bc == 1) throw ArgError("per-sheet strain needs open boundaries");
         if (!h->has_pairs) {
             int rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
             if (rc != LEO2D_OK) return rc;

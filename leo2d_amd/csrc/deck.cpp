@@ -91,6 +91,7 @@ int material_of(const std::string& name)
 {
     if (name == "Graphene") return MAT_GRAPHENE;
     if (name == "GrapheneSimple") return MAT_GRAPHENE_SIMPLE;
+    if (name == "GrapheneSimpleStrain") return MAT_GRAPHENE_SIMPLE_STRAIN;
     throw DeckError("Failed to find material! Aborting...");
 }
 
@@ -216,6 +217,12 @@ Deck parse_deck(const std::string& text)
                 for (int a = 0; a < 2; ++a)
                     for (int b = 0; b < 2; ++b) u[a][b] = std::atof(tk.next().c_str());
                 o.dm["u_ij"] = u;
+            } else if (key == "HETEROSTRAIN") {
+                // uniaxial strain eps along x with Poisson ratio nu on the last sheet
+                tk.next();
+                const double eps = std::atof(tk.next().c_str());
+                const double nu = std::atof(tk.next().c_str());
+                o.dm["heterostrain"] = {{eps, nu}};
             } else if (key == "MATRIX_SAVE") o.i["matrix_save"] = std::atoi(val().c_str());
             else if (key == "MATRIX_POS_SAVE") o.i["matrix_pos_save"] = std::atoi(val().c_str());
             else if (key == "VERBOSE_SAVE") o.i["verbose_save"] = std::atoi(val().c_str());

@@ -234,6 +234,14 @@ std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
     }
     proto.num_targets = (int)targets.size();
     for (size_t t = 0; t < targets.size(); ++t) proto.targets[t] = targets[t];
+    if (auto hs = o.dm.find("heterostrain"); hs != o.dm.end()) {
+        if (g.bc == 1) throw DeckError("HETEROSTRAIN needs open boundaries");
+        const double eps = hs->second[0][0], nu = hs->second[0][1];
+        const int top = ns - 1;
+        proto.sheet_strain_on = 1;
+        proto.sheet_strain[top][0][0] = eps;
+        proto.sheet_strain[top][1][1] = -nu * eps;
+    }
 
     std::vector<leo2d_job> jobs;
     const long long st = o.geti("solver_type");

@@ -38,8 +38,9 @@ hipError_t launch_colsum(const uint16_t* dZ, int rows, int cols, float* db, hipS
 hipError_t launch_gather(const float* X, int in, const float* T, int out, const int64_t* idx, int rows, uint16_t* Xb,
                          float* Tb, hipStream_t s);
 hipError_t launch_cast_bf16(const float* x, int64_t n, uint16_t* y, hipStream_t s);
-// mean[r][c] over E outputs Ys[e*estride + r*cols + c]; spread[r] = max_c std_e
-hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int rows, int cols, float* mean,
+// mean[r][c] over E outputs Ys[e*estride + r*cols + c]; spread[r][b] = RMS over the
+// outputs of block b (cols / nblk each) of the ensemble standard deviation
+hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int rows, int cols, int nblk, float* mean,
                                  float* spread, hipStream_t s);
 // Adam step `step` (1-based) on FP32 master weights, refreshing their bf16 copy
 hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,

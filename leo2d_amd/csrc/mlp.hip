@@ -204,15 +204,18 @@ __global__ __launch_bounds__(kT) void adam_kernel(float* p, const float* grad, f
     }
 }
 
-// per row: mean over the E ensemble members and the largest standard deviation over the outputs
+// per row: mean over the E ensemble members and, for each of nblk equal blocks
+// of the outputs, the root mean square over the block of the ensemble standard
+// deviation (block grid: blockIdx.y)
 __global__ __launch_bounds__(kT) void ensemble_stats_kernel(const float* Ys, int E, int64_t estride, int rows, int cols,
-                                                            float* mean, float* spread)
+                                                            int nblk, float* mean, float* spread)
 {
-    const int r = blockIdx.x;
-    if (r >= rows) return;
+    const int r = blockIdx.x, b = blockIdx.y;
+    if (r >= rows || b >= nblk) return;
+    const int bw = cols / nblk;
     __shared__ float s[kT];
-    float worst = 0.f;
-    for (int c = threadIdx.x; c < cols; c += kT) {
+    float acc = 0.f;
+    for (int c = b * bw + threadIdx.x; c < (b + 1) * bw; c += kT) {
         const int64_t o = (int64_t)r * cols + c;
         float mu = 0.f;
         for (int e = 0; e < E; ++e) mu += Ys[e * estride + o];
@@ -223,15 +226,15 @@ __global__ __launch_bounds__(kT) void ensemble_stats_kernel(const float* Ys, int
             var += d * d;
         }
         mean[o] = mu;
-        worst = fmaxf(worst, sqrtf(var / (float)E));
+        acc += var / (float)E;
     }
-    s[threadIdx.x] = worst;
+    s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kT / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) spread[r] = s[0];
+    if (threadIdx.x == 0) spread[(int64_t)r * nblk + b] = sqrtf(s[0] / (float)bw);
 }
 
 int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + kT - 1) / kT)); }
@@ -278,11 +281,13 @@ hipError_t launch_cast_bf16(const float* x, int64_t n, uint16_t* y, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int rows, int cols, float* mean,
+hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int rows, int cols, int nblk, float* mean,
                                  float* spread, hipStream_t s)
 {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ensemble_stats_kernel, dim3(rows), dim3(kT), 0, s, Ys, E, estride, rows, cols, mean, spread);
+    if (nblk <= 0 || cols % nblk != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ensemble_stats_kernel, dim3(rows, nblk), dim3(kT), 0, s, Ys, E, estride, rows, cols, nblk, mean,
+                       spread);
     return hipGetLastError();
 }
 

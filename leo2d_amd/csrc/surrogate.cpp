@@ -413,10 +413,12 @@ int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, con
     });
 }
 
-int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, float* mean, float* spread)
+int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, int32_t blocks, float* mean,
+                       float* spread)
 {
-    if (!nets || count < 1 || !x || n < 0 || !spread) return LEO2D_ERR_ARG;
+    if (!nets || count < 1 || !x || n < 0 || !spread || blocks < 1) return LEO2D_ERR_ARG;
     leo2d_mlp* m0 = nets[0];
+    if (m0->out() % blocks != 0) return LEO2D_ERR_ARG;
     for (int e = 0; e < count; ++e)
         if (!nets[e] || nets[e]->dims != m0->dims || nets[e]->device != m0->device) return LEO2D_ERR_ARG;
     return guarded(m0, [&]() {
@@ -425,7 +427,7 @@ int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, in
         DevBuf Ys, dmean, dspread;
         dalloc(Ys, sizeof(float) * (size_t)count * chunk * out, "ensemble outputs");
         dalloc(dmean, sizeof(float) * (size_t)chunk * out, "ensemble mean");
-        dalloc(dspread, sizeof(float) * (size_t)chunk, "ensemble spread");
+        dalloc(dspread, sizeof(float) * (size_t)chunk * blocks, "ensemble spread");
         for (int e = 0; e < count; ++e) ensure_capacity(nets[e], std::min<int64_t>(n, chunk));
         for (int64_t r0 = 0; r0 < n; r0 += chunk) {
             const int rows = (int)std::min<int64_t>(chunk, n - r0);
@@ -436,13 +438,14 @@ int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, in
                 check(hipStreamSynchronize(m->stream), "ensemble forward");
             }
             check(leo2d::launch_ensemble_stats(static_cast<const float*>(Ys.p), count, (int64_t)chunk * out, rows, out,
-                                               f32(dmean), f32(dspread), m0->stream),
+                                               blocks, f32(dmean), f32(dspread), m0->stream),
                   "ensemble stats");
             if (mean)
                 check(hipMemcpyAsync(mean + (size_t)r0 * out, dmean.p, sizeof(float) * (size_t)rows * out,
                                      hipMemcpyDeviceToHost, m0->stream),
                       "ensemble mean");
-            check(hipMemcpyAsync(spread + r0, dspread.p, sizeof(float) * rows, hipMemcpyDeviceToHost, m0->stream),
+            check(hipMemcpyAsync(spread + r0 * blocks, dspread.p, sizeof(float) * rows * blocks, hipMemcpyDeviceToHost,
+                                 m0->stream),
                   "ensemble spread");
             check(hipStreamSynchronize(m0->stream), "ensemble");
         }

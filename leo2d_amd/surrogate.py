@@ -50,12 +50,16 @@ def nearest_cell(theta_deg: float, imin: int = 2, imax: int = 200) -> int:
     return lo if abs(cell_angle_deg(lo) - theta_deg) <= abs(cell_angle_deg(hi) - theta_deg) else hi
 
 
-def sample_configs(n: int, rng, theta_range=(0.8, 3.0), strain_sd=0.003, snap=True) -> np.ndarray:
-    """Uniform shift, N(0, strain_sd) symmetric strain, uniform twist (snapped to
-    the nearest commensurate cell's twist when snap)."""
+def sample_configs(n: int, rng, theta_range=(0.8, 3.0), strain_sd=0.003, snap=True, log_theta=False) -> np.ndarray:
+    """Uniform shift, N(0, strain_sd) symmetric strain, twist uniform (or
+    log-uniform) in theta_range, snapped to the nearest commensurate cell's
+    twist when snap."""
     b = rng.random((n, 2))
     e = rng.normal(0.0, strain_sd, (n, 3))
-    th = rng.uniform(theta_range[0], theta_range[1], n)
+    if log_theta:
+        th = np.exp(rng.uniform(math.log(theta_range[0]), math.log(theta_range[1]), n))
+    else:
+        th = rng.uniform(theta_range[0], theta_range[1], n)
     if snap:
         th = np.array([cell_angle_deg(nearest_cell(t)) for t in th])
     return np.column_stack([b, e, th])
@@ -175,13 +179,17 @@ def train_ensemble(x, t, members=4, hidden=256, layers=3, epochs=60, batch=256, 
     return nets, losses
 
 
-def predict_with_fallback(nets, exact: ExactLocalKPM, cfgs, threshold=1e-3):
+def predict_with_fallback(nets, exact: ExactLocalKPM, cfgs, threshold=1e-3, blocks=None):
     """Ensemble prediction [n][4][M] with exact local KPM where the spread
-    exceeds the threshold.  Returns (moments, fallback mask, spread, report)."""
+    (RMS of the ensemble standard deviation over the M moments of a
+    sublattice) exceeds the threshold: over the sublattice blocks[i] of each
+    configuration when given, else over its largest of the four.  Returns
+    (moments, fallback mask, spread, report)."""
     M = exact.M
     t0 = time.perf_counter()
     x = L.mlp_features(cfgs)
-    mean, spread = L.mlp_ensemble(nets, x)
+    mean, sp4 = L.mlp_ensemble(nets, x, blocks=4)
+    spread = sp4.max(axis=1) if blocks is None else sp4[np.arange(len(x)), blocks]
     t_inf = time.perf_counter() - t0
     mom = mean.reshape(-1, 4, M).astype(np.float64)
     fb = spread > threshold

@@ -59,32 +59,35 @@ POLY_ORDER = {M}
 """
 
 
-def tblg_flake_deck(theta_deg: float, radius: float, M: int = 1024, name: str = "tblg_flake") -> str:
+def tblg_flake_deck(theta_deg: float, radius: float, M: int = 1024, name: str = "tblg_flake",
+                    material: str = "GrapheneSimple", heterostrain=None, rescale: float = 10.5) -> str:
     """Open (disk) bilayer flake of the given radius in A (MAXSIZE also sets the
-    disk radius, app/src/main.cpp:103), sheet 2 twisted by theta_deg."""
+    disk radius, app/src/main.cpp:103), sheet 2 twisted by theta_deg;
+    heterostrain = (eps, nu) strains sheet 2 uniaxially along x."""
     S = int(math.ceil(radius))
+    hs = f"HETEROSTRAIN = {heterostrain[0]!r} {heterostrain[1]!r}\n" if heterostrain else ""
     return f"""JOB_NAME = {name}
 MAXSIZE = {S}
 MINSIZE = -{S}
 BOUNDARY_CONDITION = 0
 NUM_SHEETS = 2
 START_SHEET 1
-MATERIAL = GrapheneSimple
+MATERIAL = {material}
 HEIGHT = 0
 ANGLE = 0
 END_SHEET 1
 START_SHEET 2
-MATERIAL = GrapheneSimple
+MATERIAL = {material}
 HEIGHT = 3.35
 ANGLE = {theta_deg!r}
 END_SHEET 2
-SOLVER_TYPE = SQ
+{hs}SOLVER_TYPE = SQ
 NUM_TARGET_SHEETS = 1
 TARGET_SHEETS = 1
 NUM_SHIFT_SHEETS = 1
 SHIFT_SHEETS = 2
 NSHIFTS = 1
-ENERGY_RESCALE = 10.5
+ENERGY_RESCALE = {rescale!r}
 ENERGY_SHIFT = 0.0
 POLY_ORDER = {M}
 """

@@ -20,6 +20,7 @@
 #define N_PI 3.14159265358979323846
 #define N_PI_6 0.523598775598298873077
 #define N_SQRT3_2 0.866025403784438646764
+#define GR_ACC 1.429981146728865 /* a / sqrt(3), the unstrained nearest-neighbour bond (A) */
 #define N_SQRT3_6 0.288675134594812882255
 
 /* graphene geometry: src/materials/graphene.h:78-95 */
@@ -237,8 +238,8 @@ static double simple_intra(int orb_row, int orb_col, int v0, int v1)
 
 double orc_intralayer(int mat, int orb_row, int orb_col, int v0, int v1)
 {
-    return mat == ORC_MAT_GRAPHENE_SIMPLE ? simple_intra(orb_row, orb_col, v0, v1)
-                                          : graphene_intra(orb_row, orb_col, v0, v1);
+    return mat != ORC_MAT_GRAPHENE ? simple_intra(orb_row, orb_col, v0, v1)
+                                   : graphene_intra(orb_row, orb_col, v0, v1);
 }
 
 static double c_to_c(int orb_row, int orb_col, double v0, double v1, double v2, double theta_row, double theta_col)
@@ -291,8 +292,8 @@ static double simple_inter(double v0, double v1, double v2)
 double orc_interlayer(int mat, int orb_row, int orb_col, double dx, double dy, double dz,
                       double theta_row, double theta_col)
 {
-    return mat == ORC_MAT_GRAPHENE_SIMPLE ? simple_inter(dx, dy, dz)
-                                          : c_to_c(orb_row, orb_col, dx, dy, dz, theta_row, theta_col);
+    return mat != ORC_MAT_GRAPHENE ? simple_inter(dx, dy, dz)
+                                   : c_to_c(orb_row, orb_col, dx, dy, dz, theta_row, theta_col);
 }
 
 /* ------------------------------------------------------------------ */
@@ -500,6 +501,12 @@ void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
         i2pos[i * 3 + 0] = g->i2p[i * 3 + 0] + sh[0] * s1_a[0][0] + sh[1] * s1_a[1][0];
         i2pos[i * 3 + 1] = g->i2p[i * 3 + 1] + sh[0] * s1_a[0][1] + sh[1] * s1_a[1][1];
         i2pos[i * 3 + 2] = g->i2p[i * 3 + 2];
+        if (job->sheet_strain_on == 1) {
+            const double(*e)[2] = job->sheet_strain[s];
+            double tx = i2pos[i * 3 + 0], ty = i2pos[i * 3 + 1];
+            i2pos[i * 3 + 0] = (1.0 + e[0][0]) * tx + e[0][1] * ty;
+            i2pos[i * 3 + 1] = e[1][0] * tx + (1.0 + e[1][1]) * ty;
+        }
         if (job->uniform_strain == 1) {
             double tx = i2pos[i * 3 + 0], ty = i2pos[i * 3 + 1];
             i2pos[i * 3 + 0] = (1.0 + job->u[0][0]) * tx + job->u[0][1] * ty;
@@ -513,7 +520,7 @@ void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
  * decided by IEEE-exact operations on both the CPU and the GPU. */
 static int inter_in_support(int mat, double dx, double dy, double dz)
 {
-    if (mat == ORC_MAT_GRAPHENE_SIMPLE) return sqrt(dx * dx + dy * dy + dz * dz) < 6.0;
+    if (mat != ORC_MAT_GRAPHENE) return sqrt(dx * dx + dy * dy + dz * dz) < 6.0;
     if (abs((int)dz) > 5) return 0;
     return sqrt(dx * dx + dy * dy) < GR_INTER_CUTOFF;
 }
@@ -571,6 +578,11 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                 int support;
                 if (blk == 0) {
                     double raw_t = pr->t;
+                    if (mat == ORC_MAT_GRAPHENE_SIMPLE_STRAIN && raw_t != 0) {
+                        /* strained bond: t = -2.7 exp(-3.37 (d / a_cc - 1)) */
+                        double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+                        raw_t = raw_t * exp(-3.37 * (sqrt(dx * dx + dy * dy + dz * dz) / GR_ACC - 1.0));
+                    }
                     int diag = cpx ? (new_k == k_i && sc_i == 0 && sc_j == 0) : (new_k == k_i);
                     if (diag) {
                         if (job->elec_on == 1) t = (raw_t + eshift + z1 * job->E) / resc;

@@ -45,6 +45,7 @@ extern "C" {
 /* materials */
 #define ORC_MAT_GRAPHENE 0        /* src/materials/graphene.cpp */
 #define ORC_MAT_GRAPHENE_SIMPLE 1 /* NN t=-2.7 eV, t_perp = 0.48 exp(-(r-3.35)/0.45), r<6 A */
+#define ORC_MAT_GRAPHENE_SIMPLE_STRAIN 2 /* as 1, NN t = -2.7 exp(-3.37 (d/a_cc - 1)) at bond length d */
 
 typedef struct {
     int mat;
@@ -102,6 +103,8 @@ typedef struct {
     int complex_matrix;
     int uniform_strain; double u[2][2];
     int pruned;                         /* 0 reference-exact slots, 1 drop zero slots */
+    int sheet_strain_on;                /* per-sheet strain (open boundaries), before u */
+    double sheet_strain[ORC_MAX_SHEETS][2][2];
 } orc_job;
 
 typedef struct {

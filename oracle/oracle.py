@@ -30,7 +30,9 @@ MAX_SHEETS = 8
 MAT_GRAPHENE = 0
 MAT_GRAPHENE_SIMPLE = 1
 # src/materials/materials.cpp:15-21, plus the workload model name
-MATERIALS = {"Graphene": MAT_GRAPHENE, "GrapheneSimple": MAT_GRAPHENE_SIMPLE}
+MAT_GRAPHENE_SIMPLE_STRAIN = 2
+MATERIALS = {"Graphene": MAT_GRAPHENE, "GrapheneSimple": MAT_GRAPHENE_SIMPLE,
+             "GrapheneSimpleStrain": MAT_GRAPHENE_SIMPLE_STRAIN}
 
 GR_A = 2.4768
 SQRT3_2 = 0.866025403784438646764
@@ -95,6 +97,8 @@ class Job(C.Structure):
         ("uniform_strain", C.c_int),
         ("u", (C.c_double * 2) * 2),
         ("pruned", C.c_int),
+        ("sheet_strain_on", C.c_int),
+        ("sheet_strain", ((C.c_double * 2) * 2) * MAX_SHEETS),
     ]
 
 
@@ -343,6 +347,11 @@ def parse_deck(text: str) -> Deck:
                     for j in range(2):
                         u[i][j] = atof(nxt())
                 opts["u_ij"] = u
+            elif tok == "HETEROSTRAIN":
+                nxt()
+                eps = atof(nxt())
+                nu = atof(nxt())
+                opts["heterostrain"] = (eps, nu)
             elif tok == "MATRIX_SAVE":
                 nxt(); opts["matrix_save"] = atoi(nxt())
             elif tok == "MATRIX_POS_SAVE":
@@ -650,6 +659,15 @@ def make_job(deck: Deck, job: dict, pruned: int = 0) -> Job:
             for b in range(2):
                 j.u[a][b] = o["u_ij"][a][b]
     j.pruned = pruned
+    if "heterostrain" in o:
+        # uniaxial strain [[eps, 0], [0, -nu eps]] on the last sheet (open boundaries)
+        if o["boundary_condition"] == 1:
+            raise DeckError("HETEROSTRAIN needs open boundaries")
+        eps, nu = o["heterostrain"]
+        top = len(deck.sdata) - 1
+        j.sheet_strain_on = 1
+        j.sheet_strain[top][0][0] = eps
+        j.sheet_strain[top][1][1] = -nu * eps
     return j
 
 

@@ -31,7 +31,7 @@ def test_exports_every_declared_symbol():
 
 
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
-             "tblg_gshift_strain.in", "tblg_simple_flake.in"]
+             "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -72,6 +72,11 @@ def test_jobs_match_oracle(oracle_mod, name):
             assert (j.k[0], j.k[1]) == tuple(oj["k_vec"])
         assert j.complex_matrix == orc.complex_matrix(od.opts)
         assert j.energy_rescale == od.opts["energy_rescale"]
+        oj_struct = orc.make_job(od, oj)
+        assert j.sheet_strain_on == oj_struct.sheet_strain_on
+        for s in range(g.info.num_sheets):
+            for a in range(2):
+                assert tuple(j.sheet_strain[s][a]) == tuple(oj_struct.sheet_strain[s][a])
 
 
 def test_commensurate_cell_count():

@@ -99,9 +99,13 @@ def test_mlp_ensemble_mean_and_spread():
         ys = np.stack([m.forward(x) for m in nets])
         mean, spread = L.mlp_ensemble(nets, x)
         assert np.max(np.abs(mean - ys.mean(axis=0))) <= 1e-5 * max(1.0, np.max(np.abs(ys)))
-        assert np.max(np.abs(spread - ys.std(axis=0).max(axis=1))) <= 1e-5 * max(1.0, np.max(spread))
+        want = np.sqrt(np.mean(ys.var(axis=0), axis=1))
+        assert np.max(np.abs(spread - want)) <= 1e-5 * max(1.0, np.max(want))
         _, s2 = L.mlp_ensemble(nets, x, with_mean=False)
         assert np.array_equal(s2, spread)
+        _, s4 = L.mlp_ensemble(nets, x, blocks=4)
+        want4 = np.sqrt(np.mean(ys.var(axis=0).reshape(len(x), 4, 12), axis=2))
+        assert s4.shape == (len(x), 4) and np.max(np.abs(s4 - want4)) <= 1e-5 * max(1.0, np.max(want4))
     finally:
         for m in nets:
             m.close()

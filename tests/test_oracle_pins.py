@@ -11,6 +11,7 @@
 """
 import ctypes as C
 import math
+import os
 import re
 
 import numpy as np
@@ -440,3 +441,41 @@ def test_mlp_features_periodic_in_shift():
         c2[:, 0] += d[0]
         c2[:, 1] += d[1]
         assert np.max(np.abs(O.features(c2) - f0)) < 1e-5
+
+
+def test_strained_hopping_closed_form(oracle_mod):
+    """GrapheneSimpleStrain with HETEROSTRAIN = eps nu on the top sheet: every
+    bottom-sheet bond keeps t = -2.7 eV, every top-sheet bond has the strained
+    length a_cc |(I + e) R(theta) n_k| of its unit bond direction n_k and
+    t = -2.7 exp(-3.37 (d / a_cc - 1)) eV, e = [[eps, 0], [0, -nu eps]]."""
+    orc = oracle_mod
+    text = open(os.path.join(os.path.dirname(__file__), "decks", "tblg_strain_flake.in")).read()
+    od = orc.parse_deck(text)
+    og = orc.Geometry(orc.geom_desc(od))
+    job = orc.make_job(od, orc.build_jobs(od, orc.target_list(od, og))[0], pruned=1)
+    rp, ci, v = og.assemble(job)
+    pos = og.job_positions(job).reshape(-1, 3)
+    resc, eps, nu, th = 10.5, 0.02, 0.16, math.radians(2.0)
+    acc = 2.4768 / math.sqrt(3.0)
+    E = np.array([[1 + eps, 0.0], [0.0, 1 - nu * eps]])
+    Rt = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    dirs = [np.array([math.cos(a), math.sin(a)]) for a in (math.pi / 6, 5 * math.pi / 6, -math.pi / 2)]
+    top_lengths = sorted(acc * np.linalg.norm(E @ Rt @ d) for d in dirs)
+    n_bottom = n_top = 0
+    for r in range(len(rp) - 1):
+        for q in range(rp[r], rp[r + 1]):
+            c = ci[q]
+            dz = pos[c, 2] - pos[r, 2]
+            if c == r or dz != 0.0:
+                continue
+            d = float(np.linalg.norm(pos[c] - pos[r]))
+            if pos[r, 2] == 0.0:
+                assert abs(d - acc) < 1e-12 and abs(v[q] * resc + 2.7) < 1e-12
+                n_bottom += 1
+            else:
+                assert min(abs(d - L) for L in top_lengths) < 1e-12, d
+                assert abs(v[q] * resc + 2.7 * math.exp(-3.37 * (d / acc - 1.0))) < 1e-12
+                n_top += 1
+    assert n_bottom > 100 and n_top > 100
+    # the three strained bond families are distinct and bracket a_cc
+    assert top_lengths[0] < acc < top_lengths[-1]
