@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0, help="vectors per device block (0 = all R at once)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
                     help="c1: 1.05 deg 8x8 tBLG (headline); c2: 32 uneven cells x 8 k-points batched; "
                          "c3: large heterostrained flake, FP32 recursion; c4: learned local-moment surrogate")
@@ -164,15 +165,15 @@ def main():
 
     def run(precision, steps, warmup, time_steps):
         for _ in range(warmup):
-            dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid)
+            dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid, block=args.block)
         times = []
         mu = None
         for _ in range(steps):
             t0 = time.perf_counter()
-            mu = dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid)
+            mu = dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid, block=args.block)
             times.append(time.perf_counter() - t0)
         # per-launch durations (HIP events around every fused step, separate pass)
-        dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid,
+        dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=precision, grid=args.grid, block=args.block,
                 time_steps=time_steps)
         tm = dev.timing()
         return mu, times, tm

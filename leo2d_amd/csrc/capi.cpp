@@ -1000,9 +1000,12 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         const bool keep_v = dev_scheme == 2;
         int G = p->grid;
         if (G <= 0) {
+            // R >= 64: four workgroups per CU (one wavefront per row); fewer rows in
+            // flight keep each XCD's gathered rows in its L2 (measured best on the
+            // 1.05 deg 8x8 cell: 1.66 ms per FP64 step at 1024 vs 1.79 ms at 2048)
             const int rows_per_iter = std::max(1, 256 / B);   // 4 wavefronts x 64/B rows
             const int want = (n + rows_per_iter - 1) / rows_per_iter;
-            G = std::min(2048, std::max(8, (want + 7) / 8 * 8));
+            G = std::min(B >= 64 ? 1024 : 2048, std::max(8, (want + 7) / 8 * 8));
         }
         if (G % 8 != 0) throw ArgError("grid must be a multiple of 8");
         const size_t vb = leo2d::vec_bytes(vec_type);
@@ -1075,13 +1078,32 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));
             }
             if ((sa.variant == 8 || sa.variant == 9) && (!spatial || B % 64 != 0)) sa.variant = 1;
-            if (B < 64 && sa.variant != 3 && sa.variant != 4) sa.variant = 0;
+            if (sa.variant == 11) {
+                // LDS budget per workgroup (default 48 KiB: three workgroups per CU)
+                int kb = 48;
+                if (const char* e = std::getenv("LEO2D_TILE_LDS_KB")) kb = std::max(8, std::min(150, std::atoi(e)));
+                const int ucap11 = kb * 1024 / (16 * (int)leo2d::vec_bytes(vec_type));
+                if (B % 16 != 0 || B > 128 || ucap11 < 40 || ucap11 > 65535 ||
+                    !build_tiles(h, drp, dcol, ucap11, spatial)) {
+                    sa.variant = 1;
+                } else {
+                    sa.tile_lds_bytes = ucap11 * 16 * (int)leo2d::vec_bytes(vec_type);
+                    if (p->grid <= 0) {
+                        int dev_id = 0, cus = 256;
+                        if (hipGetDevice(&dev_id) == hipSuccess)
+                            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+                        const int per_cu = std::max(1, (160 * 1024) / (sa.tile_lds_bytes + 1024));
+                        sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));   // part holds G slots
+                    }
+                }
+            }
+            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11) sa.variant = 0;
             kernel_used = sa.variant;
             if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
                 // larger workgroups: keep the total number of resident wavefronts
                 sa.grid = std::max(8, (sa.variant == 6 ? G / 4 : G / 2) / 8 * 8);
             }
-            if (sa.variant == 3 || sa.variant == 4 || sa.variant == 5) {
+            if (sa.variant == 3 || sa.variant == 4 || sa.variant == 5 || sa.variant == 11) {
                 sa.ntiles = h->tiles.ntiles;
                 sa.tile_row0 = (const int32_t*)h->tiles.row0.p;
                 sa.tile_u0 = (const int64_t*)h->tiles.u0.p;

@@ -27,6 +27,8 @@
 // fixed-order tree over workgroups in kpm_reduce.  No floating-point atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "kpm_common.h"
 
@@ -200,7 +202,47 @@ __device__ inline void fold_slots(float2& a)
     fold_slots(a.y);
 }
 
-template <typename HT, typename XT, int VPL, int BATCH, int NT = kThreads>
+// Streaming accesses (the matrix, T_{j-1} and T_{j+1}) can be issued with the
+// non-temporal hint so that they do not displace the gathered T_j rows in L2.
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+typedef float flt2v __attribute__((ext_vector_type(2)));
+template <bool S> __device__ inline double ld_s(const double* p) { return S ? __builtin_nontemporal_load(p) : *p; }
+template <bool S> __device__ inline float ld_s(const float* p) { return S ? __builtin_nontemporal_load(p) : *p; }
+template <bool S> __device__ inline int ld_s(const int* p) { return S ? __builtin_nontemporal_load(p) : *p; }
+template <bool S> __device__ inline double2 ld_s(const double2* p)
+{
+    if (!S) return *p;
+    const dbl2v v = __builtin_nontemporal_load(reinterpret_cast<const dbl2v*>(p));
+    return make_double2(v.x, v.y);
+}
+template <bool S> __device__ inline float2 ld_s(const float2* p)
+{
+    if (!S) return *p;
+    const flt2v v = __builtin_nontemporal_load(reinterpret_cast<const flt2v*>(p));
+    return make_float2(v.x, v.y);
+}
+template <bool S> __device__ inline void st_s(double* p, double v)
+{
+    if (S) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <bool S> __device__ inline void st_s(float* p, float v)
+{
+    if (S) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <bool S> __device__ inline void st_s(double2* p, double2 v)
+{
+    if (S) __builtin_nontemporal_store(dbl2v{v.x, v.y}, reinterpret_cast<dbl2v*>(p));
+    else *p = v;
+}
+template <bool S> __device__ inline void st_s(float2* p, float2 v)
+{
+    if (S) __builtin_nontemporal_store(flt2v{v.x, v.y}, reinterpret_cast<flt2v*>(p));
+    else *p = v;
+}
+
+template <typename HT, typename XT, int VPL, int BATCH, int NT = kThreads, bool STREAM = false>
 __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
 {
     using AT = typename acc_of<XT>::type;
@@ -244,8 +286,8 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
     int64_t q0 = row < n ? rp[row] : 0, q1 = row < n ? rp[row + 1] : 0;
     q0 = uniform64(q0);
     q1 = uniform64(q1);
-    int my_c = (row < n && q0 + lane < q1) ? col[q0 + lane] : 0;
-    HT my_h = (row < n && q0 + lane < q1) ? val[q0 + lane] : zero_h<HT>();
+    int my_c = (row < n && q0 + lane < q1) ? ld_s<STREAM>(col + q0 + lane) : 0;
+    HT my_h = (row < n && q0 + lane < q1) ? ld_s<STREAM>(val + q0 + lane) : zero_h<HT>();
 
     for (; row < n; row += stride) {
         const int next = row + stride;
@@ -282,25 +324,25 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
             }
             base += 64;
             if (base >= q1) break;
-            c_reg = (base + lane < q1) ? col[base + lane] : 0;
-            h_reg = (base + lane < q1) ? val[base + lane] : zero_h<HT>();
+            c_reg = (base + lane < q1) ? ld_s<STREAM>(col + base + lane) : 0;
+            h_reg = (base + lane < q1) ? ld_s<STREAM>(val + base + lane) : zero_h<HT>();
         }
         // next row's entries (issued before this row's epilogue traffic)
         nq0 = uniform64(nq0);
         nq1 = uniform64(nq1);
         if (next < n) {
-            my_c = (nq0 + lane < nq1) ? col[nq0 + lane] : 0;
-            my_h = (nq0 + lane < nq1) ? val[nq0 + lane] : zero_h<HT>();
+            my_c = (nq0 + lane < nq1) ? ld_s<STREAM>(col + nq0 + lane) : 0;
+            my_h = (nq0 + lane < nq1) ? ld_s<STREAM>(val + nq0 + lane) : zero_h<HT>();
         }
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             const size_t idx = (size_t)row * R + lane + 64 * v;
             const XT c = cur[idx];
             XT p;
-            if (beta) p = prev[idx];
+            if (beta) p = ld_s<STREAM>(prev + idx);
             else zero(p);
             const XT o = comb(alpha, acc[v], p);
-            out[idx] = o;
+            st_s<STREAM>(out + idx, o);
             d1[v] += nrm2(c);
             d2[v] += redot(o, c);
             if (direct) d3[v] += redot(vin[idx], c);
@@ -910,6 +952,208 @@ __global__ __launch_bounds__(kThreads) void kpm_step_smem_kernel(const int64_t* 
     }
 }
 
+// ---- variant 11: LDS-tiled step, entries broadcast within 16-lane rows ------------
+// Rows are grouped into tiles (at most kTileMaxRows consecutive rows whose
+// distinct columns fit the LDS budget).  For each group of 16 vectors, the
+// workgroup stages the tile's distinct neighbour rows (16 x element each, one
+// 256-byte run for complex FP64) in LDS once, then every row reads its
+// neighbours from LDS: the L2 sees each gathered row once per tile instead of
+// once per referencing row.  Lane = (row slot 0..3) x (vector 0..15): the 16
+// lanes of a row load 16 of its (local column, value) entries in one
+// coalesced access and pass them to each other with DPP row broadcasts
+// (row_newbcast), so no entry load sits on the dependent path of an LDS read.
+// The first chunk of a slot's next row and its T_j / T_{j-1} elements are
+// loaded before the current row is computed.  Several workgroups share a CU
+// (LDS budget set at launch), so staging in one overlaps compute in another.
+template <int K>
+__device__ inline int bc16(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+}
+template <int K>
+__device__ inline double bc16(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = bc16<K>((int)(b & 0xffffffffLL));
+    const int hi = bc16<K>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int K>
+__device__ inline float bc16(float v) { return __int_as_float(bc16<K>(__float_as_int(v))); }
+template <int K>
+__device__ inline double2 bc16(double2 v) { return make_double2(bc16<K>(v.x), bc16<K>(v.y)); }
+template <int K>
+__device__ inline float2 bc16(float2 v) { return make_float2(bc16<K>(v.x), bc16<K>(v.y)); }
+
+// acc += sum over the chunk's first `cnt` entries (entry kk held by lane kk of the 16-lane row)
+template <typename HT, typename XT, typename AT, int K>
+__device__ inline void chunk_mac(AT& acc, int cnt, int mc, HT mh, const XT* xs, int e)
+{
+    if constexpr (K < 16) {
+        if (K < cnt) {
+            const int c = bc16<K>(mc);
+            const HT h = bc16<K>(mh);
+            mac(acc, h, xs[c * 16 + e]);
+            chunk_mac<HT, XT, AT, K + 1>(acc, cnt, mc, mh, xs, e);
+        }
+    }
+}
+
+template <typename HT, typename XT, int NG>
+__global__ __launch_bounds__(kThreads) void kpm_step_tiled_dpp_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int RG = 16;
+    constexpr int SU = 8;                    // staging loads in flight per thread
+    constexpr int SLOTS = kThreads / RG;     // rows staged per pass
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int ucap = a.tile_lds_bytes / (RG * (int)sizeof(XT));
+    XT* xs = reinterpret_cast<XT*>(lds_raw);
+    const int R = RG * NG;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sub = lane >> 4;               // row slot in the wavefront
+    const int e = lane & 15;                 // vector in the group
+    const int slot = threadIdx.x >> 4;       // staging slot
+    const int rslot = wave * 4 + sub;        // row slot in the workgroup (0..15)
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const uint16_t* __restrict__ lcol = a.lcol;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1[NG], d2[NG], d3[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) { d1[g] = 0.0; d2[g] = 0.0; d3[g] = 0.0; }
+
+    for (int tile = lb; tile < a.ntiles; tile += G) {
+        const int r_begin = a.tile_row0[tile], r_end = a.tile_row0[tile + 1];
+        const int64_t u_begin = a.tile_u0[tile];
+        const int U = min((int)(a.tile_u0[tile + 1] - u_begin), ucap);
+#pragma unroll 1
+        for (int g = 0; g < NG; ++g) {
+            __syncthreads();   // every row of the previous group has read xs
+            for (int m0 = slot; m0 < U; m0 += SLOTS * SU) {
+                XT xv[SU];
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xv[m] = cur[(size_t)a.ucols[u_begin + u] * R + g * RG + e];
+                }
+#pragma unroll
+                for (int m = 0; m < SU; ++m) {
+                    const int u = m0 + m * SLOTS;
+                    if (u < U) xs[u * RG + e] = xv[m];
+                }
+            }
+            __syncthreads();
+            // rows r_begin + rslot, + 16, ...; first chunk of the first row
+            int row = r_begin + rslot;
+            int64_t q0 = 0, q1 = 0;
+            int mc = 0;
+            HT mh = zero_h<HT>();
+            if (row < r_end) {
+                q0 = rp[row];
+                q1 = rp[row + 1];
+                if (q0 + e < q1) {
+                    mc = lcol[q0 + e];
+                    mh = val[q0 + e];
+                }
+            }
+            for (; row < r_end; row += 16) {
+                const size_t idx = (size_t)row * R + g * RG + e;
+                const XT cv = cur[idx];
+                XT pv;
+                if (beta) pv = prev[idx];
+                else zero(pv);
+                const XT vv = direct ? vin[idx] : cv;
+                // next row of this slot: first chunk in flight during this row
+                const int nrow = row + 16;
+                int64_t n0 = 0, n1 = 0;
+                int nmc = 0;
+                HT nmh = zero_h<HT>();
+                if (nrow < r_end) {
+                    n0 = rp[nrow];
+                    n1 = rp[nrow + 1];
+                    if (n0 + e < n1) {
+                        nmc = lcol[n0 + e];
+                        nmh = val[n0 + e];
+                    }
+                }
+                AT acc;
+                zero(acc);
+                for (int64_t base = q0;;) {
+                    const int cnt = (int)min((int64_t)16, q1 - base);
+                    chunk_mac<HT, XT, AT, 0>(acc, cnt, mc, mh, xs, e);
+                    base += 16;
+                    if (base >= q1) break;
+                    mc = (base + e < q1) ? (int)lcol[base + e] : 0;
+                    mh = (base + e < q1) ? val[base + e] : zero_h<HT>();
+                }
+                const XT o = comb(alpha, acc, pv);
+                out[idx] = o;
+                d1[g] += nrm2(cv);
+                d2[g] += redot(o, cv);
+                if (direct) d3[g] += redot(vv, cv);
+                q0 = n0;
+                q1 = n1;
+                mc = nmc;
+                mh = nmh;
+            }
+        }
+    }
+
+    // fold the row slots of a wavefront, then the wavefronts (LDS reused)
+#pragma unroll
+    for (int off = RG; off < 64; off <<= 1) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            d1[g] += __shfl_xor(d1[g], off);
+            d2[g] += __shfl_xor(d2[g], off);
+            d3[g] += __shfl_xor(d3[g], off);
+        }
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds_raw);   // [kWaves][R][3]
+    if (sub == 0) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int r = g * RG + e;
+            red[(wave * R + r) * 3 + 0] = d1[g];
+            red[(wave * R + r) * 3 + 1] = d2[g];
+            red[(wave * R + r) * 3 + 2] = d3[g];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        double sum = red[t];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) sum += red[w * R * 3 + t];
+        a.part[(size_t)lb * R * 3 + t] = sum;
+    }
+}
+
+template <typename HT, typename XT, int NG>
+hipError_t launch_tiled_dpp(const StepArgs& a, hipStream_t s)
+{
+    // LDS: staged rows, at least the [kWaves][R][3] doubles of the final fold
+    const size_t shmem = std::max((size_t)a.tile_lds_bytes, sizeof(double) * kWaves * 16 * NG * 3);
+    hipError_t e = hipFuncSetAttribute((const void*)kpm_step_tiled_dpp_kernel<HT, XT, NG>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kpm_step_tiled_dpp_kernel<HT, XT, NG>), dim3(a.grid), dim3(kThreads), shmem, s, a);
+    return hipGetLastError();
+}
+
 template <typename HT, typename XT, int NG>
 hipError_t launch_tiled(const StepArgs& a, hipStream_t s)
 {
@@ -931,6 +1175,15 @@ hipError_t launch_typed(const StepArgs& a, hipStream_t s)
     if (a.variant == 5 && a.ntiles > 0) {
         if (a.R == 64) return launch_tiled_wave<HT, XT, 1>(a, s);
         if (a.R == 128) return launch_tiled_wave<HT, XT, 2>(a, s);
+    }
+    if (a.variant == 11 && a.ntiles > 0) {
+        switch (a.R) {
+        case 16: return launch_tiled_dpp<HT, XT, 1>(a, s);
+        case 32: return launch_tiled_dpp<HT, XT, 2>(a, s);
+        case 64: return launch_tiled_dpp<HT, XT, 4>(a, s);
+        case 128: return launch_tiled_dpp<HT, XT, 8>(a, s);
+        default: break;
+        }
     }
     if (a.variant == 4 && a.ntiles > 0) {
         switch (a.R) {
@@ -967,6 +1220,8 @@ hipError_t launch_typed(const StepArgs& a, hipStream_t s)
         hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8, 1024>), dim3(a.grid), dim3(1024), 0, s, a);
     else if (LPR == 64 && a.variant == 7)
         hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8, 512>), dim3(a.grid), dim3(512), 0, s, a);
+    else if (LPR == 64 && a.variant == 10)
+        hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8, kThreads, true>), dim3(a.grid), dim3(kThreads), 0, s, a);
     else if (LPR == 64 && (a.variant == 1 || a.variant >= 8))
         hipLaunchKernelGGL((kpm_step_wave_kernel<HT, XT, VPL, 8>), dim3(a.grid), dim3(kThreads), 0, s, a);
     else if (LPR == 64 && a.variant == 2)
