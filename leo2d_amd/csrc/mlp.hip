@@ -137,6 +137,128 @@ __global__ __launch_bounds__(kT) void gemm_bf16_kernel(GemmArgs g)
         }
 }
 
+// ---- vectorised GEMM (gfx950 16x16x32 bf16 MFMA) -------------------------------
+// Same product and epilogues.  Operands are staged as 16-byte runs of 8 bf16
+// (along k when that is the contiguous direction, else along m / n), A as
+// As[m][k] and B transposed as Bt[n][k] in LDS, so every MFMA operand is one
+// 16-byte LDS read.  The next k-tile is loaded into registers while the
+// current one is multiplied.  Needs K, the contiguous strides and the base
+// addresses to be multiples of 8 elements (checked by the launcher).
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+constexpr int kPad = 8;
+
+template <bool AK>
+__device__ inline u16x8 load_a(const GemmArgs& g, int m0, int k0, int tid, int& mm, int& kk)
+{
+    // AK: 8 consecutive k of one row m (thread -> (m, k-run)); else 8 consecutive m of one k
+    if (AK) {
+        mm = tid >> 2;
+        kk = (tid & 3) * 8;
+    } else {
+        kk = tid >> 3;
+        mm = (tid & 7) * 8;
+    }
+    const int m = m0 + mm, k = k0 + kk;
+    u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < g.K && m < g.M) v = *reinterpret_cast<const u16x8*>(g.A + (int64_t)m * g.sam + (int64_t)k * g.sak);
+    return v;
+}
+template <bool BK>
+__device__ inline u16x8 load_b(const GemmArgs& g, int n0, int k0, int tid, int& nn, int& kk)
+{
+    if (BK) {
+        nn = tid >> 2;
+        kk = (tid & 3) * 8;
+    } else {
+        kk = tid >> 3;
+        nn = (tid & 7) * 8;
+    }
+    const int n = n0 + nn, k = k0 + kk;
+    u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < g.K && n < g.N) v = *reinterpret_cast<const u16x8*>(g.B + (int64_t)k * g.sbk + (int64_t)n * g.sbn);
+    return v;
+}
+
+template <int EPI, bool AK, bool BK>
+__global__ __launch_bounds__(kT) void gemm_bf16_v2_kernel(GemmArgs g)
+{
+    __shared__ __align__(16) uint16_t As[kTM][kTK + kPad];
+    __shared__ __align__(16) uint16_t Bt[kTN][kTK + kPad];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    int am, ak, bn, bk;
+    u16x8 ra = load_a<AK>(g, m0, 0, tid, am, ak);
+    u16x8 rb = load_b<BK>(g, n0, 0, tid, bn, bk);
+    for (int k0 = 0; k0 < g.K; k0 += kTK) {
+        __syncthreads();   // previous tile fully read
+        if (AK) {
+            *reinterpret_cast<u16x8*>(&As[am][ak]) = ra;
+        } else {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) As[am + t][ak] = ra[t];
+        }
+        if (BK) {
+            *reinterpret_cast<u16x8*>(&Bt[bn][bk]) = rb;
+        } else {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) Bt[bn + t][bk] = rb[t];
+        }
+        __syncthreads();
+        if (k0 + kTK < g.K) {   // next tile in flight during the products
+            ra = load_a<AK>(g, m0, k0 + kTK, tid, am, ak);
+            rb = load_b<BK>(g, n0, k0 + kTK, tid, bn, bk);
+        }
+        bf16x8 af[2], bfr[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            af[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            bfr[j] = *reinterpret_cast<const bf16x8*>(&Bt[wn * 32 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 32 + j * 16 + (lane & 15);
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + t;
+                if (row >= g.M) continue;
+                const int64_t o = (int64_t)row * g.scm + col;
+                const float v = acc[i][j][t];
+                if (EPI == EPI_STORE_F32) {
+                    g.C32[o] = v;
+                } else if (EPI == EPI_BIAS_F32) {
+                    g.C32[o] = v + g.bias[col];
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + g.bias[col];
+                    g.Z[o] = z;
+                    g.C16[o] = f2bf(gelu(z));
+                } else {
+                    g.C16[o] = f2bf(v * gelu_grad(g.Z[o]));
+                }
+            }
+        }
+}
+
 // dY = scale (Y - T) as bf16 and per-row squared-error sums (FP64, fixed order)
 __global__ __launch_bounds__(kT) void mse_grad_kernel(const float* Y, const float* T, int rows, int cols, float scale,
                                                       uint16_t* dY, double* row_loss)
@@ -241,10 +363,46 @@ int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4
 
 }  // namespace
 
+template <int EPI>
+void launch_v2(const GemmArgs& g, bool ak, bool bk, dim3 grid, hipStream_t s)
+{
+    if (ak && bk) hipLaunchKernelGGL((gemm_bf16_v2_kernel<EPI, true, true>), grid, dim3(kT), 0, s, g);
+    else if (ak) hipLaunchKernelGGL((gemm_bf16_v2_kernel<EPI, true, false>), grid, dim3(kT), 0, s, g);
+    else if (bk) hipLaunchKernelGGL((gemm_bf16_v2_kernel<EPI, false, true>), grid, dim3(kT), 0, s, g);
+    else hipLaunchKernelGGL((gemm_bf16_v2_kernel<EPI, false, false>), grid, dim3(kT), 0, s, g);
+}
+
+// the 16-byte staging needs: K a multiple of 8; a unit stride along k or along
+// m (A) / n (B) with the other stride a multiple of 8; for m- / n-runs, M / N a
+// multiple of 8; base addresses on 16 bytes
+bool v2_ok(const GemmArgs& g, bool& ak, bool& bk)
+{
+    if (g.K % 8 != 0) return false;
+    if (((uintptr_t)g.A | (uintptr_t)g.B) % 16 != 0) return false;
+    if (g.sak == 1 && g.sam % 8 == 0) ak = true;
+    else if (g.sam == 1 && g.sak % 8 == 0 && g.M % 8 == 0) ak = false;
+    else return false;
+    if (g.sbk == 1 && g.sbn % 8 == 0) bk = true;
+    else if (g.sbn == 1 && g.sbk % 8 == 0 && g.N % 8 == 0) bk = false;
+    else return false;
+    return true;
+}
+
 hipError_t launch_gemm_bf16(const GemmArgs& g, int epi, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
     const dim3 grid((g.N + kTN - 1) / kTN, (g.M + kTM - 1) / kTM);
+    bool ak = false, bk = false;
+    if (v2_ok(g, ak, bk)) {
+        switch (epi) {
+        case EPI_STORE_F32: launch_v2<EPI_STORE_F32>(g, ak, bk, grid, s); break;
+        case EPI_BIAS_F32: launch_v2<EPI_BIAS_F32>(g, ak, bk, grid, s); break;
+        case EPI_BIAS_GELU: launch_v2<EPI_BIAS_GELU>(g, ak, bk, grid, s); break;
+        case EPI_GELU_BWD: launch_v2<EPI_GELU_BWD>(g, ak, bk, grid, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (epi) {
     case EPI_STORE_F32: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_STORE_F32>, grid, dim3(kT), 0, s, g); break;
     case EPI_BIAS_F32: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_BIAS_F32>, grid, dim3(kT), 0, s, g); break;
