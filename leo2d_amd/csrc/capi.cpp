@@ -106,6 +106,15 @@ struct leo2d_handle {
         int ucap = 0, ntiles = 0;
         DevBuf row0, u0, ucols, lcol;
     } tiles;
+    // row groups: kGroupRows consecutive rows of the recursion matrix with their
+    // merged column list and a value per (column, row) (zero where absent)
+    struct Groups {
+        bool valid = false, spatial = false;
+        int ngroups = 0;
+        int64_t nent = 0;
+        DevBuf rp, col, val, val32;
+        bool val32_valid = false;
+    } groups;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
     // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
@@ -264,6 +273,7 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->pcsr_valid = false;
     h->pval32_valid = false;
     h->tiles.valid = false;
+    h->groups.valid = false;
     h->tiles.failed = false;
 }
 
@@ -331,6 +341,72 @@ bool build_tiles(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int u
     T.valid = true;
     T.failed = false;
     return true;
+}
+
+// Row groups of the recursion matrix: rows g*K .. g*K+K-1 share one sorted
+// column list (the union of their columns) and store K values per column,
+// zero where a row has no entry; duplicate (row, column) entries are summed.
+// kCsrPad trailing entries (column 0, values 0) allow whole-chunk loads.
+void build_groups(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, const void* dval, bool spatial)
+{
+    auto& Gr = h->groups;
+    if (Gr.valid && Gr.spatial == spatial) return;
+    constexpr int K = leo2d::kGroupRows;
+    const int n = h->n;
+    const int w = h->is_complex ? 2 : 1;
+    hipStream_t s = h->stream;
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col((size_t)h->nnz);
+    std::vector<double> val((size_t)w * h->nnz);
+    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "groups");
+    if (h->nnz) {
+        check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "groups");
+        check(hipMemcpyAsync(val.data(), dval, sizeof(double) * w * h->nnz, hipMemcpyDeviceToHost, s), "groups");
+    }
+    check(hipStreamSynchronize(s), "groups");
+    const int ng = (n + K - 1) / K;
+    std::vector<int64_t> grp(ng + 1, 0);
+    std::vector<int32_t> gcol;
+    std::vector<double> gval;
+    gcol.reserve((size_t)h->nnz);
+    gval.reserve((size_t)h->nnz * K * w);
+    std::vector<std::pair<int32_t, int64_t>> ent;   // (column, k * nnz-index)
+    for (int g = 0; g < ng; ++g) {
+        ent.clear();
+        for (int k = 0; k < K && g * K + k < n; ++k) {
+            const int r = g * K + k;
+            for (int64_t q = rp[r]; q < rp[r + 1]; ++q) ent.push_back({col[q], (int64_t)k << 40 | q});
+        }
+        std::sort(ent.begin(), ent.end(),
+                  [](const std::pair<int32_t, int64_t>& a, const std::pair<int32_t, int64_t>& b) {
+                      return a.first < b.first;
+                  });
+        for (size_t e = 0; e < ent.size();) {
+            const int32_t c = ent[e].first;
+            const size_t base = gval.size();
+            gcol.push_back(c);
+            gval.resize(base + (size_t)K * w, 0.0);
+            for (; e < ent.size() && ent[e].first == c; ++e) {
+                const int k = (int)(ent[e].second >> 40);
+                const int64_t q = ent[e].second & ((1LL << 40) - 1);
+                for (int t = 0; t < w; ++t) gval[base + (size_t)k * w + t] += val[(size_t)w * q + t];
+            }
+        }
+        grp[g + 1] = (int64_t)gcol.size();
+    }
+    Gr.nent = (int64_t)gcol.size();
+    for (int p = 0; p < leo2d::kCsrPad; ++p) {
+        gcol.push_back(0);
+        gval.resize(gval.size() + (size_t)K * w, 0.0);
+    }
+    to_device(Gr.rp, grp.data(), sizeof(int64_t) * grp.size(), s, "group rows");
+    to_device(Gr.col, gcol.data(), sizeof(int32_t) * gcol.size(), s, "group columns");
+    to_device(Gr.val, gval.data(), sizeof(double) * gval.size(), s, "group values");
+    check(hipStreamSynchronize(s), "groups");
+    Gr.ngroups = ng;
+    Gr.spatial = spatial;
+    Gr.valid = true;
+    Gr.val32_valid = false;
 }
 
 // Morton (Z-order) rank of every orbital by in-plane position on a 1.42 A
@@ -1078,6 +1154,31 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));
             }
             if ((sa.variant == 8 || sa.variant == 9) && (!spatial || B % 64 != 0)) sa.variant = 1;
+            if (sa.variant == 12) {
+                if (!spatial || B != 64) {
+                    sa.variant = 1;
+                } else {
+                    build_groups(h, drp, dcol, h->pval.p, spatial);
+                    auto& Gr = h->groups;
+                    const void* gv = Gr.val.p;
+                    if (mixed) {
+                        if (!Gr.val32_valid) {
+                            const size_t count =
+                                (size_t)(Gr.nent + leo2d::kCsrPad) * leo2d::kGroupRows * (h->is_complex ? 2 : 1);
+                            dalloc(Gr.val32, sizeof(float) * count, "fp32 group values");
+                            check(leo2d::launch_convert_f64_to_f32((const double*)Gr.val.p, (float*)Gr.val32.p,
+                                                                   (int64_t)count, s),
+                                  "fp32 group values");
+                            Gr.val32_valid = true;
+                        }
+                        gv = Gr.val32.p;
+                    }
+                    sa.ngroups = Gr.ngroups;
+                    sa.grp_rp = (const int64_t*)Gr.rp.p;
+                    sa.grp_col = (const int32_t*)Gr.col.p;
+                    sa.grp_val = gv;
+                }
+            }
             if (sa.variant == 11) {
                 // LDS budget per workgroup (default 48 KiB: three workgroups per CU)
                 int kb = 48;

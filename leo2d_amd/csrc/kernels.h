@@ -45,10 +45,17 @@ struct StepArgs {
     const uint16_t* lcol;       // [nnz] tile-local column index
     int tile_lds_bytes;         // variant 5: LDS bytes for the staged x rows
     int xcd_rows;               // variants 1/2/8/9: rows per XCD slice (0 = interleave rows over the grid)
+    // variant 12: row groups of kGroupRows rows, merged columns, kGroupRows values per column
+    int ngroups;
+    const int64_t* grp_rp;      // [ngroups + 1]
+    const int32_t* grp_col;     // [entries + kCsrPad]
+    const void* grp_val;        // [entries + kCsrPad][kGroupRows]
 };
 
 // trailing (column 0, value 0) entries after nnz in the spatially ordered CSR
 constexpr int kCsrPad = 16;
+// rows per row group (variant 12)
+constexpr int kGroupRows = 4;
 
 // LDS budget of one tiled workgroup (two workgroups per CU)
 constexpr int kTileLdsBytes = 72 * 1024;

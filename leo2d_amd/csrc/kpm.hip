@@ -1154,6 +1154,81 @@ hipError_t launch_tiled_dpp(const StepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---- variant 12: row groups -------------------------------------------------------
+// A wavefront owns kGroupRows consecutive rows (spatial order) at once.  Their
+// column lists are merged ahead of time, so every distinct neighbour row is
+// gathered once for the group and multiplied into all kGroupRows accumulators
+// (values are zero where a row lacks the column).  On the 1.05 deg 8x8 cell
+// this gathers 57 % of the rows the per-row kernels gather, for 2.3x the value
+// bytes.  Columns and values come through the scalar path in BATCH-entry chunks.
+template <typename HT, typename XT, int BATCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_group_kernel(const int64_t* __restrict__ grp,
+                                                                  const int32_t* __restrict__ gcol,
+                                                                  const HT* __restrict__ gval,
+                                                                  const XT* __restrict__ cur, XT* __restrict__ po,
+                                                                  const XT* __restrict__ vin, int n, int ng,
+                                                                  double alpha, int beta, double* __restrict__ part)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int K = kGroupRows;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    const int stride = G * kWaves;
+    const bool direct = vin != nullptr;
+
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    for (int g = __builtin_amdgcn_readfirstlane(lb * kWaves + wave); g < ng; g += stride) {
+        const int64_t q0 = grp[g], q1 = grp[g + 1];
+        AT acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) zero(acc[k]);
+        for (int64_t q = q0; q < q1; q += BATCH) {
+            const ColChunk<BATCH> cc = *reinterpret_cast<const ColChunk<BATCH>*>(gcol + q);
+            const ValChunk<HT, BATCH * K> vc = *reinterpret_cast<const ValChunk<HT, BATCH * K>*>(gval + q * K);
+            XT x[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) x[u] = cur[(size_t)cc.c[u] * 64 + lane];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u)
+                if (q + u < q1) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) mac(acc[k], vc.h[u * K + k], x[u]);
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int row = g * K + k;
+            if (row >= n) break;
+            const size_t idx = (size_t)row * 64 + lane;
+            const XT cv = cur[idx];
+            XT p;
+            if (beta) p = po[idx];
+            else zero(p);
+            const XT o = comb(alpha, acc[k], p);
+            po[idx] = o;
+            d1 += nrm2(cv);
+            d2 += redot(o, cv);
+            if (direct) d3 += redot(vin[idx], cv);
+        }
+    }
+
+    __shared__ double red[kWaves][64][3];
+    red[wave][lane][0] = d1;
+    red[wave][lane][1] = d2;
+    red[wave][lane][2] = d3;
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double sum = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) sum += red[w][r][k];
+        part[((size_t)lb * 64 + r) * 3 + k] = sum;
+    }
+}
+
 template <typename HT, typename XT, int NG>
 hipError_t launch_tiled(const StepArgs& a, hipStream_t s)
 {
@@ -1202,6 +1277,14 @@ hipError_t launch_typed(const StepArgs& a, hipStream_t s)
         case 128: return launch_tiled<HT, XT, 8>(a, s);
         default: break;
         }
+    }
+    if (LPR == 64 && VPL == 1 && a.variant == 12 && a.ngroups > 0 && a.prev == a.out) {
+        constexpr int GB = sizeof(HT) >= 16 ? 4 : 8;   // entries per scalar chunk
+        hipLaunchKernelGGL((kpm_step_group_kernel<HT, XT, GB>), dim3(a.grid), dim3(kThreads), 0, s, a.grp_rp,
+                           a.grp_col, static_cast<const HT*>(a.grp_val), static_cast<const XT*>(a.cur),
+                           static_cast<XT*>(a.out), static_cast<const XT*>(a.v), a.n, a.ngroups, a.alpha, a.beta,
+                           a.part);
+        return hipGetLastError();
     }
     if (LPR == 64 && (a.variant == 8 || a.variant == 9) && a.prev == a.out) {
         const XT* cur = static_cast<const XT*>(a.cur);

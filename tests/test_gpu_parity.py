@@ -252,7 +252,7 @@ def test_spatial_order_matches_natural_order(dev, name):
     assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12"])
 @pytest.mark.parametrize("R", [4, 16, 64, 128])
 def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     """Every step-kernel variant (lane-per-vector, wavefront-per-row with 8/16
