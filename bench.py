@@ -84,7 +84,7 @@ def main():
                     help="c1: 1.05 deg 8x8 tBLG (headline); c2: 32 uneven cells x 8 k-points batched; "
                          "c3: large heterostrained flake, FP32 recursion; c4: learned local-moment surrogate")
     ap.add_argument("--c3-radius", type=float, default=1409.0, help="c3: flake radius (A); 1409 A -> 4.7 M orbitals")
-    ap.add_argument("--c4-train", type=int, default=20000, help="c4: training configurations")
+    ap.add_argument("--c4-train", type=int, default=200000, help="c4: training samples (orbitals)")
     ap.add_argument("--c4-M", type=int, default=1024, help="c4: local moments per sublattice")
     ap.add_argument("--c4-epochs", type=int, default=100)
     ap.add_argument("--c4-theta-min", type=float, default=0.3,
@@ -379,30 +379,31 @@ def run_c3(args):
 
 
 def run_c4(args):
-    """Learned local-moment surrogate: exact local moments of random stacking
-    configurations (shift, strain, twist in [theta_min, 3] deg) from batched KPM with
-    target starts, an ensemble of four 64 -> 3 x 256 GELU -> 4 M networks
-    trained on the GPU with the BF16 MFMA kernels, then inference on every
-    orbital of a twisted flake with exact local KPM where the ensemble spread
-    exceeds 1e-3."""
+    """Learned local-moment surrogate.  Exact local moments of random orbitals
+    of random commensurate cells (twist log-uniform in [theta_min, 3] deg,
+    random shift of sheet 2, strain N(0, 0.3 %)) from batched KPM with target
+    starts; an ensemble of four 72 -> 3 x 256 GELU -> M networks trained on the
+    GPU with the BF16 MFMA kernels; then inference on every orbital of a
+    twisted flake, with exact local KPM where the ensemble spread exceeds 1e-3."""
     import leo2d_amd as L
     from leo2d_amd import surrogate as S
     from leo2d_amd import workloads as W
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    M = args.c4_M
-    n_train, n_val = args.c4_train, max(256, args.c4_train // 20)
+    M, R = args.c4_M, 32
+    n_val = max(1024, args.c4_train // 20)
+    n_jobs = (args.c4_train + n_val + R - 1) // R
     rng = np.random.default_rng(4)
     dev = L.Device(0)
-    ex = S.ExactLocalKPM(dev, M, chunk=256)
-    # log-uniform twists: as many training cells per octave of angle, down to the flake's twist
-    cfg = S.sample_configs(n_train + n_val, rng, theta_range=(args.c4_theta_min, 3.0), log_theta=True)
+    ex = S.ExactLocalKPM(dev, M, chunk=128)
     t0 = time.perf_counter()
-    mom = ex.moments(cfg)
+    d = ex.samples(n_jobs, rng, R=R, theta_range=(args.c4_theta_min, 3.0))
     t_gen = time.perf_counter() - t0
-    log(f"c4: {len(cfg)} exact configurations (M={M}) in {t_gen:.1f}s ({len(cfg) / t_gen:.0f} configs/s)")
-    x = L.mlp_features(cfg)
-    t = mom.reshape(len(cfg), -1).astype(np.float32)
+    n_all = len(d["block"])
+    n_train = n_all - n_val
+    log(f"c4: {n_all} exact samples ({n_jobs} cells x {R} orbitals, M={M}) in {t_gen:.1f}s ({n_all / t_gen:.0f}/s)")
+    x = S.site_features(d["cfg"], d["block"])
+    t = d["moments"].astype(np.float32)
     t0 = time.perf_counter()
     nets, losses = S.train_ensemble(x[:n_train], t[:n_train], members=4, hidden=256, layers=3,
                                     epochs=args.c4_epochs, batch=512, lr=1e-3)
@@ -411,18 +412,19 @@ def run_c4(args):
     fwd_flops = 2 * sum(dims[l] * dims[l + 1] for l in range(len(dims) - 1))
     train_flops = 3 * fwd_flops * n_train * args.c4_epochs * len(nets)
     log(f"c4: trained 4 x {args.c4_epochs} epochs in {t_train:.1f}s ({train_flops / t_train / 1e12:.1f} TFLOP/s "
-        f"bf16 incl. host sequencing), final losses {[float(l[-1]) for l in losses]}")
+        f"bf16 incl. host sequencing), final losses {[round(float(l[-1]), 4) for l in losses]}")
     # held-out error, in moments and in the local DoS (Jackson kernel, a = 10.5 eV, b = 0)
-    mean, spread = L.mlp_ensemble(nets, x[n_train:])
-    pred = mean.reshape(-1, 4, M).astype(np.float64)
-    exv = mom[n_train:]
-    rms = float(np.sqrt(np.mean((pred - exv) ** 2)))
-    _, rho_p = dev.dos(pred.reshape(-1, M), S.RESCALE, 0.0)
-    _, rho_e = dev.dos(exv.reshape(-1, M), S.RESCALE, 0.0)
-    ddos = np.max(np.abs(rho_p - rho_e).reshape(n_val, -1), axis=1)
-    base = float(np.sqrt(np.mean((t[:n_train].mean(axis=0).reshape(4, M) - exv) ** 2)))
-    log(f"c4: held-out rms moment error {rms:.3e} (training-mean predictor {base:.3e}), "
-        f"median max|dLDOS| {np.median(ddos):.3e} states/eV")
+    mean, spread_v = L.mlp_ensemble(nets, x[n_train:])
+    exv = d["moments"][n_train:]
+    bv = d["block"][n_train:]
+    rms = float(np.sqrt(np.mean((mean - exv) ** 2)))
+    bm = np.stack([t[:n_train][d["block"][:n_train] == k].mean(axis=0) for k in range(4)])
+    base = float(np.sqrt(np.mean((bm[bv] - exv) ** 2)))
+    _, rho_p = dev.dos(mean.astype(np.float64), S.RESCALE, 0.0)
+    _, rho_e = dev.dos(exv, S.RESCALE, 0.0)
+    ddos = np.max(np.abs(rho_p - rho_e), axis=1)
+    log(f"c4: held-out rms moment error {rms:.3e} (per-block mean predictor {base:.3e}), "
+        f"median max|dLDOS| {np.median(ddos):.3e} states/eV, spread median {np.median(spread_v):.2e}")
 
     # inference on every orbital of the flake
     t0 = time.perf_counter()
@@ -430,12 +432,11 @@ def run_c4(args):
     scfg, blk = S.flake_site_configs(fgeom, args.c4_theta)
     t_geo = time.perf_counter() - t0
     t0 = time.perf_counter()
-    xs = L.mlp_features(scfg)
+    xs = S.site_features(scfg, blk)
     t_feat = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, sp4 = L.mlp_ensemble(nets, xs, with_mean=False, blocks=4)
+    _, sp = L.mlp_ensemble(nets, xs, with_mean=False)
     t_inf = time.perf_counter() - t0
-    sp = sp4[np.arange(len(xs)), blk]          # disagreement on each site's own sublattice
     flagged = np.nonzero(sp > 1e-3)[0]
     log(f"c4: flake {args.c4_theta} deg, {len(scfg)} sites: features {t_feat:.2f}s, ensemble {t_inf:.2f}s "
         f"({len(scfg) / t_inf:.3e} sites/s), spread median {np.median(sp):.2e}, {len(flagged)} above 1e-3")
@@ -443,12 +444,10 @@ def run_c4(args):
     if len(flagged):
         pick = rng.choice(flagged, size=min(args.c4_fallback_sample, len(flagged)), replace=False)
         t0 = time.perf_counter()
-        exs = ex.moments(scfg[pick])
+        exs = ex.moments_at(scfg[pick], blk[pick])
         t_fb = time.perf_counter() - t0
         ms, _ = L.mlp_ensemble(nets, xs[pick])
-        ps = ms.reshape(-1, 4, M)
-        rows = np.arange(len(pick))
-        err = np.abs(ps[rows, blk[pick]] - exs[rows, blk[pick]])
+        err = np.abs(ms - exs)
         per = t_fb / len(pick)
         fb = {"flagged": int(len(flagged)), "exact_sample": int(len(pick)), "exact_s_per_site": per,
               "projected_fallback_s": per * len(flagged),
@@ -456,7 +455,22 @@ def run_c4(args):
               "mlp_rms_moment_err_at_sample": float(np.sqrt(np.mean(err ** 2)))}
         log(f"c4: exact local KPM at {args.c4_theta} deg: {per * 1e3:.1f} ms/site on {len(pick)} flagged sites; "
             f"network error there rms {fb['mlp_rms_moment_err_at_sample']:.3e}")
-    exact_all = fb.get("exact_s_per_site", t_gen / len(cfg)) * len(scfg)
+    # the networks' error on random flake sites (flagged or not), and the exact cost per site
+    pick = rng.choice(len(scfg), size=min(args.c4_fallback_sample, len(scfg)), replace=False)
+    t0 = time.perf_counter()
+    exr = ex.moments_at(scfg[pick], blk[pick])
+    t_rand = time.perf_counter() - t0
+    mr, _ = L.mlp_ensemble(nets, xs[pick])
+    err_r = np.abs(mr - exr)
+    _, rho_p = dev.dos(mr.astype(np.float64), S.RESCALE, 0.0)
+    _, rho_e = dev.dos(exr, S.RESCALE, 0.0)
+    random_sites = {"sites": int(len(pick)), "rms_moment_err": float(np.sqrt(np.mean(err_r ** 2))),
+                    "max_abs_moment_err": float(err_r.max()),
+                    "median_max_dLDOS": float(np.median(np.max(np.abs(rho_p - rho_e), axis=1)))}
+    log(f"c4: on {len(pick)} random flake sites the ensemble's rms moment error is "
+        f"{random_sites['rms_moment_err']:.3e}, median max|dLDOS| {random_sites['median_max_dLDOS']:.3e} states/eV")
+    per_site_exact = fb.get("exact_s_per_site", t_rand / len(pick))
+    exact_all = per_site_exact * len(scfg)
     surrogate_total = t_feat + t_inf + fb.get("projected_fallback_s", 0.0)
     for m in nets:
         m.close()
@@ -465,15 +479,16 @@ def run_c4(args):
         "metric": "learned local-moment surrogate: sites/s and error vs exact local KPM", "value": len(scfg) / t_inf,
         "unit": "sites/s (4-network ensemble inference)", "n_gpus": 1, "higher_is_better": True,
         "dtype": "bf16 weights, fp32 accumulation", "data": "synthetic: configurations and flake built in-process",
-        "config": {"workload": f"MLP 64 -> 3x256 GELU -> 4x{M}, ensemble of 4, {n_train} training configurations "
-                               f"(theta {args.c4_theta_min}-3 deg, strain N(0, 0.3%)), inference on a {args.c4_theta} deg flake "
-                               f"of radius {args.c4_radius} A", "sites": int(len(scfg))},
-        "exact_generation": {"configs": int(len(cfg)), "seconds": t_gen},
+        "config": {"workload": f"MLP 72 -> 3x256 GELU -> {M}, ensemble of 4, {n_train} training samples "
+                               f"(theta {args.c4_theta_min}-3 deg log-uniform, strain N(0, 0.3%)), inference on a "
+                               f"{args.c4_theta} deg flake of radius {args.c4_radius} A", "sites": int(len(scfg))},
+        "exact_generation": {"samples": int(n_all), "seconds": t_gen},
         "training": {"seconds": t_train, "epochs": args.c4_epochs, "tflops": train_flops / t_train / 1e12,
                      "final_loss": [float(l[-1]) for l in losses]},
-        "heldout": {"rms_moment_err": rms, "training_mean_rms": base, "median_max_dLDOS": float(np.median(ddos))},
+        "heldout": {"rms_moment_err": rms, "per_block_mean_rms": base, "median_max_dLDOS": float(np.median(ddos)),
+                    "median_spread": float(np.median(spread_v))},
         "inference": {"features_s": t_feat, "ensemble_s": t_inf, "geometry_s": t_geo},
-        "fallback": fb,
+        "fallback": fb, "random_flake_sites": random_sites,
         "gain": {"exact_local_kpm_all_sites_s": exact_all, "surrogate_with_fallback_s": surrogate_total,
                  "speedup": exact_all / surrogate_total},
     }), flush=True)

@@ -1,21 +1,29 @@
 """Learned local-moment surrogate with exact local-KPM fallback.
 
 A local stacking configuration c = (b1, b2, exx, eyy, exy, theta) is: the
-shift of sheet 2 against sheet 1 in fractional lattice units, a symmetric
-strain tensor applied to both sheets, and the twist angle in degrees.  Its
-exact local moments are the local-DoS Chebyshev moments of the four orbitals
-of the unit cell at the origin (sheet 1 A, B; sheet 2 A, B) in the
-commensurate (i+1, i) twisted-bilayer cell whose twist is nearest theta, with
-sheet 2 shifted by b (setConfigPositions shifts, src/locality.cpp:3577-3660)
-and the strain applied as UNIFORM_STRAIN (src/locality.cpp:3849-3857), on the
-GrapheneSimple couplings, rescaled by a fixed a = 10.5 eV.  They come from the
-library's batched KPM with target starts (one job per configuration).
+shift of sheet 2 against sheet 1 at a unit cell, in fractional lattice units
+of sheet 2; a symmetric strain tensor applied to both sheets; and the twist
+in degrees.  In a rigidly twisted bilayer the shift is the linear field
+b(r) = A^-1 (I - R^-1) r + s (mod 1), with r the unit cell's position, A the
+lattice, R the twist and s the shift applied to sheet 2.
 
-An ensemble of MLPs (64 Fourier features -> 3 x 256 GELU -> 4 M moments,
-bfloat16 weights, FP32 accumulation, trained on the GPU) predicts them; a
-configuration whose ensemble spread (largest standard deviation over the
-outputs) exceeds a threshold falls back to the exact computation, which also
-scores the networks' error there.
+Exact local moments of an orbital come from a commensurate (i+1, i)
+twisted-bilayer cell whose twist is nearest theta, with sheet 2 shifted by s
+(setConfigPositions shifts, src/locality.cpp:3577-3660), the strain applied
+as UNIFORM_STRAIN (src/locality.cpp:3849-3857), GrapheneSimple couplings and a
+fixed rescale a = 10.5 eV, through the library's batched KPM with target
+starts.  Every orbital of such a cell is one sample: its configuration is b
+at its own unit cell, and its sublattice (sheet 1 A, B; sheet 2 A, B) is the
+block index.  Two orbitals with the same configuration and block have the
+same local environment (the stacking field around each is the same linear
+field), so a configuration can also be evaluated on demand at an origin-cell
+orbital with the matching shift; that is the fallback.
+
+An ensemble of MLPs (64 Fourier features + a one-hot block -> 3 x 256 GELU ->
+M moments; bfloat16 weights, FP32 accumulation, trained on the GPU) predicts
+the moments.  A site whose ensemble spread (RMS over the M moments of the
+ensemble standard deviation) exceeds a threshold falls back to the exact
+computation, which also scores the networks' error there.
 """
 from __future__ import annotations
 
@@ -30,6 +38,41 @@ from leo2d_amd import workloads as W
 GR_A = 2.4768                      # graphene lattice constant of the coupling model (A)
 LAT = np.array([[GR_A, 0.0], [0.5 * GR_A, 0.5 * math.sqrt(3.0) * GR_A]])   # rows a1, a2
 RESCALE = 10.5                     # eV, above every Gershgorin bound of the strained cells used
+N_IN = L.MLP_FEATURES + 8          # features + one-hot block (4) + zero padding to a multiple of 8
+TAU_B = np.array([0.5 * GR_A, GR_A / (2.0 * math.sqrt(3.0))])
+
+
+def rot(t):
+    return np.array([[math.cos(t), -math.sin(t)], [math.sin(t), math.cos(t)]])
+
+
+def site_features(cfg, block) -> np.ndarray:
+    """[n][N_IN] network inputs: the configuration's Fourier features and the block."""
+    cfg = np.atleast_2d(cfg)
+    block = np.asarray(block, dtype=np.int64)
+    f = np.zeros((cfg.shape[0], N_IN), dtype=np.float32)
+    f[:, :L.MLP_FEATURES] = L.mlp_features(cfg)
+    f[np.arange(cfg.shape[0]), L.MLP_FEATURES + block] = 1.0
+    return f
+
+
+def local_shifts(geom, shift=(0.0, 0.0)):
+    """Stacking shift b [n][2] (mod 1) at every orbital's unit cell and the
+    orbital's block 2 sheet + sublattice, for the geometry with sheet 2
+    shifted by `shift` (lattice units of sheet 2).  Sheet 1 is unrotated."""
+    i2g, i2p = geom.arrays()
+    grid, pos = i2g.reshape(-1, 4), i2p.reshape(-1, 3)[:, :2]
+    th = geom.info.angle[1]
+    R2 = rot(th)
+    Rinv = rot(-th)
+    s = np.asarray(shift, dtype=np.float64)
+    sheet2 = grid[:, 3] == 1
+    off = np.where((grid[:, 2] == 1)[:, None], np.where(sheet2[:, None], (R2 @ TAU_B)[None, :], TAU_B[None, :]), 0.0)
+    r = pos - off
+    r = r + np.where(sheet2[:, None], (R2 @ (LAT.T @ s))[None, :], 0.0)   # sheet 2 cells moved by the shift
+    d = r - r @ Rinv.T
+    b = (np.linalg.solve(LAT.T, d.T).T + s) % 1.0
+    return b, (2 * grid[:, 3] + grid[:, 2]).astype(np.int32)
 
 
 def cell_angle_deg(i: int) -> float:
@@ -66,9 +109,10 @@ def sample_configs(n: int, rng, theta_range=(0.8, 3.0), strain_sd=0.003, snap=Tr
 
 
 class ExactLocalKPM:
-    """Exact local moments [n][4][M] of configurations on one device handle."""
+    """Exact local moments on one device handle: random samples (every
+    target orbital of a cell is a sample) and on-demand configurations."""
 
-    def __init__(self, dev: L.Device, M: int, chunk: int = 512, precision=L.PREC_FP64):
+    def __init__(self, dev: L.Device, M: int, chunk: int = 256, precision=L.PREC_FP64):
         self.dev, self.M, self.chunk, self.precision = dev, M, chunk, precision
         self._geoms = {}
 
@@ -77,15 +121,16 @@ class ExactLocalKPM:
         if g is None:
             g = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=self.M)))
             g._origin = origin_orbitals(g)
+            th = g.info.angle[1]
+            # sheet-2 origin cell moves with the shift: its stacking is (I + X) s
+            g._X = np.linalg.solve(LAT.T, (np.eye(2) - rot(-th)) @ rot(th) @ LAT.T)
             self._geoms[i] = g
         return g
 
-    def moments(self, cfgs) -> np.ndarray:
-        cfgs = np.atleast_2d(np.asarray(cfgs, dtype=np.float64))
-        n = cfgs.shape[0]
-        out = np.zeros((n, 4, self.M))
-        cells = np.array([nearest_cell(t) for t in cfgs[:, 5]])
-        order = np.argsort(cells, kind="stable")      # one geometry upload per cell
+    def _run(self, jobs, R):
+        """jobs: list of (cell i, shift, strain, targets[R]) -> moments [len(jobs)][R][M]."""
+        out = np.zeros((len(jobs), R, self.M))
+        order = sorted(range(len(jobs)), key=lambda q: jobs[q][0])   # one geometry upload per cell
         dev = self.dev
         dev.batch_clear()
         pending, targets = [], []
@@ -94,34 +139,71 @@ class ExactLocalKPM:
         def flush():
             if not pending:
                 return
-            mu = dev.batch_kpm(self.M, 4, targets=np.array(targets), precision=self.precision, graph=False)
+            mu = dev.batch_kpm(self.M, R, targets=np.array(targets), precision=self.precision, graph=False)
             out[np.array(pending)] = mu
             dev.batch_clear()
             pending.clear()
             targets.clear()
 
         for q in order:
-            i = int(cells[q])
+            i, sh, e, tg = jobs[q]
             g = self._geom(i)
             if i != cur:
                 dev.set_geometry(g)
                 dev.pairs_build()
                 cur = i
-            b1, b2, exx, eyy, exy, _ = cfgs[q]
             job = g.jobs()[0]
-            job.shift[1][0], job.shift[1][1], job.shift[1][2] = b1, b2, 0.0
+            job.shift[1][0], job.shift[1][1], job.shift[1][2] = float(sh[0]), float(sh[1]), 0.0
             job.uniform_strain = 1
-            job.u[0][0], job.u[0][1], job.u[1][0], job.u[1][1] = exx, exy, exy, eyy
+            job.u[0][0], job.u[0][1], job.u[1][0], job.u[1][1] = e[0], e[2], e[2], e[1]
             job.complex_matrix = 0
             job.energy_rescale, job.energy_shift = RESCALE, 0.0
             dev.assemble(job, L.PATTERN_PRUNED)
             dev.batch_append()
             pending.append(q)
-            targets.append(g._origin)
+            targets.append(np.asarray(tg, dtype=np.int32))
             if len(pending) >= self.chunk:
                 flush()
         flush()
         return out
+
+    def samples(self, n_jobs, rng, R=32, theta_range=(0.8, 3.0), strain_sd=0.003, log_theta=True):
+        """R random orbitals of each of n_jobs random cells (twist, shift of
+        sheet 2, strain).  Returns dict(cfg [n][6], block [n], moments [n][M],
+        cell, shift, target) with n = n_jobs R."""
+        cfg0 = sample_configs(n_jobs, rng, theta_range=theta_range, strain_sd=strain_sd, log_theta=log_theta)
+        jobs, cfg, block, cell, shift, target = [], [], [], [], [], []
+        for q in range(n_jobs):
+            i = nearest_cell(cfg0[q, 5])
+            g = self._geom(i)
+            tg = rng.choice(g.n, size=R, replace=False).astype(np.int32)
+            b, blk = local_shifts(g, cfg0[q, :2])
+            th = abs(math.degrees(g.info.angle[1]))
+            for t in tg:
+                cfg.append([b[t, 0], b[t, 1], cfg0[q, 2], cfg0[q, 3], cfg0[q, 4], th])
+                block.append(blk[t])
+                cell.append(i)
+                shift.append(cfg0[q, :2])
+                target.append(t)
+            jobs.append((i, cfg0[q, :2], cfg0[q, 2:5], tg))
+        mom = self._run(jobs, R).reshape(-1, self.M)
+        return {"cfg": np.array(cfg), "block": np.array(block, dtype=np.int32), "moments": mom,
+                "cell": np.array(cell), "shift": np.array(shift), "target": np.array(target)}
+
+    def moments_at(self, cfgs, blocks) -> np.ndarray:
+        """Exact moments [n][M] of configurations on the given blocks, at an
+        origin-cell orbital of the cell nearest each twist, sheet 2 shifted so
+        that the orbital's stacking is b."""
+        cfgs = np.atleast_2d(np.asarray(cfgs, dtype=np.float64))
+        blocks = np.asarray(blocks)
+        jobs = []
+        for c, k in zip(cfgs, blocks):
+            i = nearest_cell(c[5])
+            g = self._geom(i)
+            sh = c[:2] if k < 2 else np.linalg.solve(np.eye(2) + g._X, c[:2])
+            jobs.append((i, sh, c[2:5], g._origin))
+        mu = self._run(jobs, 4)
+        return mu[np.arange(len(cfgs)), blocks]
 
 
 def origin_orbitals(geom) -> np.ndarray:
@@ -164,7 +246,7 @@ def train_ensemble(x, t, members=4, hidden=256, layers=3, epochs=60, batch=256, 
     ts = ((t - mu) / sd).astype(np.float32)
     nets, losses = [], []
     for e in range(members):
-        net = L.Mlp(device, L.MLP_FEATURES, hidden, layers, t.shape[1], seed=seed * 1000 + e)
+        net = L.Mlp(device, x.shape[1], hidden, layers, t.shape[1], seed=seed * 1000 + e)
         losses.append(net.train(x, ts, epochs=epochs, batch=batch, lr=lr, seed=seed * 1000 + 100 + e))
         p = net.params()
         dims = net.dims
@@ -179,24 +261,20 @@ def train_ensemble(x, t, members=4, hidden=256, layers=3, epochs=60, batch=256, 
     return nets, losses
 
 
-def predict_with_fallback(nets, exact: ExactLocalKPM, cfgs, threshold=1e-3, blocks=None):
-    """Ensemble prediction [n][4][M] with exact local KPM where the spread
-    (RMS of the ensemble standard deviation over the M moments of a
-    sublattice) exceeds the threshold: over the sublattice blocks[i] of each
-    configuration when given, else over its largest of the four.  Returns
-    (moments, fallback mask, spread, report)."""
-    M = exact.M
+def predict_with_fallback(nets, exact: ExactLocalKPM, cfgs, blocks, threshold=1e-3):
+    """Ensemble prediction [n][M] of each site's block, with exact local KPM
+    where the ensemble spread exceeds the threshold.  Returns (moments,
+    fallback mask, spread, report)."""
     t0 = time.perf_counter()
-    x = L.mlp_features(cfgs)
-    mean, sp4 = L.mlp_ensemble(nets, x, blocks=4)
-    spread = sp4.max(axis=1) if blocks is None else sp4[np.arange(len(x)), blocks]
+    x = site_features(cfgs, blocks)
+    mean, spread = L.mlp_ensemble(nets, x)
     t_inf = time.perf_counter() - t0
-    mom = mean.reshape(-1, 4, M).astype(np.float64)
+    mom = mean.astype(np.float64)
     fb = spread > threshold
     report = {"sites": int(len(cfgs)), "fallback": int(fb.sum()), "inference_s": t_inf}
     if fb.any():
         t1 = time.perf_counter()
-        ex = exact.moments(cfgs[fb])
+        ex = exact.moments_at(cfgs[fb], np.asarray(blocks)[fb])
         report["fallback_s"] = time.perf_counter() - t1
         err = np.abs(mom[fb] - ex)
         report["mlp_max_abs_err_at_fallback"] = float(err.max())
@@ -206,22 +284,8 @@ def predict_with_fallback(nets, exact: ExactLocalKPM, cfgs, threshold=1e-3, bloc
 
 
 def flake_site_configs(geom, theta_deg: float):
-    """Configuration of every orbital of a twisted flake (sheet 1 at angle 0,
-    sheet 2 at theta): the local shift of sheet 2's lattice at the orbital's
-    unit cell's A position r, b = A^-1 (I - R(-theta)) r mod 1 in sheet 2's
-    lattice basis (the shift that the training cells apply at their origin),
-    no strain.  Returns (cfg [n][6], block [n] = 2 sheet + sublattice)."""
-    i2g, i2p = geom.arrays()
-    grid, pos = i2g.reshape(-1, 4), i2p.reshape(-1, 3)
-    th = math.radians(theta_deg)
-    Rm = np.array([[math.cos(-th), -math.sin(-th)], [math.sin(-th), math.cos(-th)]])
-    R2 = Rm.T
-    # the unit cell's A position: r - tau_l on sheet 1, r - R(theta) tau_l on sheet 2
-    tauB = np.array([0.5 * GR_A, GR_A / (2.0 * math.sqrt(3.0))])
-    off = np.where(grid[:, 2:3] == 1, np.where(grid[:, 3:4] == 1, (R2 @ tauB)[None, :], tauB[None, :]), 0.0)
-    r = pos[:, :2] - off
-    d = r - r @ Rm.T
-    b = np.linalg.solve(LAT.T, d.T).T % 1.0
-    cfg = np.column_stack([b, np.zeros((len(r), 3)), np.full(len(r), theta_deg)])
-    block = 2 * grid[:, 3] + grid[:, 2]
-    return cfg, block.astype(np.int32)
+    """Configuration and block of every orbital of a twisted flake (sheet 1 at
+    angle 0, sheet 2 at theta): the stacking at its unit cell, no strain."""
+    b, block = local_shifts(geom)
+    cfg = np.column_stack([b, np.zeros((len(b), 3)), np.full(len(b), theta_deg)])
+    return cfg, block
