@@ -29,7 +29,11 @@
  *                    tests/tblg_test{1,2}.test.
  *   random phases  : no reference counterpart (new input form); its integer
  *                    hash is pinned against an independent Python restatement.
- *   Gaussian trace vectors (std::default_random_engine): parity unpinned.
+ *   Gaussian trace vectors: drawn by oracle.py's restatement of libstdc++'s
+ *                    minstd_rand0 + normal_distribution, pinned bit for bit
+ *                    against tests/golden/trace_gaussians.txt (written by a
+ *                    libstdc++ program built here) and the standard's
+ *                    10000th minstd_rand0 value 1043618065.
  */
 #ifndef LEO2D_ORACLE_H
 #define LEO2D_ORACLE_H

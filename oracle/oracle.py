@@ -810,6 +810,68 @@ def matrix_dat_text(rp, ci, val) -> str:
     return "".join(out)
 
 
+# --------------------------------------------------------------------------
+# KPM trace sampling: std::default_random_engine + std::normal_distribution as
+# locality.cpp:2275-2276, 2301-2303 use them (libstdc++ 11, the image's
+# library: bits/random.h, bits/random.tcc)
+
+
+class MinstdRand0:
+    """minstd_rand0 (= libstdc++'s default_random_engine): x <- 16807 x mod
+    (2^31 - 1), default seed 1; returns the new state."""
+
+    def __init__(self, seed: int = 1):
+        self.x = seed % 2147483647 or 1
+
+    def __call__(self) -> int:
+        self.x = (16807 * self.x) % 2147483647
+        return self.x
+
+
+def generate_canonical(g) -> float:
+    """std::generate_canonical<double, 53>: with range r = 2^31 - 2 it takes
+    m = 2 draws (log2 r truncates to 30), sum = (g1 - 1) + (g2 - 1) r, / r^2."""
+    r = 2147483646.0
+    s, t = 0.0, 1.0
+    for _ in range(2):
+        s += float(g() - 1) * t
+        t *= r
+    v = s / t
+    return v if v < 1.0 else math.nextafter(1.0, 0.0)
+
+
+class NormalDistribution:
+    """std::normal_distribution<double>(0, 1), Marsaglia polar method: each
+    accepted pair (x, y) returns y * mult and keeps x * mult for the next call."""
+
+    def __init__(self):
+        self.saved = None
+
+    def __call__(self, g) -> float:
+        if self.saved is not None:
+            v, self.saved = self.saved, None
+            return v
+        while True:
+            x = 2.0 * generate_canonical(g) - 1.0
+            y = 2.0 * generate_canonical(g) - 1.0
+            r2 = x * x + y * y
+            if not (r2 > 1.0 or r2 == 0.0):
+                break
+        mult = math.sqrt(-2 * math.log(r2) / r2)
+        self.saved = x * mult
+        return y * mult * 1.0 + 0.0
+
+
+def trace_samples(opts: dict, n: int) -> int:
+    """num_trace_samps (locality.cpp:2257-2270): KPM_TRACE_SAMPS, or for -1
+    (int) 1e6 / n clamped to [1, 100]."""
+    s = opts["kpm_trace_samps"]
+    if s == -1:
+        s = int(1000000 / float(n))
+        s = min(max(s, 1), 100)
+    return s
+
+
 def run_deck(text: str, pruned: int = 0):
     """Whole DoS path for a deck on the CPU: returns (deck, geometry, jobs, results)."""
     deck = parse_deck(text)
@@ -819,9 +881,19 @@ def run_deck(text: str, pruned: int = 0):
     jobs = build_jobs(deck, targets)
     o = deck.opts
     results = []
+    trace = o["kpm_trace"] == 1
+    if trace:
+        # one generator for all jobs, vectors drawn in job order (locality.cpp:2275-2310)
+        gen, dist = MinstdRand0(), NormalDistribution()
+        S = trace_samples(o, geom.n)
     for job in jobs:
         rp, ci, v = geom.assemble(make_job(deck, job, pruned))
-        mu = kpm(rp, ci, v, o["poly_order"], targets=job["target_list"])
-        rho = dos(mu, o["energy_rescale"], o["energy_shift"]) if o["dos_transform"] == 1 else mu
+        if trace:
+            vecs = np.array([[dist(gen) for _ in range(geom.n)] for _ in range(S)])
+            mu = kpm(rp, ci, v, o["poly_order"], vecs=vecs)
+            rho = dos_trace(mu, o["energy_rescale"], o["energy_shift"], 1.0 / geom.n)
+        else:
+            mu = kpm(rp, ci, v, o["poly_order"], targets=job["target_list"])
+            rho = dos(mu, o["energy_rescale"], o["energy_shift"]) if o["dos_transform"] == 1 else mu
         results.append((job, rho))
     return deck, geom, jobs, results

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 DECKS = os.path.join(ROOT, "tests", "decks")
 PARITY_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_gshift_strain.in",
-                "tblg_simple_flake.in", "tblg_c0_twin.in", "tblg_strain_flake.in"]
+                "tblg_simple_flake.in", "tblg_c0_twin.in", "tblg_strain_flake.in", "tblg_trace.in"]
 
 
 def deck_text(name):

@@ -479,3 +479,42 @@ def test_strained_hopping_closed_form(oracle_mod):
     assert n_bottom > 100 and n_top > 100
     # the three strained bond families are distinct and bracket a_cc
     assert top_lengths[0] < acc < top_lengths[-1]
+
+
+def test_trace_generator_matches_libstdcxx(oracle_mod):
+    """The oracle's restatement of std::default_random_engine +
+    std::normal_distribution reproduces, bit for bit, the values a program built
+    with this image's libstdc++ draws (tests/golden/trace_gaussians.txt, made by
+    tests/golden/make_trace_gaussians.sh); minstd_rand0's 10000th output is the
+    value the C++ standard requires, 1043618065."""
+    orc = oracle_mod
+    want = [float(line) for line in open(os.path.join(os.path.dirname(__file__), "golden", "trace_gaussians.txt"))]
+    g, d = orc.MinstdRand0(), orc.NormalDistribution()
+    got = [d(g) for _ in range(len(want))]
+    assert got == want
+    g = orc.MinstdRand0()
+    for _ in range(9999):
+        g()
+    assert g() == 1043618065
+    # generate_canonical stays in [0, 1)
+    g = orc.MinstdRand0()
+    u = [orc.generate_canonical(g) for _ in range(2000)]
+    assert min(u) >= 0.0 and max(u) < 1.0
+
+
+def test_trace_deck_normalisation(oracle_mod):
+    """Trace DoS of the oracle: the Chebyshev-Gauss quadrature of rho recovers
+    the sample-averaged mu_0 / N = mean |v|^2 / N of the job's Gaussian vectors."""
+    orc = oracle_mod
+    text = open(os.path.join(os.path.dirname(__file__), "decks", "tblg_trace.in")).read()
+    deck, geom, jobs, results = orc.run_deck(text)
+    o = deck.opts
+    M, a = o["poly_order"], o["energy_rescale"]
+    g, d = orc.MinstdRand0(), orc.NormalDistribution()
+    S = orc.trace_samples(o, geom.n)
+    assert S == 6 and len(results) == 4          # NSHIFTS = 2: a 2 x 2 shift grid
+    for job, rho in results:
+        vecs = np.array([[d(g) for _ in range(geom.n)] for _ in range(S)])
+        x = np.cos((np.arange(M) + 0.5) * np.pi / M)
+        quad = np.sum(rho * np.pi * a * np.sqrt(1 - x * x)) / M
+        assert abs(quad - np.mean(np.sum(vecs * vecs, axis=1)) / geom.n) < 1e-10
