@@ -4,6 +4,7 @@
 // oracle's restatement against the reference's code on this host.  No
 // reference source is copied here; this file only calls the reference's
 // public functions.
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -55,6 +56,7 @@ long ref_sheet_intra_pairs(int min0, int min1, int max0, int max1, double max_R,
         sd.supercell_stride = {{stride[0], stride[1]}, {stride[2], stride[3]}};
     }
     Sheet sheet(sd);
+    std::fflush(stdout);  // Sheet's constructor prints; keep it inside the caller's capture
     if (n_orbitals_out) *n_orbitals_out = sheet.getMaxIndex();
     std::vector<int> ai, aj;
     std::vector<double> at;
@@ -88,6 +90,7 @@ int ref_sheet_positions(int min0, int min1, int max0, int max1, double max_R, in
         sd.supercell_stride = {{stride[0], stride[1]}, {stride[2], stride[3]}};
     }
     Sheet sheet(sd);
+    std::fflush(stdout);  // Sheet's constructor prints; keep it inside the caller's capture
     int n = sheet.getMaxIndex();
     if (pos && n <= cap)
         for (int k = 0; k < n; ++k)
