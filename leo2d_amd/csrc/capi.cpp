@@ -47,13 +47,14 @@ struct NanError : std::runtime_error {
 };
 
 // step-kernel variant for R >= 64 (LEO2D_KPM_VARIANT overrides the default):
-// the wavefront-per-row kernel with readlane broadcasts (1) for FP64, the same
-// with the matrix on the scalar path (8) for the FP32 recursion, whose 8-byte
-// complex values leave the vector-memory path to the gathers
+// the wavefront-per-row kernel with readlane broadcasts (1) for FP64; for the
+// FP32 recursion, two rows per wavefront with 16-byte gathers and the matrix
+// on the scalar path (15; it falls back to the one-row scalar-path kernel 8
+// where it does not apply)
 int kpm_variant(bool mixed, bool spatial)
 {
     if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) return std::atoi(e);
-    return mixed && spatial ? 8 : 1;
+    return mixed && spatial ? 15 : 1;
 }
 
 // row schedule of the wavefront-per-row kernels: 1 = one contiguous slice of
@@ -115,6 +116,12 @@ struct leo2d_handle {
         DevBuf rp, col, val, val32;
         bool val32_valid = false;
     } groups;
+    // row tiles of the vector-chunked step (variant 13) and each workgroup's run of them
+    struct Chunks {
+        bool valid = false, spatial = false;
+        int ecap = 0, tmax = 0, grid = 0, ntiles = 0;
+        DevBuf row0, wg;
+    } chunks;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
     // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
@@ -275,6 +282,54 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->tiles.valid = false;
     h->groups.valid = false;
     h->tiles.failed = false;
+    h->chunks.valid = false;
+}
+
+// Row tiles for the vector-chunked step: consecutive rows, at most `tmax` rows
+// and `ecap` entries per tile; then `grid` contiguous runs of tiles with about
+// equal entry counts, one per workgroup.  Returns false when one row alone has
+// more than `ecap` entries.
+bool build_chunks(leo2d_handle* h, const int64_t* drp, int ecap, int tmax, int grid, bool spatial)
+{
+    auto& T = h->chunks;
+    if (T.valid && T.ecap == ecap && T.tmax == tmax && T.grid == grid && T.spatial == spatial) return true;
+    const int n = h->n;
+    hipStream_t s = h->stream;
+    std::vector<int64_t> rp(n + 1);
+    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "chunk tiles");
+    check(hipStreamSynchronize(s), "chunk tiles");
+    std::vector<int32_t> row0;
+    std::vector<int64_t> e0;
+    int r = 0;
+    while (r < n) {
+        const int start = r;
+        while (r < n && r - start < tmax && rp[r + 1] - rp[start] <= ecap) ++r;
+        if (r == start) {
+            T.valid = false;
+            return false;
+        }
+        row0.push_back(start);
+        e0.push_back(rp[start]);
+    }
+    const int ntiles = (int)row0.size();
+    row0.push_back(n);
+    const int64_t total = rp[n];
+    std::vector<int32_t> wg(grid + 1);
+    for (int g = 0; g < grid; ++g) {
+        const int64_t want = total * g / grid;
+        wg[g] = (int32_t)(std::lower_bound(e0.begin(), e0.end(), want) - e0.begin());
+    }
+    wg[grid] = ntiles;
+    to_device(T.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "chunk tiles");
+    to_device(T.wg, wg.data(), sizeof(int32_t) * wg.size(), s, "chunk tiles");
+    check(hipStreamSynchronize(s), "chunk tiles");
+    T.ecap = ecap;
+    T.tmax = tmax;
+    T.grid = grid;
+    T.spatial = spatial;
+    T.ntiles = ntiles;
+    T.valid = true;
+    return true;
 }
 
 // Greedy row tiles for the LDS-tiled step: consecutive rows (at most
@@ -1198,7 +1253,45 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     }
                 }
             }
-            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11) sa.variant = 0;
+            if (sa.variant == 13) {
+                // workgroup size, LDS budget per workgroup and rows per tile
+                // (LEO2D_CHUNK_THREADS / _LDS_KB / _TMAX override the defaults)
+                auto env_int = [](const char* k, int d) {
+                    const char* e = std::getenv(k);
+                    return e ? std::atoi(e) : d;
+                };
+                const int nt = env_int("LEO2D_CHUNK_THREADS", 512);
+                const int kb = std::max(16, std::min(160, env_int("LEO2D_CHUNK_LDS_KB", 78)));
+                const int tmax = std::max(8, std::min(4096, env_int("LEO2D_CHUNK_TMAX", 256)));
+                const size_t hb = mat_type == leo2d::MAT_CPX_F64 ? 16 : (mat_type == leo2d::MAT_REAL_F32 ? 4 : 8);
+                const size_t fixed = sizeof(double) * (nt / 64) * B * 3 + sizeof(int32_t) * (tmax + 1);
+                const long ecap = ((long)kb * 1024 - (long)fixed) / (long)(hb + 4);
+                const bool type_ok = vec_type == leo2d::VEC_CPX_F64 || vec_type == leo2d::VEC_CPX_F32 ||
+                                     vec_type == leo2d::VEC_REAL_F64;
+                int grid13 = G;
+                if (p->grid <= 0) {
+                    int dev_id = 0, cus = 256;
+                    if (hipGetDevice(&dev_id) == hipSuccess)
+                        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+                    const int per_cu = std::max(1, std::min((160 * 1024) / (kb * 1024), 2048 / std::max(64, nt)));
+                    grid13 = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));   // part holds G slots
+                }
+                if (!spatial || (B != 64 && B != 128) || !type_ok || (nt != 256 && nt != 512 && nt != 1024) ||
+                    ecap < 64 || !build_chunks(h, drp, (int)ecap, tmax, grid13, spatial)) {
+                    sa.variant = 1;
+                } else {
+                    sa.grid = grid13;
+                    sa.chunk_row0 = (const int32_t*)h->chunks.row0.p;
+                    sa.chunk_wg = (const int32_t*)h->chunks.wg.p;
+                    sa.chunk_ecap = (int)ecap;
+                    sa.chunk_tmax = tmax;
+                    sa.chunk_threads = nt;
+                }
+            }
+            if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
+            if ((sa.variant == 15 || sa.variant == 16) && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32))
+                sa.variant = mixed && spatial && B % 64 == 0 ? 8 : 1;
+            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14) sa.variant = 0;
             kernel_used = sa.variant;
             if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
                 // larger workgroups: keep the total number of resident wavefronts

@@ -50,6 +50,13 @@ struct StepArgs {
     const int64_t* grp_rp;      // [ngroups + 1]
     const int32_t* grp_col;     // [entries + kCsrPad]
     const void* grp_val;        // [entries + kCsrPad][kGroupRows]
+    // variant 13: row tiles of at most chunk_tmax rows and chunk_ecap entries;
+    // workgroup g (logical, XCD-aware) walks tiles [chunk_wg[g], chunk_wg[g+1])
+    const int32_t* chunk_row0;  // [tiles + 1]
+    const int32_t* chunk_wg;    // [grid + 1]
+    int chunk_ecap;
+    int chunk_tmax;
+    int chunk_threads;          // 256, 512 or 1024
 };
 
 // trailing (column 0, value 0) entries after nnz in the spatially ordered CSR

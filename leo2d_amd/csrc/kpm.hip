@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 #include "kpm_common.h"
@@ -296,6 +297,16 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
             nq0 = rp[next];
             nq1 = rp[next + 1];
         }
+        // the row's own T_j and T_{j-1} elements, issued ahead of the gathers
+        // so that their latency overlaps the row's accumulation
+        XT own_c[VPL], own_p[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const size_t idx = (size_t)row * R + lane + 64 * v;
+            own_c[v] = cur[idx];
+            if (beta) own_p[v] = ld_s<STREAM>(prev + idx);
+            else zero(own_p[v]);
+        }
         AT acc[VPL];
 #pragma unroll
         for (int v = 0; v < VPL; ++v) zero(acc[v]);
@@ -337,11 +348,8 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             const size_t idx = (size_t)row * R + lane + 64 * v;
-            const XT c = cur[idx];
-            XT p;
-            if (beta) p = ld_s<STREAM>(prev + idx);
-            else zero(p);
-            const XT o = comb(alpha, acc[v], p);
+            const XT c = own_c[v];
+            const XT o = comb(alpha, acc[v], own_p[v]);
             st_s<STREAM>(out + idx, o);
             d1[v] += nrm2(c);
             d2[v] += redot(o, c);
@@ -894,45 +902,68 @@ __global__ __launch_bounds__(kThreads) void kpm_step_smem_kernel(const int64_t* 
 #pragma unroll
     for (int v = 0; v < VPL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
 
-    for (int row = __builtin_amdgcn_readfirstlane(first + wave); row < n; row += stride) {
-        const int64_t q0 = rp[row], q1 = rp[row + 1];
+    // whole-chunk loads: the arrays carry BATCH padding entries (column 0,
+    // value 0) past nnz and no chunk starts beyond nnz.  The column chunk that
+    // the next gathers need (the row's next chunk, or the next row's first) is
+    // loaded one chunk ahead, while the current chunk's gathers are in flight.
+    const int64_t nnz = rp[nrows];
+    int row = __builtin_amdgcn_readfirstlane(first + wave);
+    int64_t q0 = 0, q1 = 0;
+    if (row < n) {
+        q0 = rp[row];
+        q1 = rp[row + 1];
+    }
+    ColChunk<BATCH> cc = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(q0, nnz));
+    for (; row < n; row += stride) {
+        const int next = row + stride;
+        int64_t nq0 = 0, nq1 = 0;
+        if (next < n) {
+            nq0 = rp[next];
+            nq1 = rp[next + 1];
+        }
+        // the row's own T_j and T_{j-1} elements, issued ahead of the gathers
+        XT own_c[VPL], own_p[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            const size_t idx = (size_t)row * R + lane + 64 * v;
+            own_c[v] = cur[idx];
+            if (beta) own_p[v] = po[idx];
+            else zero(own_p[v]);
+        }
         AT acc[VPL];
 #pragma unroll
         for (int v = 0; v < VPL; ++v) zero(acc[v]);
         for (int64_t q = q0; q < q1; q += BATCH) {
-            // whole-chunk loads: the arrays carry BATCH padding entries
-            // (column 0, value 0) past nnz, so q + BATCH never leaves them
-            const ColChunk<BATCH> cc = *reinterpret_cast<const ColChunk<BATCH>*>(col + q);
-            const ValChunk<HT, BATCH> hc = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + q);
-            const int* c = cc.c;
-            const HT* h = hc.h;
             XT x[BATCH][VPL];
 #pragma unroll
             for (int u = 0; u < BATCH; ++u) {
-                const XT* xr = cur + (size_t)c[u] * R;
+                const XT* xr = cur + (size_t)cc.c[u] * R;
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) x[u][v] = xr[lane + 64 * v];
             }
+            const ValChunk<HT, BATCH> hc = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + q);
+            const int64_t nxt = q + BATCH < q1 ? q + BATCH : nq0;
+            cc = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(nxt, nnz));
 #pragma unroll
             for (int u = 0; u < BATCH; ++u)
                 if (q + u < q1) {
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v) mac(acc[v], h[u], x[u][v]);
+                    for (int v = 0; v < VPL; ++v) mac(acc[v], hc.h[u], x[u][v]);
                 }
         }
+        if (q0 == q1) cc = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(nq0, nnz));
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             const size_t idx = (size_t)row * R + lane + 64 * v;
-            const XT cv = cur[idx];
-            XT p;
-            if (beta) p = po[idx];
-            else zero(p);
-            const XT o = comb(alpha, acc[v], p);
+            const XT cv = own_c[v];
+            const XT o = comb(alpha, acc[v], own_p[v]);
             po[idx] = o;
             d1[v] += nrm2(cv);
             d2[v] += redot(o, cv);
             if (direct) d3[v] += redot(vin[idx], cv);
         }
+        q0 = nq0;
+        q1 = nq1;
     }
 
     __shared__ double red[kWaves][R][3];
@@ -1154,6 +1185,497 @@ hipError_t launch_tiled_dpp(const StepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---- variant 13: vector chunks over LDS-staged row tiles ----------------------
+// The step's bound is the gather stream: every T_j row is fetched once per
+// nonzero that points at it (about 32 times per step on bilayer graphene).
+// This kernel arranges those fetches so that most of them hit in the CU's L1:
+//  * a workgroup owns a contiguous run of row tiles (in the spatial order a
+//    compact patch of the lattice) and walks it tile by tile;
+//  * a tile's CSR entries (and tile-relative row offsets) are staged in LDS once;
+//  * the tile is then swept once per chunk of C vectors, C chosen so that one
+//    row's chunk is a single 128-byte line (C = 8 for complex FP64, 16 for
+//    complex FP32).  A lane owns one (row, vector) pair and a wavefront covers
+//    64 / C rows, so a gather instruction fetches 64 / C whole lines.
+// Within one sweep the tile's distinct neighbour lines (about 3 per row) stay
+// in L1, so the L2 request stream falls from ~32 lines per row and chunk to ~3.
+// Partials: each sweep folds its per-lane sums over the wavefront's row slots
+// (fixed xor tree) into this wavefront's LDS slot; the end folds the
+// wavefronts in order.  The tile order, sweep order and trees are fixed, so the
+// moments are deterministic.
+template <typename HT, typename XT, int C, int NT, int NCH>
+__global__ __launch_bounds__(NT) void kpm_step_chunk_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int RPW = 64 / C;          // rows per wavefront
+    constexpr int NW = NT / 64;
+    constexpr int R = C * NCH;
+    constexpr int U = 8;                 // gathers in flight per lane
+    extern __shared__ __align__(16) unsigned char lds[];
+    double* red = reinterpret_cast<double*>(lds);                                   // [NW][R][3]
+    HT* s_val = reinterpret_cast<HT*>(lds + sizeof(double) * NW * R * 3);           // [ecap]
+    int32_t* s_col = reinterpret_cast<int32_t*>(s_val + a.chunk_ecap);              // [ecap]
+    int32_t* s_rp = s_col + a.chunk_ecap;                                           // [tmax + 1]
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int rsub = lane / C;
+    const int vi = lane % C;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    const int t_begin = a.chunk_wg[lb];
+    const int t_end = a.chunk_wg[lb + 1];
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const int32_t* __restrict__ col = a.col;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    for (int t = lane; t < R * 3; t += 64) red[wave * R * 3 + t] = 0.0;
+
+    for (int t = t_begin; t < t_end; ++t) {
+        const int r0 = a.chunk_row0[t];
+        const int nr = a.chunk_row0[t + 1] - r0;
+        const int64_t e0 = rp[r0];
+        const int ne = (int)(rp[r0 + nr] - e0);
+        __syncthreads();   // the previous tile's entries are no longer read
+        for (int q = threadIdx.x; q < ne; q += NT) {
+            s_col[q] = col[e0 + q];
+            s_val[q] = val[e0 + q];
+        }
+        for (int q = threadIdx.x; q <= nr; q += NT) s_rp[q] = (int)(rp[r0 + q] - e0);
+        __syncthreads();
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int voff = ch * C + vi;
+            double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+            for (int rb = 0; rb < nr; rb += NW * RPW) {
+                const int lr = rb + wave * RPW + rsub;
+                if (lr < nr) {
+                    const int kb = s_rp[lr], ke = s_rp[lr + 1];
+                    AT acc;
+                    zero(acc);
+                    for (int k = kb; k < ke; k += U) {
+                        XT x[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int kk = min(k + u, ke - 1);
+                            x[u] = cur[(size_t)s_col[kk] * R + voff];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (k + u < ke) mac(acc, s_val[k + u], x[u]);
+                    }
+                    const size_t idx = (size_t)(r0 + lr) * R + voff;
+                    const XT c = cur[idx];
+                    XT p;
+                    if (beta) p = prev[idx];
+                    else zero(p);
+                    const XT o = comb(alpha, acc, p);
+                    out[idx] = o;
+                    d1 += nrm2(c);
+                    d2 += redot(o, c);
+                    if (direct) d3 += redot(vin[idx], c);
+                }
+            }
+#pragma unroll
+            for (int m = C; m < 64; m <<= 1) {
+                d1 += __shfl_xor(d1, m);
+                d2 += __shfl_xor(d2, m);
+                d3 += __shfl_xor(d3, m);
+            }
+            if (rsub == 0) {
+                double* slot = red + ((size_t)wave * R + ch * C + vi) * 3;
+                slot[0] += d1;
+                slot[1] += d2;
+                slot[2] += d3;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += NT) {
+        double sum = red[t];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) sum += red[w * R * 3 + t];
+        a.part[(size_t)lb * R * 3 + t] = sum;
+    }
+}
+
+template <typename HT, typename XT, int C, int NT, int NCH>
+hipError_t launch_chunk_nt(const StepArgs& a, hipStream_t s)
+{
+    const size_t shmem = sizeof(double) * (NT / 64) * C * NCH * 3 +
+                         (sizeof(HT) + sizeof(int32_t)) * (size_t)a.chunk_ecap + sizeof(int32_t) * (a.chunk_tmax + 1);
+    hipError_t e = hipFuncSetAttribute((const void*)kpm_step_chunk_kernel<HT, XT, C, NT, NCH>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kpm_step_chunk_kernel<HT, XT, C, NT, NCH>), dim3(a.grid), dim3(NT), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <typename HT, typename XT, int C, int NCH>
+hipError_t launch_chunk_r(const StepArgs& a, hipStream_t s)
+{
+    switch (a.chunk_threads) {
+    case 256: return launch_chunk_nt<HT, XT, C, 256, NCH>(a, s);
+    case 512: return launch_chunk_nt<HT, XT, C, 512, NCH>(a, s);
+    case 1024: return launch_chunk_nt<HT, XT, C, 1024, NCH>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename HT, typename XT>
+hipError_t launch_chunk(const StepArgs& a, hipStream_t s)
+{
+    constexpr int C = 128 / (int)sizeof(XT);   // one 128-byte line per row and chunk
+    if (a.R == 64) return launch_chunk_r<HT, XT, C, 64 / C>(a, s);
+    if (a.R == 128) return launch_chunk_r<HT, XT, C, 128 / C>(a, s);
+    return hipErrorInvalidValue;
+}
+
+// ---- variant 14: vector chunks split over the XCDs -----------------------------
+// The gathers of the wavefront-per-row kernels are served by L2 about 90 % of
+// the time; the rest (neighbours far apart in the row order, such as across
+// the periodic boundary) come from the memory-side cache or HBM and dominate
+// the step.  Each XCD's L2 holds rows of T_j as 1 KiB (R = 64, complex FP64)
+// units.  This kernel gives each XCD a chunk of 16 vectors only: XCD x works on
+// chunk x mod (R/16) and, when R/16 < 8, on part x / (R/16) of the rows.  An
+// XCD's L2 then holds 256-byte row chunks, so it keeps R/16 times more
+// neighbour rows resident; every XCD reads the matrix entries of its rows
+// (R/16 = 4 parts' worth of H overall at R = 64).
+// Lane = (row slot 0..3) x (vector 0..15): the 16 lanes of a slot load 16 of
+// their row's (column, value) entries in one access and pass them to each
+// other with DPP row broadcasts; gathers for 16 entries are in flight at once
+// (lanes past their row's end are masked off), and the next chunk of entries
+// is loaded before the current one is consumed.  A workgroup's 4 wavefronts
+// own 16 consecutive rows; the XCD's workgroups sweep the XCD's rows in that
+// interleave, so the XCD works on one compact window of rows at a time.
+template <typename XT, int R, int K>
+__device__ inline void xchunk_gather(XT* xs, int m, int mc, const XT* __restrict__ cur, int voff)
+{
+    if constexpr (K < 16) {
+        const int c = bc16<K>(mc);
+        if (K < m) xs[K] = cur[(size_t)c * R + voff];
+        xchunk_gather<XT, R, K + 1>(xs, m, mc, cur, voff);
+    }
+}
+template <typename HT, typename XT, typename AT, int K>
+__device__ inline void xchunk_mac(AT& acc, int m, HT mh, const XT* xs)
+{
+    if constexpr (K < 16) {
+        const HT h = bc16<K>(mh);
+        if (K < m) mac(acc, h, xs[K]);
+        xchunk_mac<HT, XT, AT, K + 1>(acc, m, mh, xs);
+    }
+}
+
+template <typename HT, typename XT, int NCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_xchunk_kernel(StepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int C = 16;                 // vectors per chunk
+    constexpr int R = C * NCH;
+    constexpr int P = 8 / NCH;            // row parts
+    constexpr int RB = kWaves * 4;        // rows per workgroup block
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane >> 4;
+    const int vi = lane & 15;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int xcd = b & 7;
+    const int GW = G >> 3;
+    const int w = b >> 3;
+    const int lb = xcd * GW + w;
+    const int chunk = xcd % NCH;
+    const int part = xcd / NCH;
+    const int lo = (int)((int64_t)a.n * part / P);
+    const int hi = (int)((int64_t)a.n * (part + 1) / P);
+    const int voff = chunk * C + vi;
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const int32_t* __restrict__ col = a.col;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+    const bool direct = vin != nullptr;
+
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    const int stride = GW * RB;
+    int r0 = lo + w * RB + wave * 4;      // first row of this wavefront's quad
+    int row = r0 + slot;
+    int64_t kb = 0;
+    int cnt = 0;
+    if (row < hi) {
+        kb = rp[row];
+        cnt = (int)(rp[row + 1] - kb);
+    }
+    int mc = 0;
+    HT mh = zero_h<HT>();
+    if (vi < cnt) {
+        mc = col[kb + vi];
+        mh = val[kb + vi];
+    }
+    for (; r0 < hi; r0 += stride) {
+        const int nrow = r0 + stride + slot;
+        int64_t nkb = 0;
+        int ncnt = 0;
+        if (nrow < hi) {
+            nkb = rp[nrow];
+            ncnt = (int)(rp[nrow + 1] - nkb);
+        }
+        XT own_c, own_p;
+        zero(own_c);
+        zero(own_p);
+        if (row < hi) {
+            const size_t idx = (size_t)row * R + voff;
+            own_c = cur[idx];
+            if (beta) own_p = prev[idx];
+        }
+        int mx = max(cnt, __shfl_xor(cnt, 16));
+        mx = max(mx, __shfl_xor(mx, 32));
+        mx = __builtin_amdgcn_readfirstlane(mx);
+        AT acc;
+        zero(acc);
+        for (int k0 = 0; k0 < mx; k0 += 16) {
+            // entries consumed after this chunk: the row's next 16, or the next quad's first 16
+            int pc = 0;
+            HT ph = zero_h<HT>();
+            if (k0 + 16 < mx) {
+                const int q = k0 + 16 + vi;
+                if (q < cnt) {
+                    pc = col[kb + q];
+                    ph = val[kb + q];
+                }
+            } else if (vi < ncnt) {
+                pc = col[nkb + vi];
+                ph = val[nkb + vi];
+            }
+            const int m = cnt - k0;
+            XT xs[16];
+            xchunk_gather<XT, R, 0>(xs, m, mc, cur, voff);
+            xchunk_mac<HT, XT, AT, 0>(acc, m, mh, xs);
+            mc = pc;
+            mh = ph;
+        }
+        if (mx == 0) {
+            mc = 0;
+            mh = zero_h<HT>();
+            if (vi < ncnt) {
+                mc = col[nkb + vi];
+                mh = val[nkb + vi];
+            }
+        }
+        if (row < hi) {
+            const size_t idx = (size_t)row * R + voff;
+            const XT c = own_c;
+            const XT o = comb(alpha, acc, own_p);
+            out[idx] = o;
+            d1 += nrm2(c);
+            d2 += redot(o, c);
+            if (direct) d3 += redot(vin[idx], c);
+        }
+        row = nrow;
+        kb = nkb;
+        cnt = ncnt;
+    }
+
+    d1 += __shfl_xor(d1, 16);
+    d2 += __shfl_xor(d2, 16);
+    d3 += __shfl_xor(d3, 16);
+    d1 += __shfl_xor(d1, 32);
+    d2 += __shfl_xor(d2, 32);
+    d3 += __shfl_xor(d3, 32);
+    __shared__ double red[kWaves][C][3];
+    if (slot == 0) {
+        red[wave][vi][0] = d1;
+        red[wave][vi][1] = d2;
+        red[wave][vi][2] = d3;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double sum = 0.0;
+        if (r / C == chunk) {
+            const int v = r % C;
+            sum = red[0][v][k];
+#pragma unroll
+            for (int q = 1; q < kWaves; ++q) sum += red[q][v][k];
+        }
+        a.part[(size_t)lb * R * 3 + t] = sum;
+    }
+}
+
+template <typename HT, typename XT>
+hipError_t launch_xchunk(const StepArgs& a, hipStream_t s)
+{
+    switch (a.R) {
+    case 16: hipLaunchKernelGGL((kpm_step_xchunk_kernel<HT, XT, 1>), dim3(a.grid), dim3(kThreads), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((kpm_step_xchunk_kernel<HT, XT, 2>), dim3(a.grid), dim3(kThreads), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((kpm_step_xchunk_kernel<HT, XT, 4>), dim3(a.grid), dim3(kThreads), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((kpm_step_xchunk_kernel<HT, XT, 8>), dim3(a.grid), dim3(kThreads), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---- variant 15: FP32 recursion with 16-byte gathers -----------------------------
+// A complex FP32 row of T_j (R = 64) is 512 bytes.  The wavefront-per-row
+// kernels gather it with 8-byte lanes, so a gather instruction moves half the
+// bytes of the FP64 case for the same address work.  Here each lane carries a
+// pair of vectors (16 bytes), 32 lanes cover a row and a wavefront owns two
+// consecutive rows, one per half: one instruction gathers a neighbour row of
+// each (1 KiB).  Both rows' entries come through the scalar path in
+// BATCH-entry chunks (as in variant 8) and each half selects its own row's
+// entry; every row is still summed entry by entry in CSR order.
+template <typename HT, int BATCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_pair_kernel(const int64_t* __restrict__ rp,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const HT* __restrict__ val,
+                                                                 const float4* __restrict__ cur,
+                                                                 float4* __restrict__ po,
+                                                                 const float4* __restrict__ vin, int nrows,
+                                                                 double alpha, int beta, double* __restrict__ part)
+{
+    constexpr int R = 64;
+    constexpr int RQ = R / 2;             // float4 per row
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int half = lane >> 5;
+    const int hl = lane & 31;             // vectors 2 hl and 2 hl + 1
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    const int stride = G * kWaves;
+    const int npairs = (nrows + 1) / 2;
+    const int64_t nnz = rp[nrows];        // chunk loads start at or before nnz (BATCH padding entries follow)
+    const bool direct = vin != nullptr;
+    const float al = (float)alpha;
+
+    double d1a = 0.0, d1b = 0.0, d2a = 0.0, d2b = 0.0, d3a = 0.0, d3b = 0.0;
+    // extents of pair pr: rows 2 pr and 2 pr + 1 (the second may not exist)
+    auto extents = [&](int p, int64_t& qa, int64_t& ea, int64_t& qb, int64_t& eb) {
+        qa = ea = qb = eb = 0;
+        if (p < npairs) {
+            const int ra = 2 * p;
+            qa = rp[ra];
+            ea = rp[ra + 1];
+            qb = ea;
+            eb = ra + 1 < nrows ? rp[ra + 2] : ea;
+        }
+    };
+    int pr = __builtin_amdgcn_readfirstlane(lb * kWaves + wave);
+    int64_t qA, eA, qB, eB;
+    extents(pr, qA, eA, qB, eB);
+    // whole-chunk loads: the arrays carry BATCH padding entries past nnz.  The
+    // shorter row's chunk may run past its end (masked below); clamping keeps
+    // every chunk start at or before nnz.  The column chunks that the next
+    // gathers need (this pair's next chunk, or the next pair's first) are
+    // loaded one chunk ahead, while the current chunk's gathers are in flight.
+    ColChunk<BATCH> ca = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(qA, nnz));
+    ColChunk<BATCH> cb = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(qB, nnz));
+    for (; pr < npairs; pr += stride) {
+        int64_t nqA, neA, nqB, neB;
+        extents(pr + stride, nqA, neA, nqB, neB);
+        const int rA = 2 * pr;
+        const bool hasB = rA + 1 < nrows;
+        const int row = half ? min(rA + 1, nrows - 1) : rA;
+        const bool live = half == 0 || hasB;
+        const size_t own = (size_t)row * RQ + hl;
+        float4 oc = make_float4(0.f, 0.f, 0.f, 0.f), op = oc;
+        if (live) {
+            oc = cur[own];
+            if (beta) op = po[own];
+        }
+        const int64_t my_end = half ? eB : eA;
+        const int64_t my_beg = half ? qB : qA;
+        const int64_t span = max(eA - qA, eB - qB);
+        float2 a0 = make_float2(0.f, 0.f), a1 = a0;
+        for (int64_t k = 0; k < span; k += BATCH) {
+            const int64_t sA = min(qA + k, nnz), sB = min(qB + k, nnz);
+            float4 x[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int c = half ? cb.c[u] : ca.c[u];
+                x[u] = cur[(size_t)c * RQ + hl];
+            }
+            const ValChunk<HT, BATCH> ha = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + sA);
+            const ValChunk<HT, BATCH> hb = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + sB);
+            const bool more = k + BATCH < span;
+            ca = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(more ? qA + k + BATCH : nqA, nnz));
+            cb = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(more ? qB + k + BATCH : nqB, nnz));
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const HT h = half ? hb.h[u] : ha.h[u];
+                if (my_beg + k + u < my_end) {
+                    mac(a0, h, make_float2(x[u].x, x[u].y));
+                    mac(a1, h, make_float2(x[u].z, x[u].w));
+                }
+            }
+        }
+        if (span == 0) {
+            ca = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(nqA, nnz));
+            cb = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(nqB, nnz));
+        }
+        if (live) {
+            const float2 c0 = make_float2(oc.x, oc.y), c1 = make_float2(oc.z, oc.w);
+            float2 o0, o1;
+            o0.x = al * a0.x - op.x;
+            o0.y = al * a0.y - op.y;
+            o1.x = al * a1.x - op.z;
+            o1.y = al * a1.y - op.w;
+            po[own] = make_float4(o0.x, o0.y, o1.x, o1.y);
+            d1a += nrm2(c0);
+            d1b += nrm2(c1);
+            d2a += redot(o0, c0);
+            d2b += redot(o1, c1);
+            if (direct) {
+                const float4 vv = vin[own];
+                d3a += redot(make_float2(vv.x, vv.y), c0);
+                d3b += redot(make_float2(vv.z, vv.w), c1);
+            }
+        }
+        qA = nqA;
+        eA = neA;
+        qB = nqB;
+        eB = neB;
+    }
+
+    // fold the halves (fixed order), then the wavefronts
+    d1a += __shfl_xor(d1a, 32);
+    d1b += __shfl_xor(d1b, 32);
+    d2a += __shfl_xor(d2a, 32);
+    d2b += __shfl_xor(d2b, 32);
+    d3a += __shfl_xor(d3a, 32);
+    d3b += __shfl_xor(d3b, 32);
+    __shared__ double red[kWaves][R][3];
+    if (half == 0) {
+        red[wave][2 * hl][0] = d1a;
+        red[wave][2 * hl][1] = d2a;
+        red[wave][2 * hl][2] = d3a;
+        red[wave][2 * hl + 1][0] = d1b;
+        red[wave][2 * hl + 1][1] = d2b;
+        red[wave][2 * hl + 1][2] = d3b;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double sum = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) sum += red[w][r][k];
+        part[((size_t)lb * R + r) * 3 + k] = sum;
+    }
+}
+
 // ---- variant 12: row groups -------------------------------------------------------
 // A wavefront owns kGroupRows consecutive rows (spatial order) at once.  Their
 // column lists are merged ahead of time, so every distinct neighbour row is
@@ -1276,6 +1798,27 @@ hipError_t launch_typed(const StepArgs& a, hipStream_t s)
         case 64: return launch_tiled<HT, XT, 4>(a, s);
         case 128: return launch_tiled<HT, XT, 8>(a, s);
         default: break;
+        }
+    }
+    if constexpr (sizeof(XT) >= 8) {
+        if (a.variant == 13 && a.chunk_wg != nullptr) return launch_chunk<HT, XT>(a, s);
+    }
+    if (a.variant == 14) return launch_xchunk<HT, XT>(a, s);
+    if constexpr (sizeof(XT) == 8 && sizeof(HT) <= 8) {
+        if (a.variant == 15 || a.variant == 16) {
+            // complex FP32 vectors only (float2), R = 64, T_{j-1} and T_{j+1} in one buffer
+            if (a.R != 64 || a.prev != a.out || sizeof(XT) != 2 * sizeof(float) || !std::is_same<XT, float2>::value)
+                return hipErrorInvalidValue;
+            const float4* cur4 = static_cast<const float4*>(a.cur);
+            float4* po4 = static_cast<float4*>(a.out);
+            const float4* v4 = static_cast<const float4*>(a.v);
+            if (a.variant == 15)
+                hipLaunchKernelGGL((kpm_step_pair_kernel<HT, 8>), dim3(a.grid), dim3(kThreads), 0, s, a.row_ptr, a.col,
+                                   static_cast<const HT*>(a.val), cur4, po4, v4, a.n, a.alpha, a.beta, a.part);
+            else
+                hipLaunchKernelGGL((kpm_step_pair_kernel<HT, 16>), dim3(a.grid), dim3(kThreads), 0, s, a.row_ptr,
+                                   a.col, static_cast<const HT*>(a.val), cur4, po4, v4, a.n, a.alpha, a.beta, a.part);
+            return hipGetLastError();
         }
     }
     if (LPR == 64 && VPL == 1 && a.variant == 12 && a.ngroups > 0 && a.prev == a.out) {

@@ -252,7 +252,7 @@ def test_spatial_order_matches_natural_order(dev, name):
     assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15", "16"])
 @pytest.mark.parametrize("R", [4, 16, 64, 128])
 def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     """Every step-kernel variant (lane-per-vector, wavefront-per-row with 8/16
@@ -274,6 +274,93 @@ def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     mur = dev.kpm(M, random_phase=(R, 5), scheme=L.SCHEME_REFERENCE)
     omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
     assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+
+
+@pytest.mark.parametrize("cfg", [("512", "78", "256"), ("256", "16", "8"), ("1024", "40", "37"), ("512", "120", "4096")])
+@pytest.mark.parametrize("R", [64, 128])
+def test_chunk_variant_shapes(dev, oracle_mod, monkeypatch, cfg, R):
+    """Vector-chunked tiles (variant 13) under every workgroup size, LDS budgets
+    that cut tiles by entry count or by row count, and explicit grids (runs of
+    tiles per workgroup, empty runs): moments equal the oracle's for random
+    phases (both schemes), for a real H with complex vectors, and for the
+    vector-start scheme (the third dot)."""
+    nt, kb, tmax = cfg
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", "13")
+    monkeypatch.setenv("LEO2D_CHUNK_THREADS", nt)
+    monkeypatch.setenv("LEO2D_CHUNK_LDS_KB", kb)
+    monkeypatch.setenv("LEO2D_CHUNK_TMAX", tmax)
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = 0.009, 0.021
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 40
+    vecs = oracle_mod.random_phase(geom.n, R, 4)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    for grid in (0, 8, 24):
+        for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+            mu = dev.kpm(M, random_phase=(R, 4), scheme=L.SCHEME_DOUBLING, precision=prec, grid=grid)
+            assert np.max(np.abs(mu - omu)) <= tol * geom.n, (cfg, R, grid, prec)
+    assert dev.timing().kernel == 13
+    mur = dev.kpm(M, random_phase=(R, 4), scheme=L.SCHEME_REFERENCE)
+    omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+    assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+    # real H (k = 0), complex random-phase vectors
+    job.complex_matrix = 0
+    job.k[0], job.k[1] = 0.0, 0.0
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    mu = dev.kpm(M, random_phase=(R, 4), scheme=L.SCHEME_DOUBLING)
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+    # real start vectors through the reference's vector-start scheme (third dot)
+    V = np.random.default_rng(6).standard_normal((R, geom.n))
+    mu = dev.kpm(M, vectors=V)
+    assert dev.timing().kernel == 13
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=V, nthreads=8)
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+
+
+@pytest.mark.parametrize("R", [16, 32, 64, 128])
+def test_xcd_chunk_variant(dev, oracle_mod, monkeypatch, R):
+    """Vector chunks split over the XCDs (variant 14): every (chunk, row part)
+    pair is covered exactly once for grids whose per-XCD share does not divide
+    the rows; moments equal the oracle's for random phases (both precisions), a
+    real H with complex vectors, and real vectors through the vector-start
+    scheme (third dot)."""
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", "14")
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = -0.006, 0.017
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 40
+    vecs = oracle_mod.random_phase(geom.n, R, 8)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    for grid in (0, 8, 40):
+        for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+            mu = dev.kpm(M, random_phase=(R, 8), scheme=L.SCHEME_DOUBLING, precision=prec, grid=grid)
+            assert np.max(np.abs(mu - omu)) <= tol * geom.n, (R, grid, prec)
+            assert dev.timing().kernel == 14
+    mur = dev.kpm(M, random_phase=(R, 8), scheme=L.SCHEME_REFERENCE)
+    omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+    assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+    job.complex_matrix = 0
+    job.k[0], job.k[1] = 0.0, 0.0
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    mu = dev.kpm(M, random_phase=(R, 8), scheme=L.SCHEME_DOUBLING)
+    assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n
+    V = np.random.default_rng(7).standard_normal((R, geom.n))
+    for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+        mu = dev.kpm(M, vectors=V, precision=prec)
+        omu = oracle_mod.kpm(rp, ci, v, M, vecs=V, nthreads=8)
+        assert np.max(np.abs(mu - omu)) <= tol * geom.n, (R, prec)
 
 
 @pytest.mark.parametrize("variant", ["1", "2", "8", "9"])
