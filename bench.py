@@ -515,6 +515,10 @@ def kernel_name(variant, t):
         return f"kpm_step_smem_kernel<{t},{t},1,8>"
     if variant == 1:
         return f"kpm_step_wave_kernel<{t},{t},1,8,256>"
+    if variant in (15, 16):
+        return f"kpm_step_pair_kernel<{t},{8 if variant == 15 else 16}>"
+    if variant == 14:
+        return f"kpm_step_xchunk_kernel<{t},{t},4>"
     return f"kpm step variant {variant}"
 
 

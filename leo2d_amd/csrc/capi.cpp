@@ -1291,7 +1291,17 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
             if ((sa.variant == 15 || sa.variant == 16) && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32))
                 sa.variant = mixed && spatial && B % 64 == 0 ? 8 : 1;
-            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14) sa.variant = 0;
+            if (sa.variant == 17) {
+                // 16-byte lanes: rows per wavefront = 1024 / (B x element); complex vectors only
+                const int per_row = B * (int)leo2d::vec_bytes(vec_type);
+                const bool cpx = vec_type == leo2d::VEC_CPX_F64 || vec_type == leo2d::VEC_CPX_F32;
+                // (a complex FP64 matrix does not fit the scalar registers of this kernel)
+                if (!spatial || !cpx || mat_type == leo2d::MAT_CPX_F64 || (1024 / per_row != 2 && 1024 / per_row != 4) ||
+                    1024 % per_row != 0)
+                    sa.variant = B >= 64 ? 1 : 0;
+            }
+            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14 && sa.variant != 17)
+                sa.variant = 0;
             kernel_used = sa.variant;
             if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
                 // larger workgroups: keep the total number of resident wavefronts

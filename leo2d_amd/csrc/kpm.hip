@@ -297,16 +297,6 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
             nq0 = rp[next];
             nq1 = rp[next + 1];
         }
-        // the row's own T_j and T_{j-1} elements, issued ahead of the gathers
-        // so that their latency overlaps the row's accumulation
-        XT own_c[VPL], own_p[VPL];
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-            const size_t idx = (size_t)row * R + lane + 64 * v;
-            own_c[v] = cur[idx];
-            if (beta) own_p[v] = ld_s<STREAM>(prev + idx);
-            else zero(own_p[v]);
-        }
         AT acc[VPL];
 #pragma unroll
         for (int v = 0; v < VPL; ++v) zero(acc[v]);
@@ -348,8 +338,11 @@ __global__ __launch_bounds__(NT) void kpm_step_wave_kernel(StepArgs a)
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             const size_t idx = (size_t)row * R + lane + 64 * v;
-            const XT c = own_c[v];
-            const XT o = comb(alpha, acc[v], own_p[v]);
+            const XT c = cur[idx];
+            XT p;
+            if (beta) p = ld_s<STREAM>(prev + idx);
+            else zero(p);
+            const XT o = comb(alpha, acc[v], p);
             st_s<STREAM>(out + idx, o);
             d1[v] += nrm2(c);
             d2[v] += redot(o, c);
@@ -1528,6 +1521,18 @@ hipError_t launch_xchunk(const StepArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// 16-byte lane elements: two complex FP32 values (float4) or one complex FP64 (double2)
+__device__ inline void zero16(float4& a) { a = make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ inline void zero16(double2& a) { a = make_double2(0.0, 0.0); }
+__device__ inline void split16(const float4& a, float2* x)
+{
+    x[0] = make_float2(a.x, a.y);
+    x[1] = make_float2(a.z, a.w);
+}
+__device__ inline void split16(const double2& a, double2* x) { x[0] = a; }
+__device__ inline float4 join16(const float2* x) { return make_float4(x[0].x, x[0].y, x[1].x, x[1].y); }
+__device__ inline double2 join16(const double2* x) { return x[0]; }
+
 // ---- variant 15: FP32 recursion with 16-byte gathers -----------------------------
 // A complex FP32 row of T_j (R = 64) is 512 bytes.  The wavefront-per-row
 // kernels gather it with 8-byte lanes, so a gather instruction moves half the
@@ -1676,6 +1681,175 @@ __global__ __launch_bounds__(kThreads) void kpm_step_pair_kernel(const int64_t* 
     }
 }
 
+// ---- variant 17: several rows per wavefront with 16-byte gathers (R = 16, 32) ----
+// The general form of variant 15 for smaller vector blocks: a row of T_j is
+// R x element bytes, each lane carries 16 bytes of it (two complex FP32 or one
+// complex FP64 vectors), so a wavefront covers RPW = 64 x 16 / (R x element)
+// rows at once and one gather instruction fetches a neighbour row for each of
+// them.  Each row's entries come through the scalar path in BATCH-entry chunks,
+// the column chunks one chunk ahead; every lane group selects its own row's
+// entry, and every row is summed entry by entry in CSR order.
+template <typename T, int N> struct Chunks { T v[N]; };
+
+template <typename HT, typename XT, int RPW, int BATCH>
+__global__ __launch_bounds__(kThreads) void kpm_step_rows_kernel(const int64_t* __restrict__ rp,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const HT* __restrict__ val,
+                                                                 const void* __restrict__ cur_v,
+                                                                 void* __restrict__ po_v,
+                                                                 const void* __restrict__ vin_v, int nrows,
+                                                                 double alpha, int beta, double* __restrict__ part)
+{
+    constexpr int VL = 16 / (int)sizeof(XT);   // vectors per lane
+    constexpr int LPR = 64 / RPW;              // lanes per row
+    constexpr int R = LPR * VL;
+    using LT = typename std::conditional<VL == 2, float4, double2>::type;   // 16-byte lane element
+    using AT = typename acc_of<XT>::type;
+    const LT* __restrict__ cur = static_cast<const LT*>(cur_v);
+    LT* __restrict__ po = static_cast<LT*>(po_v);
+    const LT* __restrict__ vin = static_cast<const LT*>(vin_v);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int grp = lane / LPR;                // row slot of this lane
+    const int gl = lane % LPR;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    const int stride = G * kWaves;
+    const int nset = (nrows + RPW - 1) / RPW;  // row sets of RPW consecutive rows
+    const int64_t nnz = rp[nrows];             // chunk loads start at or before nnz (BATCH padding follows)
+    const bool direct = vin != nullptr;
+
+    double d1[VL], d2[VL], d3[VL];
+#pragma unroll
+    for (int v = 0; v < VL; ++v) { d1[v] = 0.0; d2[v] = 0.0; d3[v] = 0.0; }
+
+    // row r of set s: q[r] .. e[r] (empty past the last row)
+    auto extents = [&](int set, int64_t* q, int64_t* e) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) { q[r] = 0; e[r] = 0; }
+        if (set < nset) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int row = min(set * RPW + r, nrows);
+                q[r] = rp[row];
+                e[r] = set * RPW + r < nrows ? rp[row + 1] : q[r];
+            }
+        }
+    };
+    int set = __builtin_amdgcn_readfirstlane(lb * kWaves + wave);
+    int64_t q[RPW], e[RPW];
+    extents(set, q, e);
+    Chunks<ColChunk<BATCH>, RPW> cc;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) cc.v[r] = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(q[r], nnz));
+    for (; set < nset; set += stride) {
+        int64_t nq[RPW], ne[RPW];
+        extents(set + stride, nq, ne);
+        int64_t span = 0, my_beg = q[0], my_end = e[0];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            span = max(span, e[r] - q[r]);
+            if (grp == r) { my_beg = q[r]; my_end = e[r]; }
+        }
+        const int row = set * RPW + grp;
+        const bool live = row < nrows;
+        const size_t own = (size_t)row * LPR + gl;
+        LT oc, op;
+        zero16(oc);
+        zero16(op);
+        if (live) {
+            oc = cur[own];
+            if (beta) op = po[own];
+        }
+        AT acc[VL];
+#pragma unroll
+        for (int v = 0; v < VL; ++v) zero(acc[v]);
+        for (int64_t k = 0; k < span; k += BATCH) {
+            LT x[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                int c = cc.v[0].c[u];
+#pragma unroll
+                for (int r = 1; r < RPW; ++r) c = grp == r ? cc.v[r].c[u] : c;
+                x[u] = cur[(size_t)c * LPR + gl];
+            }
+            Chunks<ValChunk<HT, BATCH>, RPW> hc;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+                hc.v[r] = *reinterpret_cast<const ValChunk<HT, BATCH>*>(val + min(q[r] + k, nnz));
+            const bool more = k + BATCH < span;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+                cc.v[r] = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(more ? q[r] + k + BATCH : nq[r], nnz));
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                HT h = hc.v[0].h[u];
+#pragma unroll
+                for (int r = 1; r < RPW; ++r) h = grp == r ? hc.v[r].h[u] : h;
+                if (my_beg + k + u < my_end) {
+                    XT xv[VL];
+                    split16(x[u], xv);
+#pragma unroll
+                    for (int v = 0; v < VL; ++v) mac(acc[v], h, xv[v]);
+                }
+            }
+        }
+        if (span == 0) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) cc.v[r] = *reinterpret_cast<const ColChunk<BATCH>*>(col + min(nq[r], nnz));
+        }
+        if (live) {
+            XT cv[VL], pv[VL], ov[VL];
+            split16(oc, cv);
+            split16(op, pv);
+#pragma unroll
+            for (int v = 0; v < VL; ++v) {
+                ov[v] = comb(alpha, acc[v], pv[v]);
+                d1[v] += nrm2(cv[v]);
+                d2[v] += redot(ov[v], cv[v]);
+            }
+            po[own] = join16(ov);
+            if (direct) {
+                XT vv[VL];
+                split16(vin[own], vv);
+#pragma unroll
+                for (int v = 0; v < VL; ++v) d3[v] += redot(vv[v], cv[v]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) { q[r] = nq[r]; e[r] = ne[r]; }
+    }
+
+    // fold the row slots (fixed xor tree), then the wavefronts
+#pragma unroll
+    for (int m = LPR; m < 64; m <<= 1) {
+#pragma unroll
+        for (int v = 0; v < VL; ++v) {
+            d1[v] += __shfl_xor(d1[v], m);
+            d2[v] += __shfl_xor(d2[v], m);
+            d3[v] += __shfl_xor(d3[v], m);
+        }
+    }
+    __shared__ double red[kWaves][R][3];
+    if (grp == 0) {
+#pragma unroll
+        for (int v = 0; v < VL; ++v) {
+            red[wave][gl * VL + v][0] = d1[v];
+            red[wave][gl * VL + v][1] = d2[v];
+            red[wave][gl * VL + v][2] = d3[v];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * 3; t += kThreads) {
+        const int r = t / 3, k = t % 3;
+        double sum = red[0][r][k];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) sum += red[w][r][k];
+        part[((size_t)lb * R + r) * 3 + k] = sum;
+    }
+}
+
 // ---- variant 12: row groups -------------------------------------------------------
 // A wavefront owns kGroupRows consecutive rows (spatial order) at once.  Their
 // column lists are merged ahead of time, so every distinct neighbour row is
@@ -1804,6 +1978,21 @@ hipError_t launch_typed(const StepArgs& a, hipStream_t s)
         if (a.variant == 13 && a.chunk_wg != nullptr) return launch_chunk<HT, XT>(a, s);
     }
     if (a.variant == 14) return launch_xchunk<HT, XT>(a, s);
+    if constexpr ((std::is_same<XT, float2>::value || std::is_same<XT, double2>::value) && sizeof(HT) <= 8) {
+        if (a.variant == 17 && a.prev == a.out) {
+            constexpr int VL = 16 / (int)sizeof(XT);
+            const int rpw = 64 * VL / a.R;   // rows per wavefront
+            constexpr int BT = sizeof(HT) >= 16 ? 4 : 8;
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(a.grid), dim3(kThreads), 0, s, a.row_ptr, a.col,
+                                   static_cast<const HT*>(a.val), a.cur, a.out, a.v, a.n, a.alpha, a.beta, a.part);
+                return hipGetLastError();
+            };
+            if (rpw == 2) return go(kpm_step_rows_kernel<HT, XT, 2, BT>);
+            if (rpw == 4) return go(kpm_step_rows_kernel<HT, XT, 4, sizeof(HT) >= 8 ? 4 : 8>);
+            return hipErrorInvalidValue;
+        }
+    }
     if constexpr (sizeof(XT) == 8 && sizeof(HT) <= 8) {
         if (a.variant == 15 || a.variant == 16) {
             // complex FP32 vectors only (float2), R = 64, T_{j-1} and T_{j+1} in one buffer

@@ -252,7 +252,7 @@ def test_spatial_order_matches_natural_order(dev, name):
     assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15", "16"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15", "16", "17"])
 @pytest.mark.parametrize("R", [4, 16, 64, 128])
 def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     """Every step-kernel variant (lane-per-vector, wavefront-per-row with 8/16
@@ -361,6 +361,44 @@ def test_xcd_chunk_variant(dev, oracle_mod, monkeypatch, R):
         mu = dev.kpm(M, vectors=V, precision=prec)
         omu = oracle_mod.kpm(rp, ci, v, M, vecs=V, nthreads=8)
         assert np.max(np.abs(mu - omu)) <= tol * geom.n, (R, prec)
+
+
+@pytest.mark.parametrize("R", [16, 32])
+def test_rows_variant(dev, oracle_mod, monkeypatch, R):
+    """Several rows per wavefront with 16-byte gathers (variant 17; 2 or 4 rows
+    of 512 or 256 bytes, else the lane-per-vector kernel): a real H in both
+    precisions and a complex H in FP32, random phases through both
+    schemes, complex start vectors through the vector-start scheme, and grids
+    whose sweep leaves a partial last set of rows."""
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", "17")
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    M = 40
+    vecs = oracle_mod.random_phase(geom.n, R, 12)
+    for cpx in (0, 1):
+        job.complex_matrix = cpx
+        job.k[0], job.k[1] = (0.011, -0.013) if cpx else (0.0, 0.0)
+        dev.assemble(job, L.PATTERN_PRUNED)
+        rp, ci, v = dev.csr_copy()
+        omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+        for grid in (0, 8, 24):
+            for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+                mu = dev.kpm(M, random_phase=(R, 12), scheme=L.SCHEME_DOUBLING, precision=prec, grid=grid)
+                assert np.max(np.abs(mu - omu)) <= tol * geom.n, (cpx, R, grid, prec)
+                # 16-byte lanes need 2 or 4 rows per wavefront: R x element = 512 or 256 bytes
+                elem = 16 if prec == L.PREC_FP64 else 8
+                fits = R * elem in (256, 512) and not (cpx and prec == L.PREC_FP64)
+                assert dev.timing().kernel == (17 if fits else 0), (cpx, R, prec)
+        mur = dev.kpm(M, random_phase=(R, 12), scheme=L.SCHEME_REFERENCE)
+        omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+        assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+        rng = np.random.default_rng(13)
+        V = rng.standard_normal((R, geom.n)) + 1j * rng.standard_normal((R, geom.n))
+        ov = oracle_mod.kpm(rp, ci, v, M, vecs=V, nthreads=8)
+        for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+            mu = dev.kpm(M, vectors=V, precision=prec)
+            assert np.max(np.abs(mu - ov)) <= tol * 2 * geom.n, (cpx, R, prec)
 
 
 @pytest.mark.parametrize("variant", ["1", "2", "8", "9"])
