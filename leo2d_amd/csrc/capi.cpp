@@ -46,14 +46,17 @@ struct NanError : std::runtime_error {
     explicit NanError(const std::string& m) : std::runtime_error(m) {}
 };
 
-// step-kernel variant for R >= 64 (LEO2D_KPM_VARIANT overrides the default):
-// the wavefront-per-row kernel with readlane broadcasts (1) for FP64; for the
-// FP32 recursion, two rows per wavefront with 16-byte gathers and the matrix
-// on the scalar path (15; it falls back to the one-row scalar-path kernel 8
-// where it does not apply)
-int kpm_variant(bool mixed, bool spatial)
+// step kernel for a block of B vectors (LEO2D_KPM_VARIANT overrides the
+// default).  B >= 64: the wavefront-per-row kernel with readlane broadcasts (1)
+// for FP64; for the FP32 recursion, two rows per wavefront with 16-byte gathers
+// and the matrix on the scalar path (15; it falls back to the one-row
+// scalar-path kernel 8 where it does not apply).  B = 16 or 32 in the spatial
+// order: vector chunks split over the XCDs (14; 1.2-1.3x the lane-per-vector
+// kernel on a 4.7 M-orbital flake at B = 32).  Other B: lane per vector (0).
+int kpm_variant(bool mixed, bool spatial, int B)
 {
     if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) return std::atoi(e);
+    if (spatial && (B == 16 || B == 32)) return 14;
     return mixed && spatial ? 15 : 1;
 }
 
@@ -1188,7 +1191,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             sa.part = (double*)part.p;
             sa.grid = G;
             sa.v = keep_v ? V.p : nullptr;
-            sa.variant = kpm_variant(mixed, spatial);
+            sa.variant = kpm_variant(mixed, spatial, B);
             sa.xcd_rows = kpm_sched() == 1 ? (n + 7) / 8 : 0;
             const bool tiled = sa.variant == 3 || sa.variant == 4;
             if (tiled && (B % leo2d::kTileGroup != 0 ||
