@@ -206,6 +206,7 @@ int leo2d_batch_info(const leo2d_handle* h, int32_t* jobs, int64_t* rows, int64_
 
 #define LEO2D_BATCH_ROW_GROUPS 0    /* 64/R rows per wavefront */
 #define LEO2D_BATCH_ENTRY_SLOTS 1   /* one row per wavefront, 64/R entries in flight */
+#define LEO2D_BATCH_ROW_CHUNKS 2    /* R = 16, 32: 64/R rows per wavefront, 16-entry chunks broadcast in 16-lane rows */
 
 typedef struct {
     int32_t M;            /* moments per vector, multiple of 4 */

@@ -39,6 +39,7 @@ SCHEME_REFERENCE = 0
 SCHEME_DOUBLING = 1
 BATCH_ROW_GROUPS = 0
 BATCH_ENTRY_SLOTS = 1
+BATCH_ROW_CHUNKS = 2
 
 # every symbol include/leo2d.h declares
 EXPORTS = [

@@ -19,7 +19,13 @@
 //                           of a row walked sequentially by its lanes;
 //   mode 1 ("entry slots"): one row per wavefront, 64/R entries of the row in
 //                           flight at once (lanes e, e+R, ...), folded with
-//                           cross-lane shuffles at the end of the row.
+//                           cross-lane shuffles at the end of the row;
+//   mode 2 ("row chunks", R = 16 or 32): 64/R rows per wavefront as in mode 0,
+//                           but each 16-lane row of lanes loads 16 of its
+//                           matrix row's entries in one access and passes them
+//                           on with DPP row broadcasts, so 16 gathers are in
+//                           flight per lane and the next 16 entries load while
+//                           they land.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -143,6 +149,108 @@ __global__ __launch_bounds__(kThreads) void batch_step_kernel(BatchStepArgs a)
     }
 }
 
+template <typename HT, typename XT, int R>
+__global__ __launch_bounds__(kThreads) void batch_step_chunk_kernel(BatchStepArgs a)
+{
+    static_assert(R == 16 || R == 32, "16-lane row slots");
+    using AT = typename acc_of<XT>::type;
+    constexpr int RPW = 64 / R;          // matrix rows per wavefront
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int slot = lane / R;
+    const int vi = lane % R;
+    const int dl = lane & 15;            // lane within its 16-lane row
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+
+    const HT* __restrict__ val = static_cast<const HT*>(a.val);
+    const int32_t* __restrict__ col = a.col;
+    const int64_t* __restrict__ rp = a.row_ptr;
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    XT* out = static_cast<XT*>(a.out);   // holds T_{j-1} on entry, T_{j+1} on exit
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+
+    __shared__ double red[kWaves][R][2];
+
+    for (int t = lb; t < a.ntiles; t += G) {
+        const int row0 = a.tile_row0[t], row1 = a.tile_row0[t + 1];
+        double d1 = 0.0, d2 = 0.0;
+        for (int base = row0 + wave * RPW; base < row1; base += kWaves * RPW) {
+            const int row = base + slot;
+            const bool live = row < row1;
+            int64_t kb = 0;
+            int cnt = 0;
+            XT own_c, own_p;
+            zero(own_c);
+            zero(own_p);
+            if (live) {
+                kb = rp[row];
+                cnt = (int)(rp[row + 1] - kb);
+                const size_t idx = (size_t)row * R + vi;
+                own_c = cur[idx];
+                if (beta) own_p = out[idx];
+            }
+            int mx = cnt;
+#pragma unroll
+            for (int off = R; off < 64; off <<= 1) mx = max(mx, __shfl_xor(mx, off));
+            mx = __builtin_amdgcn_readfirstlane(mx);
+            int mc = 0;
+            HT mh;
+            zero(mh);
+            if (dl < cnt) {
+                mc = col[kb + dl];
+                mh = val[kb + dl];
+            }
+            AT acc;
+            zero(acc);
+            for (int k0 = 0; k0 < mx; k0 += 16) {
+                int pc = 0;
+                HT ph;
+                zero(ph);
+                const int q = k0 + 16 + dl;
+                if (q < cnt) {
+                    pc = col[kb + q];
+                    ph = val[kb + q];
+                }
+                const int m = cnt - k0;
+                XT xs[16];
+                xchunk_gather<XT, R, 0>(xs, m, mc, cur, vi);
+                xchunk_mac<HT, XT, AT, 0>(acc, m, mh, xs);
+                mc = pc;
+                mh = ph;
+            }
+            if (live) {
+                const size_t idx = (size_t)row * R + vi;
+                const XT o = comb(alpha, acc, own_p);
+                out[idx] = o;
+                d1 += nrm2(own_c);
+                d2 += redot(o, own_c);
+            }
+        }
+        // fold the row slots of the wavefront, then the wavefronts
+#pragma unroll
+        for (int off = R; off < 64; off <<= 1) {
+            d1 += __shfl_xor(d1, off);
+            d2 += __shfl_xor(d2, off);
+        }
+        if (lane < R) {
+            red[wave][vi][0] = d1;
+            red[wave][vi][1] = d2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * R) {
+            const int rr = threadIdx.x % R, k = threadIdx.x / R;
+            double s = red[0][rr][k];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) s += red[w][rr][k];
+            a.part[((size_t)t * R + rr) * 2 + k] = s;
+        }
+        __syncthreads();
+    }
+}
+
 // mu[p][r][.] from the tiles of job p (doubling identity from the first step):
 //   j = 0: mu_0 = <T0|T0>, mu_1 = <T1|T0>;  j >= 1: mu_2j = 2<Tj|Tj> - mu_0,
 //   mu_2j+1 = 2<Tj+1|Tj> - mu_1
@@ -204,6 +312,14 @@ __global__ void batch_offset_kernel(const int32_t* col_in, int64_t nnz, int32_t 
 template <typename HT, typename XT, int R>
 hipError_t launch_mode(const BatchStepArgs& a, hipStream_t s)
 {
+    if constexpr (R == 16 || R == 32) {
+        if (a.mode == 2) {
+            hipLaunchKernelGGL((batch_step_chunk_kernel<HT, XT, R>), dim3(a.grid), dim3(kThreads), 0, s, a);
+            return hipGetLastError();
+        }
+    } else {
+        if (a.mode == 2) return hipErrorInvalidValue;
+    }
     if (a.mode == 1)
         hipLaunchKernelGGL((batch_step_kernel<HT, XT, R, 1>), dim3(a.grid), dim3(kThreads), 0, s, a);
     else

@@ -1577,7 +1577,9 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4");
         if (R != 4 && R != 8 && R != 16 && R != 32) throw ArgError("R must be 4, 8, 16 or 32");
         if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
-        if (p->mode != LEO2D_BATCH_ROW_GROUPS && p->mode != LEO2D_BATCH_ENTRY_SLOTS) throw ArgError("unknown mode");
+        if (p->mode != LEO2D_BATCH_ROW_GROUPS && p->mode != LEO2D_BATCH_ENTRY_SLOTS && p->mode != LEO2D_BATCH_ROW_CHUNKS)
+            throw ArgError("unknown mode");
+        if (p->mode == LEO2D_BATCH_ROW_CHUNKS && R != 16 && R != 32) throw ArgError("row chunks need R = 16 or 32");
         constexpr int kMinTile = 16;
         if (p->tile_rows != 0 && p->tile_rows < kMinTile) throw ArgError("tile_rows must be 0 or at least 16");
         if (p->grid < 0 || p->grid % 8 != 0) throw ArgError("grid must be 0 or a positive multiple of 8");
@@ -1676,7 +1678,8 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 check(hipEventRecord(e0, s), "event");
                 float best = 1e30f;
                 for (int tt : {64, 128, 256, 512}) {
-                    for (int md : {0, 1}) {
+                    for (int md : {0, 1, 2}) {
+                        if (md == 2 && R != 16 && R != 32) continue;
                         int last = -1;
                         for (int per_cu : {2, 4, 8}) {
                             const int g = std::min(batch_default_grid((int)((B.rows + tt - 1) / tt) + P), 256 * per_cu);

@@ -111,7 +111,7 @@ struct BatchStepArgs {
     int ntiles;
     double* part;              // [ntiles][R][2]
     int grid;                  // workgroups (multiple of 8)
-    int mode;                  // 0 = row groups, 1 = entry slots
+    int mode;                  // 0 = row groups, 1 = entry slots, 2 = row chunks (R = 16, 32)
 };
 hipError_t launch_batch_step(int mat_type, int vec_type, const BatchStepArgs& a, hipStream_t s);
 // mu[p][r][M] of job p from its tiles [job_tile0[p], job_tile0[p+1]) (doubling identity)

@@ -989,26 +989,6 @@ __global__ __launch_bounds__(kThreads) void kpm_step_smem_kernel(const int64_t* 
 // The first chunk of a slot's next row and its T_j / T_{j-1} elements are
 // loaded before the current row is computed.  Several workgroups share a CU
 // (LDS budget set at launch), so staging in one overlaps compute in another.
-template <int K>
-__device__ inline int bc16(int v)
-{
-    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
-}
-template <int K>
-__device__ inline double bc16(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = bc16<K>((int)(b & 0xffffffffLL));
-    const int hi = bc16<K>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <int K>
-__device__ inline float bc16(float v) { return __int_as_float(bc16<K>(__float_as_int(v))); }
-template <int K>
-__device__ inline double2 bc16(double2 v) { return make_double2(bc16<K>(v.x), bc16<K>(v.y)); }
-template <int K>
-__device__ inline float2 bc16(float2 v) { return make_float2(bc16<K>(v.x), bc16<K>(v.y)); }
-
 // acc += sum over the chunk's first `cnt` entries (entry kk held by lane kk of the 16-lane row)
 template <typename HT, typename XT, typename AT, int K>
 __device__ inline void chunk_mac(AT& acc, int cnt, int mc, HT mh, const XT* xs, int e)
@@ -1348,25 +1328,6 @@ hipError_t launch_chunk(const StepArgs& a, hipStream_t s)
 // is loaded before the current one is consumed.  A workgroup's 4 wavefronts
 // own 16 consecutive rows; the XCD's workgroups sweep the XCD's rows in that
 // interleave, so the XCD works on one compact window of rows at a time.
-template <typename XT, int R, int K>
-__device__ inline void xchunk_gather(XT* xs, int m, int mc, const XT* __restrict__ cur, int voff)
-{
-    if constexpr (K < 16) {
-        const int c = bc16<K>(mc);
-        if (K < m) xs[K] = cur[(size_t)c * R + voff];
-        xchunk_gather<XT, R, K + 1>(xs, m, mc, cur, voff);
-    }
-}
-template <typename HT, typename XT, typename AT, int K>
-__device__ inline void xchunk_mac(AT& acc, int m, HT mh, const XT* xs)
-{
-    if constexpr (K < 16) {
-        const HT h = bc16<K>(mh);
-        if (K < m) mac(acc, h, xs[K]);
-        xchunk_mac<HT, XT, AT, K + 1>(acc, m, mh, xs);
-    }
-}
-
 template <typename HT, typename XT, int NCH>
 __global__ __launch_bounds__(kThreads) void kpm_step_xchunk_kernel(StepArgs a)
 {

@@ -100,6 +100,51 @@ __device__ inline double re(double2 a) { return a.x; }
 __device__ inline double re(float2 a) { return a.x; }
 __device__ inline double re(float a) { return a; }
 
+// ---- 16-lane row broadcasts --------------------------------------------------
+// bc16<K>(v): every lane receives v from lane K of its 16-lane row (DPP
+// row_newbcast).  A row of 16 lanes loads 16 CSR entries in one access and
+// passes them to each other this way.
+template <int K>
+__device__ inline int bc16(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+}
+template <int K>
+__device__ inline double bc16(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = bc16<K>((int)(b & 0xffffffffLL));
+    const int hi = bc16<K>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int K>
+__device__ inline float bc16(float v) { return __int_as_float(bc16<K>(__float_as_int(v))); }
+template <int K>
+__device__ inline double2 bc16(double2 v) { return make_double2(bc16<K>(v.x), bc16<K>(v.y)); }
+template <int K>
+__device__ inline float2 bc16(float2 v) { return make_float2(bc16<K>(v.x), bc16<K>(v.y)); }
+
+// gathers and multiply-adds for the 16 entries of one chunk held by a
+// 16-lane row (entry K in lane K); only the first m entries are live
+template <typename XT, int R, int K>
+__device__ inline void xchunk_gather(XT* xs, int m, int mc, const XT* __restrict__ cur, int voff)
+{
+    if constexpr (K < 16) {
+        const int c = bc16<K>(mc);
+        if (K < m) xs[K] = cur[(size_t)c * R + voff];
+        xchunk_gather<XT, R, K + 1>(xs, m, mc, cur, voff);
+    }
+}
+template <typename HT, typename XT, typename AT, int K>
+__device__ inline void xchunk_mac(AT& acc, int m, HT mh, const XT* xs)
+{
+    if constexpr (K < 16) {
+        const HT h = bc16<K>(mh);
+        if (K < m) mac(acc, h, xs[K]);
+        xchunk_mac<HT, XT, AT, K + 1>(acc, m, mh, xs);
+    }
+}
+
 // ---- random-phase start vectors ----------------------------------------------
 // phi(seed, r, i) in [0, 2 pi): three splitmix64 finalisations keyed by the
 // seed, the global vector index and the orbital, so every vector entry is

@@ -38,7 +38,7 @@ def batch(oracle_mod):
     dev.close()
 
 
-@pytest.mark.parametrize("mode", [L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS])
+@pytest.mark.parametrize("mode", [L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS, L.BATCH_ROW_CHUNKS])
 @pytest.mark.parametrize("tile_rows", [64, 100, 256])
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_fp64_matches_oracle(batch, mode, tile_rows, graph):
@@ -68,7 +68,13 @@ def test_batch_other_block_widths(oracle_mod, Rv):
             p = dev.batch_append()
             vecs = oracle_mod.random_phase(geom.n, Rv, 11, r0=p * Rv)
             want.append((geom.n, oracle_mod.kpm(rp, ci, v, 32, vecs=vecs, doubling_all=True, nthreads=8)))
-        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS):
+        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS, L.BATCH_ROW_CHUNKS):
+            if mode == L.BATCH_ROW_CHUNKS and Rv < 16:
+                # 16-lane row slots need 16 or 32 vectors per job
+                with pytest.raises(L.Leo2dError) as e:
+                    dev.batch_kpm(32, Rv, seed=11, mode=mode, graph=False)
+                assert e.value.code == L.ERR_ARG
+                continue
             mu = dev.batch_kpm(32, Rv, seed=11, mode=mode, graph=False)
             for p, (n, omu) in enumerate(want):
                 assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (Rv, mode, p)
@@ -76,11 +82,12 @@ def test_batch_other_block_widths(oracle_mod, Rv):
         dev.close()
 
 
-def test_batch_mixed_precision(batch):
+@pytest.mark.parametrize("mode", [L.BATCH_ROW_GROUPS, L.BATCH_ROW_CHUNKS])
+def test_batch_mixed_precision(batch, mode):
     dev, jobs = batch
-    mu = dev.batch_kpm(M, R, seed=SEED, precision=L.PREC_MIXED, graph=True)
+    mu = dev.batch_kpm(M, R, seed=SEED, precision=L.PREC_MIXED, graph=True, mode=mode)
     for p, (n, omu) in enumerate(jobs):
-        assert np.max(np.abs(mu[p] - omu)) <= 2e-5 * n, p
+        assert np.max(np.abs(mu[p] - omu)) <= 2e-5 * n, (p, mode)
 
 
 def test_batch_autotune_and_cache(batch):
