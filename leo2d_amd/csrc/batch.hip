@@ -254,17 +254,43 @@ __global__ __launch_bounds__(kThreads) void batch_step_chunk_kernel(BatchStepArg
 // mu[p][r][.] from the tiles of job p (doubling identity from the first step):
 //   j = 0: mu_0 = <T0|T0>, mu_1 = <T1|T0>;  j >= 1: mu_2j = 2<Tj|Tj> - mu_0,
 //   mu_2j+1 = 2<Tj+1|Tj> - mu_1
-__global__ void batch_reduce_kernel(const double* part, const int32_t* job_tile0, int R, int M, int j, double* mu)
+// One workgroup per job.  Thread t sums the partials (r, k) = t mod 2R over
+// the tiles tt = first + u + s*NS (u = t / 2R, NS = kReduceThreads / 2R) with
+// four running sums, so several loads are in flight; the strands are then
+// added in order u = 0, 1, ...  The order is fixed, so the moments are
+// deterministic.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(const double* part, const int32_t* job_tile0, int R,
+                                                                      int M, int j, double* mu)
 {
+    __shared__ double strand[kReduceThreads];
     const int p = blockIdx.x;
     const int t = threadIdx.x;
-    if (t >= 2 * R) return;
-    const int r = t % R, k = t / R;
-    double s = 0.0;
-    for (int tt = job_tile0[p]; tt < job_tile0[p + 1]; ++tt) s += part[((size_t)tt * R + r) * 2 + k];
-    double* m = mu + ((size_t)p * R + r) * M;
-    if (j == 0) m[k] = s;
-    else m[2 * j + k] = 2.0 * s - m[k];
+    const int w = 2 * R;
+    const int NS = kReduceThreads / w;
+    const int rk = t % w, u = t / w;
+    const int r = rk % R, k = rk / R;
+    const int t0 = job_tile0[p], t1 = job_tile0[p + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (u < NS) {
+        int tt = t0 + u;
+        for (; tt + 3 * NS < t1; tt += 4 * NS) {
+            s0 += part[((size_t)tt * R + r) * 2 + k];
+            s1 += part[((size_t)(tt + NS) * R + r) * 2 + k];
+            s2 += part[((size_t)(tt + 2 * NS) * R + r) * 2 + k];
+            s3 += part[((size_t)(tt + 3 * NS) * R + r) * 2 + k];
+        }
+        for (; tt < t1; tt += NS) s0 += part[((size_t)tt * R + r) * 2 + k];
+    }
+    strand[t] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (t < w) {
+        double s = 0.0;
+        for (int q = 0; q < NS; ++q) s += strand[q * w + t];
+        double* m = mu + ((size_t)p * R + r) * M;
+        if (j == 0) m[k] = s;
+        else m[2 * j + k] = 2.0 * s - m[k];
+    }
 }
 
 // start vectors of every job: random phases (vector r of job p is global vector
@@ -354,8 +380,8 @@ hipError_t launch_batch_step(int mat_type, int vec_type, const BatchStepArgs& a,
 hipError_t launch_batch_reduce(const double* part, const int32_t* job_tile0, int P, int R, int M, int j, double* mu,
                                hipStream_t s)
 {
-    if (P <= 0 || 2 * R > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(P), dim3(2 * R), 0, s, part, job_tile0, R, M, j, mu);
+    if (P <= 0 || 2 * R > kReduceThreads) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(P), dim3(kReduceThreads), 0, s, part, job_tile0, R, M, j, mu);
     return hipGetLastError();
 }
 

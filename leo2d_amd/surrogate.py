@@ -139,7 +139,9 @@ class ExactLocalKPM:
         def flush():
             if not pending:
                 return
-            mu = dev.batch_kpm(self.M, R, targets=np.array(targets), precision=self.precision, graph=False)
+            # 16- and 32-vector blocks use the row-chunk layout (16-entry chunks broadcast in 16-lane rows)
+            mode = L.BATCH_ROW_CHUNKS if R in (16, 32) else L.BATCH_ROW_GROUPS
+            mu = dev.batch_kpm(self.M, R, targets=np.array(targets), precision=self.precision, graph=False, mode=mode)
             out[np.array(pending)] = mu
             dev.batch_clear()
             pending.clear()
