@@ -16,6 +16,7 @@ import pytest
 from conftest import ROOT, golden_path
 
 import leo2d_amd as L
+from leo2d_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -274,6 +275,36 @@ def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     mur = dev.kpm(M, random_phase=(R, 5), scheme=L.SCHEME_REFERENCE)
     omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
     assert np.max(np.abs(mur - omur)) <= 1e-12 * geom.n
+    # FP32 recursion through the reference's vector-start scheme (the third dot)
+    mur = dev.kpm(M, random_phase=(R, 5), scheme=L.SCHEME_REFERENCE, precision=L.PREC_MIXED)
+    assert np.max(np.abs(mur - omur)) <= 2e-5 * geom.n, (variant, R)
+
+
+@pytest.mark.parametrize("variant", ["1", "8", "14", "15", "17"])
+@pytest.mark.parametrize("R", [32, 64])
+def test_variants_ragged_row_count(dev, oracle_mod, monkeypatch, variant, R):
+    """A flake whose orbital count (1022) is not a multiple of the rows a
+    wavefront, workgroup or XCD part covers (n mod 4 = 2, n mod 8 = 6): the last
+    row sets are partial.  Real H; moments equal the oracle's in both precisions
+    and both schemes."""
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", variant)
+    text = W.tblg_flake_deck(2.0, 21, M=64)
+    deck, geom, _ = _setup(dev, text)
+    assert geom.n == 1022
+    job = geom.jobs()[0]
+    job.energy_rescale, job.energy_shift = 10.5, 0.0
+    dev.assemble(job, L.PATTERN_PRUNED)
+    rp, ci, v = dev.csr_copy()
+    M = 40
+    vecs = oracle_mod.random_phase(geom.n, R, 21)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+    for grid in (0, 8, 40):
+        for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+            mu = dev.kpm(M, random_phase=(R, 21), scheme=L.SCHEME_DOUBLING, precision=prec, grid=grid)
+            assert np.max(np.abs(mu - omu)) <= tol * geom.n, (variant, R, grid, prec)
+            mu = dev.kpm(M, random_phase=(R, 21), scheme=L.SCHEME_REFERENCE, precision=prec, grid=grid)
+            assert np.max(np.abs(mu - omur)) <= tol * geom.n, (variant, R, grid, prec)
 
 
 @pytest.mark.parametrize("cfg", [("512", "78", "256"), ("256", "16", "8"), ("1024", "40", "37"), ("512", "120", "4096")])
