@@ -546,6 +546,8 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     int64_t* rp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
     int32_t* ci = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnnz + 1));
     double* v = (double*)calloc((size_t)(cpx ? 2 : 1) * (size_t)(maxnnz + 1), sizeof(double));
+    /* dxH slots (locality.cpp:4182, 4299): (x1 - x2) t accumulated with H */
+    double* vdx = cpx ? NULL : (double*)calloc((size_t)(maxnnz + 1), sizeof(double));
     char* keep = (char*)calloc((size_t)(maxnnz + 1), 1);
     int64_t ic = 0, xc = 0, input_counter = 0;
     const double resc = job->energy_rescale, eshift = job->energy_shift;
@@ -600,7 +602,10 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                     support = inter_in_support(mat, dx, dy, dz);
                 }
                 if (!cpx) {
-                    if (t != 0) v[input_counter] += t;
+                    if (t != 0) {
+                        v[input_counter] += t;
+                        vdx[input_counter] += (x1 - x2) * t;
+                    }
                 } else {
                     /* peierlsPhase (locality.cpp:7157-7161) + k.(r2 - r1) */
                     double phase = 3.14e-5 * job->B * (x2 - x1) * (0.5) * (y1 + y2);
@@ -632,7 +637,7 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                 if (!keep[q]) continue;
                 ci[w] = ci[q];
                 if (cpx) { v[2 * w] = v[2 * q]; v[2 * w + 1] = v[2 * q + 1]; }
-                else v[w] = v[q];
+                else { v[w] = v[q]; vdx[w] = vdx[q]; }
                 ++w;
             }
         }
@@ -645,13 +650,14 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     out->col = ci;
     out->val = v;
     out->is_complex = cpx;
+    out->val_dx = vdx;
     return 0;
 }
 
 void orc_csr_free(orc_csr* h)
 {
-    free(h->row_ptr); free(h->col); free(h->val);
-    h->row_ptr = NULL; h->col = NULL; h->val = NULL;
+    free(h->row_ptr); free(h->col); free(h->val); free(h->val_dx);
+    h->row_ptr = NULL; h->col = NULL; h->val = NULL; h->val_dx = NULL;
 }
 
 /* ------------------------------------------------------------------ */
@@ -914,5 +920,98 @@ void orc_dos_trace(int M, int S, const double* mu, double a, double b, double sc
     for (int i = 0; i < M; ++i) in[i] = g[i] * mt[i];
     dct3(M, in, out);
     for (int i = 0; i < M; ++i) rho[i] = scale * out[i] / (M_PI * sqrt(a * a - (E[i] - b) * (E[i] - b)));
+    free(g);
+}
+
+/* ------------------------------------------------------------------ */
+/* KPM conductivity: Locality::computeCondKPM (locality.cpp:6348-6727),  */
+/* real H, observable_type 1, cond_poly_par 0                            */
+
+/* y = A x for a real CSR matrix, rows in stored order (spmatrix.cpp:346-360) */
+static void csr_mv(int n, const int64_t* rp, const int32_t* ci, const double* a, const double* x, double* y)
+{
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int64_t q = rp[i]; q < rp[i + 1]; ++q) s += a[q] * x[ci[q]];
+        y[i] = s;
+    }
+}
+
+int orc_cond(int n, const int64_t* rp, const int32_t* ci, const double* val, const double* vdx, int M, int T,
+             const int32_t* targets, double* C)
+{
+    if (M < 2 || !vdx) return -1;
+    double* beta = (double*)malloc(sizeof(double) * (size_t)n * M);
+    double* a0 = (double*)malloc(sizeof(double) * (size_t)n * 4);
+    double *a1 = a0 + n, *a2 = a1 + n, *tmp = a2 + n;
+    for (int t = 0; t < T; ++t) {
+        const int tg = targets[t];
+        /* beta_m = T_m(H) e_t, m = 0 .. M-1 */
+        for (int i = 0; i < n; ++i) beta[i] = 0.0;
+        beta[tg] = 1.0;
+        csr_mv(n, rp, ci, val, beta, beta + n);
+        for (int m = 2; m < M; ++m) {
+            double* bm = beta + (size_t)m * n;
+            csr_mv(n, rp, ci, val, beta + (size_t)(m - 1) * n, bm);
+            for (int i = 0; i < n; ++i) bm[i] = 2.0 * bm[i] - beta[(size_t)(m - 2) * n + i];
+        }
+        /* alpha_0 = -dxH e_t (locality.cpp:4326-4357) */
+        for (int i = 0; i < n; ++i) tmp[i] = 0.0;
+        tmp[tg] = 1.0;
+        csr_mv(n, rp, ci, vdx, tmp, a0);
+        for (int i = 0; i < n; ++i) a0[i] = -a0[i];
+        double* row = C + (size_t)t * M * M;
+        /* a_j = T_j(H) alpha_0: a_1 = H a_0, a_j = 2 H a_{j-1} - a_{j-2} */
+        double *pm = a0, *pc = a1, *pn = a2;
+        for (int j = 0; j < M; ++j) {
+            const double* aj;
+            if (j == 0) {
+                aj = a0;
+            } else if (j == 1) {
+                csr_mv(n, rp, ci, val, a0, a1);
+                aj = a1;
+            } else {
+                csr_mv(n, rp, ci, val, pc, pn);
+                for (int i = 0; i < n; ++i) pn[i] = 2.0 * pn[i] - pm[i];
+                double* r = pm;
+                pm = pc;
+                pc = pn;
+                pn = r;
+                aj = pc;
+            }
+            /* C[j][m] = (dxH a_j) . beta_m, summed in row order */
+            csr_mv(n, rp, ci, vdx, aj, tmp);
+            for (int m = 0; m < M; ++m) {
+                double s = 0.0;
+                const double* bm = beta + (size_t)m * n;
+                for (int i = 0; i < n; ++i) s = s + tmp[i] * bm[i];
+                row[(size_t)j * M + m] = s;
+            }
+        }
+    }
+    free(beta);
+    free(a0);
+    return 0;
+}
+
+void orc_cond_transform(int M, const double* Cin, double* out)
+{
+    double* g = (double*)malloc(sizeof(double) * (size_t)M * 3);
+    double *col_in = g + M, *col_out = col_in + M;
+    double* mid = (double*)malloc(sizeof(double) * (size_t)M * M);
+    for (int i = 0; i < M; ++i)
+        g[i] = ((M - i) * cos((M_PI * i) / M) + sin((M_PI * i) / M) / tan(M_PI / M)) / (sqrt(2.0) * M);
+    /* rows: REDFT01 along j of 2 g_i g_j C_ij */
+    for (int i = 0; i < M; ++i) {
+        for (int j = 0; j < M; ++j) col_in[j] = 2 * g[i] * g[j] * Cin[(size_t)i * M + j];
+        dct3(M, col_in, mid + (size_t)i * M);
+    }
+    /* columns: REDFT01 along i */
+    for (int l = 0; l < M; ++l) {
+        for (int i = 0; i < M; ++i) col_in[i] = mid[(size_t)i * M + l];
+        dct3(M, col_in, col_out);
+        for (int k = 0; k < M; ++k) out[(size_t)k * M + l] = col_out[k];
+    }
+    free(mid);
     free(g);
 }

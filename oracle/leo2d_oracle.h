@@ -118,11 +118,22 @@ typedef struct {
     int32_t* col;       /* [nnz] */
     double* val;        /* [nnz] real or [2 nnz] complex interleaved */
     int is_complex;
+    double* val_dx;     /* [nnz] x-velocity matrix dxH on the same pattern (real H only; NULL if complex) */
 } orc_csr;
 
 void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos /*[3n]*/);
 int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_csr* out);
 void orc_csr_free(orc_csr* h);
+
+/* KPM conductivity (Locality::computeCondKPM, locality.cpp:6348-6727; real H,
+ * target starts): C[t][n*M + m] = (dxH T_n(H) alpha_0) . (T_m(H) e_t) with
+ * alpha_0 = -dxH e_t.  The reference leaves T_{M-1} of the beta set
+ * unassigned; here it is the Chebyshev vector itself. */
+int orc_cond(int n, const int64_t* row_ptr, const int32_t* col, const double* val, const double* val_dx, int M,
+             int T, const int32_t* targets, double* C /*[T][M*M]*/);
+/* Param_tools::matrixResponseTransform (param_tools.cpp:838-905) for one
+ * M x M block: out = 2D REDFT01 of 2 g_i g_j C_ij, g_i Jackson / sqrt(2). */
+void orc_cond_transform(int M, const double* C, double* out);
 
 /* KPM ------------------------------------------------------------------- */
 #define ORC_START_TARGET 0      /* e_t, locality.cpp:5945 computeDosKPM */

@@ -110,6 +110,7 @@ class Csr(C.Structure):
         ("col", C.POINTER(C.c_int32)),
         ("val", C.POINTER(C.c_double)),
         ("is_complex", C.c_int),
+        ("val_dx", C.POINTER(C.c_double)),
     ]
 
 
@@ -153,6 +154,8 @@ def lib():
         L.orc_energies.argtypes = [C.c_int, C.c_double, C.c_double, vp]
         L.orc_dos.argtypes = [C.c_int, vp, C.c_double, C.c_double, vp]
         L.orc_dos_trace.argtypes = [C.c_int, C.c_int, vp, C.c_double, C.c_double, C.c_double, vp]
+        L.orc_cond.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp]
+        L.orc_cond_transform.argtypes = [C.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -515,7 +518,9 @@ class Geometry:
         lib().orc_pairs_copy(p, which, _p(ij), _p(sc), _p(t))
         return ij.reshape(-1, 2), sc.reshape(-1, 2), t
 
-    def assemble(self, job: Job):
+    def assemble(self, job: Job, with_dx: bool = False):
+        """CSR of H (rp, ci, val); with_dx also returns the x-velocity matrix
+        dxH on the same pattern (real H only, locality.cpp:4182,4299)."""
         csr = Csr()
         lib().orc_assemble(self.h, self.pairs(), C.byref(job), C.byref(csr))
         n, nnz = csr.n, csr.nnz
@@ -523,10 +528,13 @@ class Geometry:
         ci = np.ctypeslib.as_array(csr.col, shape=(max(nnz, 1),))[:nnz].copy()
         w = 2 if csr.is_complex else 1
         v = np.ctypeslib.as_array(csr.val, shape=(max(w * nnz, 1),))[: w * nnz].copy()
+        dx = None
         if csr.is_complex:
             v = v.view(np.complex128)
+        elif with_dx:
+            dx = np.ctypeslib.as_array(csr.val_dx, shape=(max(nnz, 1),))[:nnz].copy()
         lib().orc_csr_free(C.byref(csr))
-        return rp, ci, v
+        return (rp, ci, v, dx) if with_dx else (rp, ci, v)
 
     def job_positions(self, job: Job):
         out = np.zeros(3 * self.n, dtype=np.float64)
@@ -701,6 +709,31 @@ def kpm(rp, ci, val, M, R=None, targets=None, vecs=None, doubling_all=False, nth
     if rc != 0:
         raise ValueError("orc_kpm failed")
     return mu
+
+
+def cond(rp, ci, val, val_dx, M, targets):
+    """KPM conductivity moments C[t][n][m] (Locality::computeCondKPM,
+    locality.cpp:6348-6727; real H, target starts), T_{M-1} of the beta set
+    computed (the reference leaves it unassigned)."""
+    rp = np.ascontiguousarray(rp, dtype=np.int64)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    d = np.ascontiguousarray(val_dx, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    out = np.zeros((len(t), M, M))
+    if lib().orc_cond(len(rp) - 1, _p(rp), _p(ci), _p(v), _p(d), M, len(t), _p(t), _p(out)) != 0:
+        raise ValueError("orc_cond failed")
+    return out
+
+
+def cond_transform(Cm):
+    """Param_tools::matrixResponseTransform (param_tools.cpp:838-905) of one
+    M x M conductivity block."""
+    Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+    M = Cm.shape[0]
+    out = np.zeros((M, M))
+    lib().orc_cond_transform(M, _p(Cm), _p(out))
+    return out
 
 
 def random_phase(n, R, seed, r0=0):
