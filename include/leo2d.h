@@ -295,6 +295,41 @@ int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, con
 int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, int64_t n, int32_t blocks, float* mean,
                        float* spread);
 
+/* ---- KPM conductivity (OBSERVABLE_TYPE = 1) -------------------------------- */
+/* leo2d_assemble plus the x-velocity matrix dxH on H's pattern: slot values
+ * (x_row + image shift - x_col) t summed like H's (locality.cpp:4182, 4299).
+ * Real H only (the reference's conductivity path is real; a complex job
+ * returns LEO2D_ERR_ARG). */
+int leo2d_assemble_velocity(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz);
+/* Copy dxH's values (same pattern and order as leo2d_csr_copy). */
+int leo2d_velocity_copy(leo2d_handle* h, double* val_dx);
+
+typedef struct {
+    int32_t M;                 /* moments per direction: C is M x M per target */
+    int32_t R;                 /* targets */
+    const int32_t* targets;    /* [R] orbitals (start vectors e_t) */
+    int32_t block;             /* targets per device pass (0: min(R, 16)) */
+} leo2d_cond_params;
+
+typedef struct {
+    double recursion_ms;       /* both Chebyshev vector sets */
+    double gemm_ms;            /* the moment GEMMs */
+    int32_t block;             /* targets per device pass used */
+    int32_t reserved;
+} leo2d_cond_timing;
+
+/* Locality::computeCondKPM (locality.cpp:6348-6727) on the device matrices of
+ * leo2d_assemble_velocity: C[t][n][m] = (dxH T_n(H) alpha_0) . (T_m(H) e_t),
+ * alpha_0 = -dxH e_t.  Both vector sets are kept in HBM and C comes from one
+ * FP64 GEMM per target.  (The reference leaves T_{M-1} of the beta set
+ * unassigned; here it is the Chebyshev vector.) */
+int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C /*[R][M][M]*/);
+/* Param_tools::conductivityTransform / matrixResponseTransform
+ * (param_tools.cpp:770-905) for T blocks: out = 2D REDFT01 of 2 g_i g_j C_ij
+ * with the Jackson g_i / sqrt(2). */
+int leo2d_cond_transform(leo2d_handle* h, int32_t M, int32_t T, const double* C, double* out);
+int leo2d_cond_last_timing(leo2d_handle* h, leo2d_cond_timing* t);
+
 #ifdef __cplusplus
 }
 #endif

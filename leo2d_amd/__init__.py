@@ -53,6 +53,8 @@ EXPORTS = [
     "leo2d_batch_last_timing", "leo2d_mlp_features", "leo2d_mlp_create", "leo2d_mlp_destroy",
     "leo2d_mlp_last_error", "leo2d_mlp_num_params", "leo2d_mlp_get_params", "leo2d_mlp_set_params",
     "leo2d_mlp_forward", "leo2d_mlp_gradients", "leo2d_mlp_train", "leo2d_mlp_ensemble",
+    "leo2d_assemble_velocity", "leo2d_velocity_copy", "leo2d_cond_moments", "leo2d_cond_transform",
+    "leo2d_cond_last_timing",
 ]
 MLP_FEATURES = 64
 CONFIG_DIM = 6
@@ -100,6 +102,14 @@ class Timing(C.Structure):
         ("nnz", C.c_int64), ("n", C.c_int32), ("R", C.c_int32), ("M", C.c_int32), ("grid", C.c_int32),
         ("kernel", C.c_int32), ("reserved", C.c_int32),
     ]
+
+
+class CondParams(C.Structure):
+    _fields_ = [("M", C.c_int32), ("R", C.c_int32), ("targets", C.POINTER(C.c_int32)), ("block", C.c_int32)]
+
+
+class CondTiming(C.Structure):
+    _fields_ = [("recursion_ms", C.c_double), ("gemm_ms", C.c_double), ("block", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BatchParams(C.Structure):
@@ -165,6 +175,11 @@ def lib() -> C.CDLL:
         L.leo2d_csr_upload.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, C.c_int32]
         L.leo2d_csr_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
         L.leo2d_csr_copy.argtypes = [vp, vp, vp, vp]
+        L.leo2d_assemble_velocity.argtypes = [vp, C.POINTER(Job), C.c_int32, C.POINTER(C.c_int64)]
+        L.leo2d_velocity_copy.argtypes = [vp, vp]
+        L.leo2d_cond_moments.argtypes = [vp, C.POINTER(CondParams), vp]
+        L.leo2d_cond_transform.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
+        L.leo2d_cond_last_timing.argtypes = [vp, C.POINTER(CondTiming)]
         L.leo2d_spectral_bounds.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.leo2d_kpm_moments.argtypes = [vp, C.POINTER(KpmParams), vp]
         L.leo2d_last_timing.argtypes = [vp, C.POINTER(Timing)]
@@ -309,6 +324,42 @@ class Device:
         nnz = C.c_int64()
         self._check(lib().leo2d_assemble(self.h, C.byref(job), pattern, C.byref(nnz)))
         return nnz.value
+
+    def assemble_velocity(self, job: Job, pattern: int = PATTERN_REFERENCE) -> int:
+        """H and the x-velocity matrix dxH on its pattern (real H only)."""
+        nnz = C.c_int64()
+        self._check(lib().leo2d_assemble_velocity(self.h, C.byref(job), pattern, C.byref(nnz)))
+        return nnz.value
+
+    def velocity_copy(self):
+        _, nnz, _ = self.csr_info()
+        v = np.zeros(max(nnz, 1), dtype=np.float64)
+        self._check(lib().leo2d_velocity_copy(self.h, _p(v)))
+        return v[:nnz]
+
+    def cond_moments(self, M, targets, block=0):
+        """KPM conductivity moments C[t][n][m] (computeCondKPM)."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        p = CondParams()
+        p.M, p.R, p.block = M, len(t), block
+        p.targets = t.ctypes.data_as(C.POINTER(C.c_int32))
+        out = np.zeros((len(t), M, M))
+        self._check(lib().leo2d_cond_moments(self.h, C.byref(p), _p(out)))
+        return out
+
+    def cond_transform(self, Cm):
+        """2D Jackson / DCT-III transform of conductivity blocks [T][M][M]."""
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        if Cm.ndim == 2:
+            Cm = Cm[None]
+        out = np.zeros_like(Cm)
+        self._check(lib().leo2d_cond_transform(self.h, Cm.shape[1], Cm.shape[0], _p(Cm), _p(out)))
+        return out
+
+    def cond_timing(self) -> CondTiming:
+        t = CondTiming()
+        self._check(lib().leo2d_cond_last_timing(self.h, C.byref(t)))
+        return t
 
     def csr_upload(self, rp, ci, val):
         rp = np.ascontiguousarray(rp, dtype=np.int64)

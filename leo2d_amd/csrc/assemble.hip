@@ -320,7 +320,7 @@ template <bool CPX>
 __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, const double* i2pos, const int64_t* pair_rp,
                                                         const int32_t* cnt_intra, const int32_t* pcol,
                                                         const int32_t* psc, const int64_t* slot_rp, int32_t* scol,
-                                                        double* sval, uint8_t* skeep)
+                                                        double* sval, uint8_t* skeep, double* sdx)
 {
     const int k_i = blockIdx.x * kThreads + threadIdx.x;
     if (k_i >= g.n) return;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
     const int ih = g.i2g[4 * (size_t)k_i + 0], jh = g.i2g[4 * (size_t)k_i + 1], l0 = g.i2g[4 * (size_t)k_i + 2];
     const double xi = i2pos[3 * (size_t)k_i], yi = i2pos[3 * (size_t)k_i + 1], zi = i2pos[3 * (size_t)k_i + 2];
     const double resc = j.energy_rescale, eshift = j.energy_shift;
-    double ar = 0.0, ai = 0.0;
+    double ar = 0.0, ai = 0.0, adx = 0.0;   // adx: x-velocity slot (real H, sdx != nullptr)
     bool keep = false;
     for (int64_t q = q0; q < q1; ++q) {
         const bool intra = q < qi;
@@ -375,7 +375,10 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
             support = inter_support(mat, dx, dy, dz);
         }
         if (!CPX) {
-            if (t != 0) ar += t;
+            if (t != 0) {
+                ar += t;
+                adx += (x1 - x2) * t;   // (delta_x) t, locality.cpp:4182, 4299
+            }
         } else {
             // peierlsPhase (locality.cpp:7157-7161) and k.(r2 - r1) (locality.cpp:4896-4905)
             double phase = 3.14e-5 * j.B * (x2 - x1) * (0.5) * (y1 + y2);
@@ -397,11 +400,13 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
                 sval[2 * slot + 1] = ai;
             } else {
                 sval[slot] = ar;
+                if (sdx) sdx[slot] = adx;
             }
             skeep[slot] = keep ? 1 : 0;
             ++slot;
             ar = 0.0;
             ai = 0.0;
+            adx = 0.0;
             keep = false;
         }
     }
@@ -482,14 +487,16 @@ hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipSt
 
 hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
                       const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
-                      int32_t* scol, double* sval, uint8_t* skeep, hipStream_t s)
+                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, hipStream_t s)
 {
-    if (j.complex_matrix)
+    if (j.complex_matrix) {
+        if (sdx) return hipErrorInvalidValue;   // velocity slots are filled for a real H only
         hipLaunchKernelGGL(fill_kernel<true>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep);
-    else
+                           pcol, psc, slot_rp, scol, sval, skeep, nullptr);
+    } else {
         hipLaunchKernelGGL(fill_kernel<false>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep);
+                           pcol, psc, slot_rp, scol, sval, skeep, sdx);
+    }
     return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "assemble.h"
 #include "devbuf.h"
 #include "host.h"
@@ -98,6 +100,14 @@ struct leo2d_handle {
     int is_complex = 0;
     DevBuf rp, col, val, val32, i2pos;
     bool val32_valid = false;
+    // x-velocity matrix dxH on H's pattern and row order (leo2d_assemble_velocity)
+    DevBuf val_dx;
+    bool has_dx = false;
+    rocblas_handle blas = nullptr;   // created on the first conductivity call, bound to the stream
+    struct CondTiming {
+        double recursion_ms = 0.0, gemm_ms = 0.0;
+        int32_t block = 0;
+    } cond_timing;
     // spatial order used inside the recursion (perm[orig] = new row)
     bool csr_geom_order = false;       // device H rows are geometry orbitals
     std::vector<int32_t> perm_host;
@@ -286,6 +296,7 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->groups.valid = false;
     h->tiles.failed = false;
     h->chunks.valid = false;
+    h->has_dx = false;
 }
 
 // Row tiles for the vector-chunked step: consecutive rows, at most `tmax` rows
@@ -609,6 +620,24 @@ int batch_default_grid(int ntiles)
 
 }  // namespace
 
+// ---- KPM conductivity (observable type 1) -------------------------------------
+namespace {
+void blas_check(rocblas_status st, const char* what)
+{
+    if (st != rocblas_status_success)
+        throw std::runtime_error(std::string(what) + ": rocBLAS status " + std::to_string((int)st));
+}
+
+rocblas_handle blas_of(leo2d_handle* h)
+{
+    if (!h->blas) {
+        blas_check(rocblas_create_handle(&h->blas), "rocblas_create_handle");
+        blas_check(rocblas_set_stream(h->blas, h->stream), "rocblas_set_stream");
+    }
+    return h->blas;
+}
+}  // namespace
+
 extern "C" {
 
 // ---- deck -------------------------------------------------------------------
@@ -750,6 +779,7 @@ void leo2d_destroy(leo2d_handle* h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->blas) (void)rocblas_destroy_handle(h->blas);
     delete h;
 }
 
@@ -921,10 +951,12 @@ int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, d
     });
 }
 
-int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out)
+namespace {
+int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out, bool velocity)
 {
     if (!h || !job || (pattern != LEO2D_PATTERN_REFERENCE && pattern != LEO2D_PATTERN_PRUNED)) return LEO2D_ERR_ARG;
     return guarded(h, [&]() {
+        if (velocity && job->complex_matrix) throw ArgError("the velocity matrix is built for a real H only");
         if (!h->has_geom) throw StateError("leo2d_assemble before leo2d_set_geometry");
         if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
             throw ArgError("energy_rescale must be finite and non-zero");
@@ -940,17 +972,20 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
         if (!h->i2pos.p || h->i2pos.bytes < sizeof(double) * 3 * (size_t)n)
             dalloc(h->i2pos, sizeof(double) * 3 * (size_t)n, "job positions");
         check(leo2d::job_positions(h->gdev, jd, (double*)h->i2pos.p, s), "job positions");
-        DevBuf scol, sval, skeep;
+        DevBuf scol, sval, skeep, sdx;
         dalloc(scol, sizeof(int32_t) * (size_t)h->n_slots, "slot columns");
         dalloc(sval, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "slot values");
         dalloc(skeep, (size_t)h->n_slots + 1, "slot flags");
+        if (velocity) dalloc(sdx, sizeof(double) * (size_t)h->n_slots, "velocity slots");
         check(leo2d::fill_slots(h->gdev, jd, (const double*)h->i2pos.p, (const int64_t*)h->pair_rp.p,
                                 (const int32_t*)h->cnt_intra.p, (const int32_t*)h->pcol.p, (const int32_t*)h->psc.p,
-                                (const int64_t*)h->slot_rp.p, (int32_t*)scol.p, (double*)sval.p, (uint8_t*)skeep.p, s),
+                                (const int64_t*)h->slot_rp.p, (int32_t*)scol.p, (double*)sval.p, (uint8_t*)skeep.p,
+                                velocity ? (double*)sdx.p : nullptr, s),
               "hamiltonian fill");
         h->rp.release();
         h->col.release();
         h->val.release();
+        h->val_dx.release();
         if (pattern == LEO2D_PATTERN_REFERENCE) {
             dalloc(h->rp, sizeof(int64_t) * (n + 1), "row pointers");
             check(hipMemcpyAsync(h->rp.p, h->slot_rp.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s), "rp");
@@ -958,6 +993,7 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
             h->val = std::move(sval);
             skeep.release();
             set_csr_owned(h, n, h->n_slots, cpx);
+            if (velocity) h->val_dx = std::move(sdx);
         } else {
             DevBuf cnt;
             dalloc(cnt, sizeof(int64_t) * n, "prune counts");
@@ -974,16 +1010,49 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
                                        (const int32_t*)scol.p, (const double*)sval.p, (const int64_t*)h->rp.p,
                                        (int32_t*)h->col.p, (double*)h->val.p, s),
                   "prune compact");
+            if (velocity) {
+                // the same compaction for the velocity slots (columns rewritten identically)
+                dalloc(h->val_dx, sizeof(double) * (size_t)std::max<int64_t>(kept, 1), "velocity values");
+                check(leo2d::prune_compact(n, 0, (const int64_t*)h->slot_rp.p, (const uint8_t*)skeep.p,
+                                           (const int32_t*)scol.p, (const double*)sdx.p, (const int64_t*)h->rp.p,
+                                           (int32_t*)h->col.p, (double*)h->val_dx.p, s),
+                      "prune compact velocity");
+            }
             check(hipStreamSynchronize(s), "prune compact");
             cnt.release();
             scol.release();
             sval.release();
             skeep.release();
+            sdx.release();
             set_csr_owned(h, n, kept, cpx);
         }
         check(hipStreamSynchronize(s), "assemble");
         h->csr_geom_order = true;
+        h->has_dx = velocity;
         if (nnz_out) *nnz_out = h->nnz;
+        return LEO2D_OK;
+    });
+}
+}  // namespace
+
+int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out)
+{
+    return assemble_impl(h, job, pattern, nnz_out, false);
+}
+
+int leo2d_assemble_velocity(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out)
+{
+    return assemble_impl(h, job, pattern, nnz_out, true);
+}
+
+int leo2d_velocity_copy(leo2d_handle* h, double* val_dx)
+{
+    if (!h || !val_dx) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr || !h->has_dx) throw StateError("no velocity matrix: use leo2d_assemble_velocity");
+        if (h->nnz)
+            check(hipMemcpy(val_dx, h->val_dx.p, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost),
+                  "velocity copy");
         return LEO2D_OK;
     });
 }
@@ -1799,6 +1868,172 @@ int leo2d_batch_last_timing(const leo2d_handle* h, leo2d_batch_timing* t)
 {
     if (!h || !t) return LEO2D_ERR_ARG;
     *t = h->batch.timing;
+    return LEO2D_OK;
+}
+
+
+
+int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
+{
+    if (!h || !p || !C) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_csr || !h->has_dx) throw StateError("leo2d_cond_moments needs leo2d_assemble_velocity");
+        const int M = p->M, R = p->R, n = h->n;
+        if (M < 2) throw ArgError("M must be at least 2");
+        if (R <= 0 || !p->targets) throw ArgError("conductivity needs R >= 1 targets");
+        for (int r = 0; r < R; ++r)
+            if (p->targets[r] < 0 || p->targets[r] >= n) throw ArgError("target outside the matrix");
+        // targets per device block: a width the step kernels take, bounded so the two
+        // vector sets (2 x B x M x n doubles) stay within 96 GiB
+        int want = p->block > 0 ? p->block : std::min(R, 16);
+        int B = 1;
+        while (B * 2 <= std::min(want, 128)) B *= 2;
+        const double budget = 96.0 * (1ull << 30);
+        while (B > 1 && 2.0 * B * (double)M * n * sizeof(double) > budget) B /= 2;
+        if (2.0 * B * (double)M * n * sizeof(double) > budget) throw ArgError("M x n too large for one target's sets");
+        hipStream_t s = h->stream;
+        const int64_t* drp = (const int64_t*)h->rp.p;
+        const int32_t* dcol = (const int32_t*)h->col.p;
+        const size_t nb = (size_t)n * B;
+        DevBuf Bset, Dset, X0, X1, Y, part, dtg, Cdev;
+        dalloc(Bset, sizeof(double) * nb * M, "beta set");
+        dalloc(Dset, sizeof(double) * nb * M, "alpha set");
+        dalloc(X0, sizeof(double) * nb, "vector block");
+        dalloc(X1, sizeof(double) * nb, "vector block");
+        dalloc(Y, sizeof(double) * nb, "vector block");
+        const int rows_per_iter = std::max(1, 256 / B);
+        const int G = std::min(B >= 64 ? 1024 : 2048, std::max(8, (((n + rows_per_iter - 1) / rows_per_iter) + 7) / 8 * 8));
+        dalloc(part, sizeof(double) * 3 * (size_t)G * B, "partials");
+        dalloc(dtg, sizeof(int32_t) * B, "targets");
+        dalloc(Cdev, sizeof(double) * (size_t)B * M * M, "conductivity moments");
+        leo2d::StepArgs sa{};
+        sa.row_ptr = drp;
+        sa.col = dcol;
+        sa.n = n;
+        sa.R = B;
+        sa.part = (double*)part.p;
+        sa.grid = G;
+        sa.variant = B >= 64 ? 1 : 0;
+        // out = alpha * A cur - (beta ? prev : 0), A = H or dxH
+        auto step = [&](const void* A, void* cur, void* prev, void* out, double alpha, int beta) {
+            sa.val = A;
+            sa.cur = cur;
+            sa.prev = prev;
+            sa.out = out;
+            sa.alpha = alpha;
+            sa.beta = beta;
+            check(leo2d::launch_kpm_step(leo2d::MAT_REAL_F64, leo2d::VEC_REAL_F64, sa, s), "conductivity step");
+        };
+        auto scatter = [&](const void* blk, int m, DevBuf& set) {
+            check(leo2d::launch_cond_scatter((const double*)blk, n, B, m, M, (double*)set.p, s), "conductivity scatter");
+        };
+        const void* dH = h->val.p;
+        const void* dX = h->val_dx.p;
+        hipEvent_t e0 = event_at(h, 0), e1 = event_at(h, 1), e2 = event_at(h, 2);
+        float rec_ms = 0.f, gemm_ms = 0.f;
+        rocblas_handle bl = blas_of(h);
+        for (int b0 = 0; b0 < R; b0 += B) {
+            const int Bb = std::min(B, R - b0);
+            std::vector<int32_t> tg(B);
+            for (int b = 0; b < B; ++b) tg[b] = p->targets[b0 + std::min(b, Bb - 1)];
+            check(hipMemcpyAsync(dtg.p, tg.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s), "targets");
+            check(hipEventRecord(e0, s), "event");
+            // beta_m = T_m(H) e_t
+            check(leo2d::launch_init_targets(X0.p, leo2d::VEC_REAL_F64, n, B, (const int32_t*)dtg.p, s), "init");
+            scatter(X0.p, 0, Bset);
+            step(dH, X0.p, X0.p, X1.p, 1.0, 0);
+            scatter(X1.p, 1, Bset);
+            void *cur = X1.p, *prv = X0.p;
+            for (int m = 2; m < M; ++m) {
+                step(dH, cur, prv, prv, 2.0, 1);
+                scatter(prv, m, Bset);
+                std::swap(cur, prv);
+            }
+            // alpha_0 = -dxH e_t; a_0 = alpha_0, a_1 = H a_0, a_j = 2 H a_{j-1} - a_{j-2}; D_j = dxH a_j
+            check(leo2d::launch_init_targets(X0.p, leo2d::VEC_REAL_F64, n, B, (const int32_t*)dtg.p, s), "init");
+            step(dX, X0.p, X0.p, X1.p, -1.0, 0);
+            step(dX, X1.p, X1.p, Y.p, 1.0, 0);
+            scatter(Y.p, 0, Dset);
+            if (M > 1) {
+                step(dH, X1.p, X1.p, X0.p, 1.0, 0);
+                step(dX, X0.p, X0.p, Y.p, 1.0, 0);
+                scatter(Y.p, 1, Dset);
+            }
+            cur = X0.p;
+            prv = X1.p;
+            for (int j = 2; j < M; ++j) {
+                step(dH, cur, prv, prv, 2.0, 1);
+                step(dX, prv, prv, Y.p, 1.0, 0);
+                scatter(Y.p, j, Dset);
+                std::swap(cur, prv);
+            }
+            check(hipEventRecord(e1, s), "event");
+            // C_b[j][m] = sum_i D_b[j][i] Bset_b[m][i]: column-major (M x M) = Bset_b^T (M x n) Dset_b (n x M)
+            const double one = 1.0, zero = 0.0;
+            blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M, M, n, &one,
+                                                     (const double*)Bset.p, n, (rocblas_stride)n * M,
+                                                     (const double*)Dset.p, n, (rocblas_stride)n * M, &zero,
+                                                     (double*)Cdev.p, M, (rocblas_stride)M * M, Bb),
+                       "conductivity GEMM");
+            check(hipEventRecord(e2, s), "event");
+            check(hipMemcpyAsync(C + (size_t)b0 * M * M, Cdev.p, sizeof(double) * (size_t)Bb * M * M,
+                                 hipMemcpyDeviceToHost, s),
+                  "moments");
+            check(hipStreamSynchronize(s), "conductivity");
+            float a = 0.f, g = 0.f;
+            check(hipEventElapsedTime(&a, e0, e1), "event");
+            check(hipEventElapsedTime(&g, e1, e2), "event");
+            rec_ms += a;
+            gemm_ms += g;
+        }
+        for (size_t q = 0; q < (size_t)R * M * M; ++q)
+            if (!std::isfinite(C[q])) throw NanError("non-finite conductivity moment");
+        h->cond_timing.recursion_ms = rec_ms;
+        h->cond_timing.gemm_ms = gemm_ms;
+        h->cond_timing.block = B;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_cond_transform(leo2d_handle* h, int32_t M, int32_t T, const double* C, double* out)
+{
+    if (!h || !C || !out || M < 2 || T <= 0) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        hipStream_t s = h->stream;
+        const size_t mm = (size_t)M * M;
+        DevBuf Cd, X, P, Yb;
+        dalloc(Cd, sizeof(double) * mm * T, "conductivity moments");
+        dalloc(X, sizeof(double) * mm * T, "weighted moments");
+        dalloc(P, sizeof(double) * mm, "DCT-III matrix");
+        dalloc(Yb, sizeof(double) * mm * T, "half transform");
+        check(hipMemcpyAsync(Cd.p, C, sizeof(double) * mm * T, hipMemcpyHostToDevice, s), "upload");
+        check(leo2d::launch_cond_weight((const double*)Cd.p, M, T, (double*)X.p, s), "weights");
+        check(leo2d::launch_dct3_matrix(M, (double*)P.p, s), "DCT-III matrix");
+        // with the row-major X read as column-major X^T, Z = P X^T P^T (column-major)
+        // is the row-major 2D transform P X P^T
+        rocblas_handle bl = blas_of(h);
+        const double one = 1.0, zero = 0.0;
+        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
+                                                 (const double*)P.p, M, 0, (const double*)X.p, M, (rocblas_stride)mm,
+                                                 &zero, (double*)Yb.p, M, (rocblas_stride)mm, T),
+                   "transform GEMM");
+        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_none, rocblas_operation_transpose, M, M, M, &one,
+                                                 (const double*)Yb.p, M, (rocblas_stride)mm, (const double*)P.p, M, 0,
+                                                 &zero, (double*)X.p, M, (rocblas_stride)mm, T),
+                   "transform GEMM");
+        check(hipMemcpyAsync(out, X.p, sizeof(double) * mm * T, hipMemcpyDeviceToHost, s), "download");
+        check(hipStreamSynchronize(s), "transform");
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_cond_last_timing(leo2d_handle* h, leo2d_cond_timing* t)
+{
+    if (!h || !t) return LEO2D_ERR_ARG;
+    t->recursion_ms = h->cond_timing.recursion_ms;
+    t->gemm_ms = h->cond_timing.gemm_ms;
+    t->block = h->cond_timing.block;
+    t->reserved = 0;
     return LEO2D_OK;
 }
 

@@ -1,0 +1,89 @@
+// cond.hip -- KPM conductivity moments (observable type 1) on the device.
+//
+// Replaces Locality::computeCondKPM (locality.cpp:6348-6727; real H, target
+// starts) and Param_tools::matrixResponseTransform (param_tools.cpp:838-905).
+// The reference builds beta_m = T_m(H) e_t (m < M), walks a_n = T_n(H) alpha_0
+// with alpha_0 = -dxH e_t, and takes C[n][m] = (dxH a_n) . beta_m one dot
+// product at a time: M^2 vector dots per target.  Here both vector sets are
+// produced by the fused Chebyshev step kernels for a block of targets at once
+// and stored as [target][m][row]; C for every target is then one strided
+// batched FP64 GEMM (rocBLAS, MFMA), and the 2D Jackson/DCT-III transform is
+// two more GEMMs against the cosine matrix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace leo2d {
+namespace {
+
+constexpr int kCondThreads = 256;
+
+// set[(b * M + m) * n + i] = blk[i * B + b]: one Chebyshev vector block [n][B]
+// stored as vector m of every target's set
+__global__ __launch_bounds__(kCondThreads) void scatter_kernel(const double* __restrict__ blk, int n, int B, int m, int M,
+                                                               double* __restrict__ set)
+{
+    const size_t total = (size_t)n * B;
+    for (size_t t = blockIdx.x * (size_t)kCondThreads + threadIdx.x; t < total; t += (size_t)gridDim.x * kCondThreads) {
+        const size_t i = t / B;
+        const int b = (int)(t % B);
+        set[((size_t)b * M + m) * n + i] = blk[t];
+    }
+}
+
+// X[t][i][j] = 2 g_i g_j C[t][i][j] with the Jackson g_i / sqrt(2)
+// (param_tools.cpp:857-890)
+__global__ __launch_bounds__(kCondThreads) void cond_weight_kernel(const double* __restrict__ C, int M, int T,
+                                                                   double* __restrict__ X)
+{
+    const size_t total = (size_t)T * M * M;
+    const double pm = M_PI / M;
+    for (size_t t = blockIdx.x * (size_t)kCondThreads + threadIdx.x; t < total; t += (size_t)gridDim.x * kCondThreads) {
+        const int j = (int)(t % M);
+        const int i = (int)((t / M) % M);
+        const double gi = ((M - i) * cos(pm * i) + sin(pm * i) / tan(pm)) / (sqrt(2.0) * M);
+        const double gj = ((M - j) * cos(pm * j) + sin(pm * j) / tan(pm)) / (sqrt(2.0) * M);
+        X[t] = 2 * gi * gj * C[t];
+    }
+}
+
+// column-major P(k, i) = w_i cos(pi i (k + 1/2) / M), w_0 = 1, w_i = 2 (FFTW
+// REDFT01), the angle reduced exactly in integers: i (2k + 1) mod 4M
+__global__ __launch_bounds__(kCondThreads) void dct3_matrix_kernel(int M, double* __restrict__ P)
+{
+    const size_t total = (size_t)M * M;
+    const long long P4 = 4LL * M;
+    for (size_t t = blockIdx.x * (size_t)kCondThreads + threadIdx.x; t < total; t += (size_t)gridDim.x * kCondThreads) {
+        const int k = (int)(t % M);
+        const int i = (int)(t / M);
+        const long long m = ((long long)i * (2LL * k + 1)) % P4;
+        const double c = cos(M_PI * (double)m / (2.0 * M));
+        P[t] = i == 0 ? 1.0 : 2.0 * c;
+    }
+}
+
+dim3 grid_for(size_t work) { return dim3((unsigned)std::min<size_t>(4096, (work + kCondThreads - 1) / kCondThreads)); }
+
+}  // namespace
+
+hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, double* set, hipStream_t s)
+{
+    hipLaunchKernelGGL(scatter_kernel, grid_for((size_t)n * B), dim3(kCondThreads), 0, s, blk, n, B, m, M, set);
+    return hipGetLastError();
+}
+
+hipError_t launch_cond_weight(const double* C, int M, int T, double* X, hipStream_t s)
+{
+    hipLaunchKernelGGL(cond_weight_kernel, grid_for((size_t)T * M * M), dim3(kCondThreads), 0, s, C, M, T, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_dct3_matrix(int M, double* P, hipStream_t s)
+{
+    hipLaunchKernelGGL(dct3_matrix_kernel, grid_for((size_t)M * M), dim3(kCondThreads), 0, s, M, P);
+    return hipGetLastError();
+}
+
+}  // namespace leo2d

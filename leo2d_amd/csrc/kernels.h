@@ -141,6 +141,12 @@ hipError_t launch_perm_fill(const int64_t* rp, const int32_t* col, const void* v
                             const int32_t* perm, int n, const int64_t* prp, int32_t* pcol, void* pval, hipStream_t s);
 hipError_t scan_exclusive_i64(const int64_t* cnt, int64_t* out, int n, hipStream_t s);
 
+// KPM conductivity (cond.hip): blk [n][B] -> set[b][m][n]; X = 2 g_i g_j C
+// (Jackson g / sqrt 2); column-major DCT-III matrix P(k, i) = w_i cos(pi i (k+1/2)/M)
+hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, double* set, hipStream_t s);
+hipError_t launch_cond_weight(const double* C, int M, int T, double* X, hipStream_t s);
+hipError_t launch_dct3_matrix(int M, double* P, hipStream_t s);
+
 // Jackson-kernel DoS: rho[t][k] = DCT-III(g * mu[t]) / (pi sqrt(a^2 - (E_k - b)^2)) * scale
 hipError_t launch_dos(const double* mu, int T, int M, double a, double b, double scale, double* E, double* rho,
                       hipStream_t s);
