@@ -1494,7 +1494,9 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
         geo.g = leo2d::build_geometry(geo.deck);
         const leo2d::Params& o = geo.deck.opts;
         if (o.geti("diagonalize") != 0) throw leo2d::DeckError("DIAGONALIZE = 1 is not on this path");
-        if (o.geti("observable_type") != 0) throw leo2d::DeckError("OBSERVABLE_TYPE = COND is not on this path");
+        const bool cond = o.geti("observable_type") == 1;
+        if (cond && o.geti("cond_poly_par") != 0) throw leo2d::DeckError("COND_POLY_PAR = 1 is not on this path");
+        if (cond && o.geti("kpm_trace") == 1) throw leo2d::DeckError("KPM_TRACE with OBSERVABLE_TYPE = COND is not on this path");
         const std::string prefix = out_prefix ? std::string(out_prefix) : o.s.at("job_name");
         std::vector<leo2d_job> jobs = leo2d::build_jobs(geo.deck, geo.g);
         int rc = leo2d_set_geometry(h, &geo);
@@ -1518,6 +1520,31 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
         for (size_t q = 0; q < jobs.size(); ++q) {
             const leo2d_job& job = jobs[q];
             int64_t nnz = 0;
+            if (cond) {
+                // Locality::computeCondKPM + conductivityTransform (locality.cpp:3133-3147); the
+                // reference's conductivity path takes a real H only (locality.cpp:6470)
+                if (job.complex_matrix) throw leo2d::DeckError("OBSERVABLE_TYPE = COND needs a real Hamiltonian");
+                rc = leo2d_assemble_velocity(h, &job, pattern, &nnz);
+                if (rc != LEO2D_OK) return rc;
+                const int T = job.num_targets;
+                std::vector<double> cm((size_t)T * M * M);
+                if (T > 0) {
+                    leo2d_cond_params cp{};
+                    cp.M = M;
+                    cp.R = T;
+                    cp.targets = job.targets;
+                    rc = leo2d_cond_moments(h, &cp, cm.data());
+                    if (rc != LEO2D_OK) return rc;
+                    if (o.geti("cond_transform") == 1) {
+                        std::vector<double> tr(cm.size());
+                        rc = leo2d_cond_transform(h, M, T, cm.data(), tr.data());
+                        if (rc != LEO2D_OK) return rc;
+                        cm.swap(tr);
+                    }
+                }
+                rho_all.push_back(cm);
+                continue;
+            }
             rc = leo2d_assemble(h, &job, pattern, &nnz);
             if (rc != LEO2D_OK) return rc;
             leo2d_kpm_params kp{};

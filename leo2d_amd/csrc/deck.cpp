@@ -20,6 +20,8 @@ Params::Params()
     s["job_name"] = "LEO2D_JOB";
     i["solver_type"] = 0;
     i["observable_type"] = 0;
+    i["cond_transform"] = 1;     // job_params.cpp:122
+    i["cond_poly_par"] = 0;      // job_params.cpp:47
     i["solver_space"] = 0;
     i["nShifts"] = 1;
     i["num_shift_sheets"] = 1;
@@ -206,6 +208,9 @@ Deck parse_deck(const std::string& text)
                 if (v == "DOS") o.i["observable_type"] = 0;
                 else if (v == "COND") o.i["observable_type"] = 1;
             } else if (key == "DOS_TRANSFORM") o.i["dos_transform"] = std::atoi(val().c_str());
+            else if (key == "COND_TRANSFORM") o.i["cond_transform"] = std::atoi(val().c_str());
+            else if (key == "COND_POLY_PAR") o.i["cond_poly_par"] = std::atoi(val().c_str());
+            else if (key == "COND_POLY_PAR_SCALE") o.i["cond_poly_par_scale"] = std::atoi(val().c_str());
             else if (key == "SOLVER_SPACE") {
                 const std::string v = val();
                 if (!v.empty() && v[0] == 'R') o.i["solver_space"] = 0;

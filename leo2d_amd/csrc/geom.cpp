@@ -366,6 +366,26 @@ std::string cheb_text(const Deck& d, const std::vector<leo2d_job>& jobs, const s
                 out << "ELEC_ON = " << o.geti("elecOn") << ", E = " << fx(o.getd("E")) << " \n";
             }
         }
+        if (o.geti("observable_type") == 1) {
+            // conductivity (param_tools.cpp:287-317): VERBOSE_SAVE = 1 writes only the
+            // header (its matrix block is commented out in the reference); 0 writes the
+            // energies, a blank line, then per target M rows of M values and a blank line
+            if (verbose == 0) {
+                if (j.job_id == 1) {
+                    for (int i = 0; i < M; ++i) out << fx(b + a * std::cos((i * 1.0 + 0.5) * M_PI / M)) << " ";
+                    out << "\n";
+                    out << "\n";
+                }
+                for (int t = 0; t < j.num_targets; ++t) {
+                    for (int i = 0; i < M; ++i) {
+                        for (int k = 0; k < M; ++k) out << fx(r[((size_t)t * M + i) * M + k]) << " ";
+                        out << "\n";
+                    }
+                    out << "\n";
+                }
+            }
+            continue;
+        }
         if (verbose == 1) {
             if (j.job_id == 1) {
                 out << "E: ";

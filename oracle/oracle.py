@@ -172,6 +172,7 @@ def default_opts() -> dict:
     """Job_params defaults that affect this path (job_params.cpp:18-156)."""
     return {
         "job_name": "LEO2D_JOB", "solver_type": 0, "observable_type": 0, "solver_space": 0,
+        "cond_transform": 1, "cond_poly_par": 0,
         "nShifts": 1, "num_shift_sheets": 1, "num_lc_points": 2,
         "lc_points": [[0.0, 0.0], [1.0, 1.0]], "num_sheets": 1, "boundary_condition": 0,
         "sc_search_size": 1, "global_shifts_on": 0, "strain_type": 0, "uniform_strain": 0,
@@ -335,6 +336,12 @@ def parse_deck(text: str) -> Deck:
                     opts["observable_type"] = 1
             elif tok == "DOS_TRANSFORM":
                 nxt(); opts["dos_transform"] = atoi(nxt())
+            elif tok == "COND_TRANSFORM":
+                nxt(); opts["cond_transform"] = atoi(nxt())
+            elif tok == "COND_POLY_PAR":
+                nxt(); opts["cond_poly_par"] = atoi(nxt())
+            elif tok == "COND_POLY_PAR_SCALE":
+                nxt(); opts["cond_poly_par_scale"] = atoi(nxt())
             elif tok == "SOLVER_SPACE":
                 nxt(); v = nxt()
                 if v[:1] == "R":
@@ -808,6 +815,17 @@ def cheb_text(deck: Deck, results: list) -> str:
             if o["magOn"] == 1 or o["elecOn"] == 1:
                 out.append("MAG_ON  = %d, B = %s \n" % (o["magOn"], fmt_fixed(o["B"])))
                 out.append("ELEC_ON = %d, E = %s \n" % (o["elecOn"], fmt_fixed(o["E"])))
+        if o["observable_type"] == 1:
+            # conductivity (param_tools.cpp:287-317): only VERBOSE_SAVE = 0 writes the
+            # matrices (the verbose block is commented out in the reference)
+            if o["verbose_save"] == 0:
+                if job["jobID"] == 1:
+                    out.append("".join(fmt_fixed(e) + " " for e in E) + "\n\n")
+                for t in range(len(tl)):
+                    for row in rho[t]:
+                        out.append("".join(fmt_fixed(x) + " " for x in row) + "\n")
+                    out.append("\n")
+            continue
         if o["verbose_save"] == 1:
             if job["jobID"] == 1:
                 out.append("E: " + "".join(fmt_fixed(e) + " " for e in E) + "\n")
@@ -919,7 +937,22 @@ def run_deck(text: str, pruned: int = 0):
         # one generator for all jobs, vectors drawn in job order (locality.cpp:2275-2310)
         gen, dist = MinstdRand0(), NormalDistribution()
         S = trace_samples(o, geom.n)
+    cond_deck = o["observable_type"] == 1
+    if cond_deck and (o["cond_poly_par"] != 0 or trace):
+        raise DeckError("COND_POLY_PAR and KPM_TRACE are not on the conductivity path")
     for job in jobs:
+        if cond_deck:
+            # computeCondKPM + conductivityTransform (locality.cpp:3133-3147); real H only
+            jb = make_job(deck, job, pruned)
+            if jb.complex_matrix:
+                raise DeckError("OBSERVABLE_TYPE = COND needs a real Hamiltonian")
+            rp, ci, v, dx = geom.assemble(jb, with_dx=True)
+            tl = job["target_list"]
+            Cm = cond(rp, ci, v, dx, o["poly_order"], tl) if len(tl) else np.zeros((0, o["poly_order"], o["poly_order"]))
+            if o["cond_transform"] == 1:
+                Cm = np.array([cond_transform(c) for c in Cm])
+            results.append((job, Cm))
+            continue
         rp, ci, v = geom.assemble(make_job(deck, job, pruned))
         if trace:
             vecs = np.array([[dist(gen) for _ in range(geom.n)] for _ in range(S)])

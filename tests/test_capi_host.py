@@ -31,7 +31,8 @@ def test_exports_every_declared_symbol():
 
 
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
-             "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in"]
+             "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
+             "tblg_cond_flake.in", "tblg_cond_raw.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -127,3 +128,19 @@ def test_mlp_features_match_oracle():
     assert np.array_equal(L.mlp_features(cfg), O.features(cfg))
     with pytest.raises(L.Leo2dError):
         L.mlp_features(np.array([[0.1, 0.2, 0, 0, 0, 0.0]]))    # twist must be positive
+
+
+def test_conductivity_deck_keys(oracle_mod):
+    """OBSERVABLE_TYPE takes DOS / COND; COND_TRANSFORM defaults to 1 and
+    COND_POLY_PAR to 0 (job_params.cpp:47,122; main.cpp:455-486), in the
+    library's parser and the oracle's alike."""
+    text = deck_text("tblg_cond_raw.in")
+    d = L.Deck(text)
+    od = oracle_mod.parse_deck(text)
+    assert d.get_int("observable_type") == od.opts["observable_type"] == 1
+    assert d.get_int("cond_transform") == od.opts["cond_transform"] == 0
+    base = text.replace("COND_TRANSFORM = 0\n", "")
+    assert L.Deck(base).get_int("cond_transform") == oracle_mod.parse_deck(base).opts["cond_transform"] == 1
+    assert L.Deck(base).get_int("cond_poly_par") == 0
+    dos = text.replace("OBSERVABLE_TYPE = COND", "OBSERVABLE_TYPE = DOS")
+    assert L.Deck(dos).get_int("observable_type") == 0

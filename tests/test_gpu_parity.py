@@ -184,9 +184,14 @@ def _cheb_numbers(text):
     return [float(x) for x in re.findall(r"-?\d+\.\d+", text)], re.sub(r"-?\d+\.\d+", "#", text)
 
 
-@pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in"])
+@pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in",
+                                  "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose"])
 def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
+    name, _, variant = name.partition(":")
     text = deck_text(name)
+    if variant == "verbose":
+        # conductivity with VERBOSE_SAVE = 1: the reference writes the job headers only
+        text = text.replace("VERBOSE_SAVE = 0", "VERBOSE_SAVE = 1")
     prefix = str(tmp_path / "out")
     dev.run_deck(text, prefix)
     got = open(prefix + ".cheb").read()
