@@ -1933,6 +1933,13 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
         dalloc(part, sizeof(double) * 3 * (size_t)G * B, "partials");
         dalloc(dtg, sizeof(int32_t) * B, "targets");
         dalloc(Cdev, sizeof(double) * (size_t)B * M * M, "conductivity moments");
+        // split the n-long GEMM reduction into S row chunks so that the M x M outputs of a pass
+        // give the GPU enough tiles (about 2048 of 128 x 128); partial products summed in order
+        const int tiles = std::max(1, ((M + 127) / 128) * ((M + 127) / 128) * B);
+        const int S = std::max(1, std::min({64, 2048 / tiles, n / 1024}));
+        const int64_t L = n / S;                 // rows per chunk; the last chunk takes the rest
+        DevBuf Cpart;
+        if (S > 1) dalloc(Cpart, sizeof(double) * (size_t)B * S * M * M, "conductivity partials");
         leo2d::StepArgs sa{};
         sa.row_ptr = drp;
         sa.col = dcol;
@@ -1997,11 +2004,33 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
             check(hipEventRecord(e1, s), "event");
             // C_b[j][m] = sum_i D_b[j][i] Bset_b[m][i]: column-major (M x M) = Bset_b^T (M x n) Dset_b (n x M)
             const double one = 1.0, zero = 0.0;
-            blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M, M, n, &one,
-                                                     (const double*)Bset.p, n, (rocblas_stride)n * M,
-                                                     (const double*)Dset.p, n, (rocblas_stride)n * M, &zero,
-                                                     (double*)Cdev.p, M, (rocblas_stride)M * M, Bb),
-                       "conductivity GEMM");
+            const size_t mm = (size_t)M * M;
+            if (S == 1) {
+                blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M, M, n,
+                                                         &one, (const double*)Bset.p, n, (rocblas_stride)n * M,
+                                                         (const double*)Dset.p, n, (rocblas_stride)n * M, &zero,
+                                                         (double*)Cdev.p, M, (rocblas_stride)mm, Bb),
+                           "conductivity GEMM");
+            } else {
+                for (int b = 0; b < Bb; ++b) {
+                    const double* Bb_ = (const double*)Bset.p + (size_t)b * n * M;
+                    const double* Db_ = (const double*)Dset.p + (size_t)b * n * M;
+                    double* Cp = (double*)Cpart.p + (size_t)b * S * mm;
+                    // chunks 0 .. S-2 of L rows, then the last chunk of n - (S-1) L rows
+                    if (S > 1)
+                        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M,
+                                                                 M, (rocblas_int)L, &one, Bb_, n, (rocblas_stride)L, Db_,
+                                                                 n, (rocblas_stride)L, &zero, Cp, M, (rocblas_stride)mm,
+                                                                 S - 1),
+                                   "conductivity GEMM");
+                    const int64_t r0 = (int64_t)(S - 1) * L;
+                    blas_check(rocblas_dgemm(bl, rocblas_operation_transpose, rocblas_operation_none, M, M,
+                                             (rocblas_int)(n - r0), &one, Bb_ + r0, n, Db_ + r0, n, &zero,
+                                             Cp + (size_t)(S - 1) * mm, M),
+                               "conductivity GEMM");
+                }
+                check(leo2d::launch_cond_chunk_sum((const double*)Cpart.p, S, mm, Bb, (double*)Cdev.p, s), "chunk sum");
+            }
             check(hipEventRecord(e2, s), "event");
             check(hipMemcpyAsync(C + (size_t)b0 * M * M, Cdev.p, sizeof(double) * (size_t)Bb * M * M,
                                  hipMemcpyDeviceToHost, s),

@@ -64,6 +64,20 @@ __global__ __launch_bounds__(kCondThreads) void dct3_matrix_kernel(int M, double
     }
 }
 
+// C[b][q] = sum over the S row chunks c of part[b][c][q], in order c = 0, 1, ...
+__global__ __launch_bounds__(kCondThreads) void chunk_sum_kernel(const double* __restrict__ part, int S, size_t mm,
+                                                                 int B, double* __restrict__ C)
+{
+    const size_t total = (size_t)B * mm;
+    for (size_t t = blockIdx.x * (size_t)kCondThreads + threadIdx.x; t < total; t += (size_t)gridDim.x * kCondThreads) {
+        const size_t b = t / mm, q = t % mm;
+        const double* p = part + b * S * mm + q;
+        double s = 0.0;
+        for (int c = 0; c < S; ++c) s += p[(size_t)c * mm];
+        C[t] = s;
+    }
+}
+
 dim3 grid_for(size_t work) { return dim3((unsigned)std::min<size_t>(4096, (work + kCondThreads - 1) / kCondThreads)); }
 
 }  // namespace
@@ -71,6 +85,12 @@ dim3 grid_for(size_t work) { return dim3((unsigned)std::min<size_t>(4096, (work 
 hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, double* set, hipStream_t s)
 {
     hipLaunchKernelGGL(scatter_kernel, grid_for((size_t)n * B), dim3(kCondThreads), 0, s, blk, n, B, m, M, set);
+    return hipGetLastError();
+}
+
+hipError_t launch_cond_chunk_sum(const double* part, int S, size_t mm, int B, double* C, hipStream_t s)
+{
+    hipLaunchKernelGGL(chunk_sum_kernel, grid_for((size_t)B * mm), dim3(kCondThreads), 0, s, part, S, mm, B, C);
     return hipGetLastError();
 }
 

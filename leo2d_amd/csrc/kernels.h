@@ -145,6 +145,8 @@ hipError_t scan_exclusive_i64(const int64_t* cnt, int64_t* out, int n, hipStream
 // (Jackson g / sqrt 2); column-major DCT-III matrix P(k, i) = w_i cos(pi i (k+1/2)/M)
 hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, double* set, hipStream_t s);
 hipError_t launch_cond_weight(const double* C, int M, int T, double* X, hipStream_t s);
+// C[b] = sum_c part[b][c] (c = 0 .. S-1, fixed order), blocks of mm values
+hipError_t launch_cond_chunk_sum(const double* part, int S, size_t mm, int B, double* C, hipStream_t s);
 hipError_t launch_dct3_matrix(int M, double* P, hipStream_t s);
 
 // Jackson-kernel DoS: rho[t][k] = DCT-III(g * mu[t]) / (pi sqrt(a^2 - (E_k - b)^2)) * scale

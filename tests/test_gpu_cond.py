@@ -76,6 +76,19 @@ def test_cond_moments_match_oracle(dev, oracle_mod, block):
     assert dev.cond_timing().block == want_block
 
 
+def test_cond_split_reduction(dev, oracle_mod):
+    """A flake large enough (n > 3000) that the moment GEMM is split into row
+    chunks whose partial products are summed afterwards."""
+    text = W.tblg_flake_deck(2.0, 40, M=16)
+    (orp, oci, ov, odx), tl = _oracle_setup(oracle_mod, text, 1)
+    geom, _ = _device_setup(dev, text, L.PATTERN_PRUNED)
+    assert geom.n > 3000
+    M = 20
+    want = oracle_mod.cond(orp, oci, ov, odx, M, list(tl))
+    got = dev.cond_moments(M, list(tl))
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
 def test_cond_periodic_cell(dev, oracle_mod):
     text = W.tblg_cell_deck(3, 2, M=16)
     (orp, oci, ov, odx), tl = _oracle_setup(oracle_mod, text, 1)
