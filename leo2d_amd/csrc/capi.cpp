@@ -1587,18 +1587,41 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
                 if (o.geti("verbose_save") == 1) rho = mu;   // verbose save prints the sample moments
             }
             rho_all.push_back(rho);
-            if (o.geti("matrix_save") > 0 && q + 1 == jobs.size()) {
-                int32_t nn;
-                int64_t z;
-                int32_t cpx;
-                leo2d_csr_info(h, &nn, &z, &cpx);
-                std::vector<int64_t> rp(nn + 1);
-                std::vector<int32_t> ci((size_t)z);
-                std::vector<double> vv((size_t)z * (cpx ? 2 : 1));
-                rc = leo2d_csr_copy(h, rp.data(), ci.data(), vv.data());
-                if (rc != LEO2D_OK) return rc;
-                std::ofstream f(prefix + "_matrix.dat");
-                f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vv.data(), cpx);
+        }
+        // files of the last job, as every job of the reference writes them in turn
+        // (locality.cpp:4360-4477): _matrix.dat, and with MATRIX_POS_SAVE the positions
+        if (!jobs.empty() && o.geti("matrix_save") > 0) {
+            int32_t nn;
+            int64_t z;
+            int32_t cpx;
+            leo2d_csr_info(h, &nn, &z, &cpx);
+            std::vector<int64_t> rp(nn + 1);
+            std::vector<int32_t> ci((size_t)z);
+            std::vector<double> vv((size_t)z * (cpx ? 2 : 1));
+            rc = leo2d_csr_copy(h, rp.data(), ci.data(), vv.data());
+            if (rc != LEO2D_OK) return rc;
+            std::ofstream f(prefix + "_matrix.dat");
+            f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vv.data(), cpx);
+        }
+        if (!jobs.empty() && o.geti("matrix_pos_save") > 0) {
+            const int nn = geo.g.n;
+            std::vector<double> pos(3 * (size_t)nn);
+            check(hipMemcpy(pos.data(), h->i2pos.p, sizeof(double) * pos.size(), hipMemcpyDeviceToHost), "positions");
+            std::vector<int32_t> i2g(geo.g.i2g.begin(), geo.g.i2g.end());
+            {
+                std::ofstream f(prefix + "_pos.dat");
+                f << leo2d::pos_dat_text(nn, pos.data(), i2g.data());
+            }
+            if (o.geti("matrix_pos_save") > 1) {
+                const int64_t counts[2] = {h->n_intra, h->n_pairs - h->n_intra};
+                const char* ext[2] = {"_intra_pos.dat", "_inter_pos.dat"};
+                for (int which = 0; which < 2; ++which) {
+                    std::vector<int32_t> ij(2 * (size_t)std::max<int64_t>(counts[which], 1));
+                    rc = leo2d_pairs_copy(h, which, ij.data(), nullptr, nullptr);
+                    if (rc != LEO2D_OK) return rc;
+                    std::ofstream f(prefix + ext[which]);
+                    f << leo2d::pair_pos_text(counts[which], ij.data(), pos.data());
+                }
             }
         }
         std::ofstream f(prefix + ".cheb");

@@ -433,4 +433,33 @@ std::string matrix_dat_text(int n, const int64_t* rp, const int32_t* col, const 
     return out;
 }
 
+std::string pos_dat_text(int n, const double* i2pos, const int32_t* i2g)
+{
+    // _pos.dat (locality.cpp:4425-4443): std::fixed, precision 12, "x    y    z    sheet    orbital"
+    std::string out;
+    char b[256];
+    for (int i = 0; i < n; ++i) {
+        std::snprintf(b, sizeof b, "%.12f    %.12f    %.12f    %d    %d\n", i2pos[3 * (size_t)i], i2pos[3 * (size_t)i + 1],
+                      i2pos[3 * (size_t)i + 2], i2g[4 * (size_t)i + 3], i2g[4 * (size_t)i + 2]);
+        out += b;
+    }
+    return out;
+}
+
+std::string pair_pos_text(int64_t np, const int32_t* ij, const double* i2pos)
+{
+    // _intra_pos.dat / _inter_pos.dat (locality.cpp:4446-4475): default stream format (%g),
+    // "x1, y1, z1, x2, y2, z2, i, j" for every pair of the list
+    std::string out;
+    char b[320];
+    for (int64_t q = 0; q < np; ++q) {
+        const double* p1 = i2pos + 3 * (size_t)ij[2 * q];
+        const double* p2 = i2pos + 3 * (size_t)ij[2 * q + 1];
+        std::snprintf(b, sizeof b, "%g, %g, %g, %g, %g, %g, %d, %d\n", p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], ij[2 * q],
+                      ij[2 * q + 1]);
+        out += b;
+    }
+    return out;
+}
+
 }  // namespace leo2d

@@ -861,6 +861,21 @@ def matrix_dat_text(rp, ci, val) -> str:
     return "".join(out)
 
 
+def pos_dat_text(i2pos, i2g) -> str:
+    """_pos.dat (locality.cpp:4425-4443): fixed, 12 decimals, 'x    y    z    sheet    orbital'."""
+    p = np.asarray(i2pos).reshape(-1, 3)
+    g = np.asarray(i2g).reshape(-1, 4)
+    return "".join("%.12f    %.12f    %.12f    %d    %d\n" % (p[i, 0], p[i, 1], p[i, 2], g[i, 3], g[i, 2])
+                   for i in range(len(p)))
+
+
+def pair_pos_text(ij, i2pos) -> str:
+    """_intra_pos.dat / _inter_pos.dat (locality.cpp:4446-4475): default stream
+    format, 'x1, y1, z1, x2, y2, z2, i, j' per pair."""
+    p = np.asarray(i2pos).reshape(-1, 3)
+    return "".join("%g, %g, %g, %g, %g, %g, %d, %d\n" % (*p[a], *p[b], a, b) for a, b in np.asarray(ij).reshape(-1, 2))
+
+
 # --------------------------------------------------------------------------
 # KPM trace sampling: std::default_random_engine + std::normal_distribution as
 # locality.cpp:2275-2276, 2301-2303 use them (libstdc++ 11, the image's

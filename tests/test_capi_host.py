@@ -32,7 +32,7 @@ def test_exports_every_declared_symbol():
 
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
-             "tblg_cond_flake.in", "tblg_cond_raw.in"]
+             "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)

@@ -214,6 +214,23 @@ def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
             assert np.allclose(ga[2:], wa[2:], rtol=1e-5, atol=1e-6)
 
 
+def test_run_deck_position_files_match_oracle(dev, oracle_mod, tmp_path):
+    """MATRIX_POS_SAVE = 2 writes the last job's positions (_pos.dat) and the
+    positions of every intra- and interlayer pair (_intra_pos.dat,
+    _inter_pos.dat); the text equals the oracle's writers byte for byte."""
+    text = deck_text("tblg_pos_save.in")
+    prefix = str(tmp_path / "out")
+    dev.run_deck(text, prefix)
+    od, og, jobs, _ = oracle_mod.run_deck(text)
+    assert len(jobs) == 4
+    pos = og.job_positions(oracle_mod.make_job(od, jobs[-1]))
+    assert open(prefix + "_pos.dat").read() == oracle_mod.pos_dat_text(pos, og.index_to_grid)
+    for which, ext in ((0, "_intra_pos.dat"), (1, "_inter_pos.dat")):
+        ij, _, _ = og.pair_arrays(which)
+        assert open(prefix + ext).read() == oracle_mod.pair_pos_text(ij, pos), ext
+    assert os.path.exists(prefix + "_matrix.dat")
+
+
 def test_spectrum_and_nan_errors(dev):
     n = 50
     rp = np.arange(n + 1, dtype=np.int64)
