@@ -1945,15 +1945,11 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
         const int64_t* drp = (const int64_t*)h->rp.p;
         const int32_t* dcol = (const int32_t*)h->col.p;
         const size_t nb = (size_t)n * B;
-        DevBuf Bset, Dset, X0, X1, Y, part, dtg, Cdev;
+        DevBuf Bset, Dset, X0, X1, dtg, Cdev;
         dalloc(Bset, sizeof(double) * nb * M, "beta set");
         dalloc(Dset, sizeof(double) * nb * M, "alpha set");
         dalloc(X0, sizeof(double) * nb, "vector block");
         dalloc(X1, sizeof(double) * nb, "vector block");
-        dalloc(Y, sizeof(double) * nb, "vector block");
-        const int rows_per_iter = std::max(1, 256 / B);
-        const int G = std::min(B >= 64 ? 1024 : 2048, std::max(8, (((n + rows_per_iter - 1) / rows_per_iter) + 7) / 8 * 8));
-        dalloc(part, sizeof(double) * 3 * (size_t)G * B, "partials");
         dalloc(dtg, sizeof(int32_t) * B, "targets");
         dalloc(Cdev, sizeof(double) * (size_t)B * M * M, "conductivity moments");
         // split the n-long GEMM reduction into S row chunks so that the M x M outputs of a pass
@@ -1963,24 +1959,6 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
         const int64_t L = n / S;                 // rows per chunk; the last chunk takes the rest
         DevBuf Cpart;
         if (S > 1) dalloc(Cpart, sizeof(double) * (size_t)B * S * M * M, "conductivity partials");
-        leo2d::StepArgs sa{};
-        sa.row_ptr = drp;
-        sa.col = dcol;
-        sa.n = n;
-        sa.R = B;
-        sa.part = (double*)part.p;
-        sa.grid = G;
-        sa.variant = B >= 64 ? 1 : 0;
-        // out = alpha * A cur - (beta ? prev : 0), A = H or dxH
-        auto step = [&](const void* A, void* cur, void* prev, void* out, double alpha, int beta) {
-            sa.val = A;
-            sa.cur = cur;
-            sa.prev = prev;
-            sa.out = out;
-            sa.alpha = alpha;
-            sa.beta = beta;
-            check(leo2d::launch_kpm_step(leo2d::MAT_REAL_F64, leo2d::VEC_REAL_F64, sa, s), "conductivity step");
-        };
         auto scatter = [&](const void* blk, int m, DevBuf& set) {
             check(leo2d::launch_cond_scatter((const double*)blk, n, B, m, M, (double*)set.p, s), "conductivity scatter");
         };
@@ -1995,35 +1973,34 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
             for (int b = 0; b < B; ++b) tg[b] = p->targets[b0 + std::min(b, Bb - 1)];
             check(hipMemcpyAsync(dtg.p, tg.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s), "targets");
             check(hipEventRecord(e0, s), "event");
-            // beta_m = T_m(H) e_t
+            // beta_m = T_m(H) e_t, written into the set by the step itself
             check(leo2d::launch_init_targets(X0.p, leo2d::VEC_REAL_F64, n, B, (const int32_t*)dtg.p, s), "init");
             scatter(X0.p, 0, Bset);
-            step(dH, X0.p, X0.p, X1.p, 1.0, 0);
-            scatter(X1.p, 1, Bset);
+            auto dual = [&](const void* hv, const void* dv, const void* x, void* prev, void* out, double alpha, int beta,
+                            int j, DevBuf* hset, DevBuf* dset) {
+                check(leo2d::launch_cond_dual_step(drp, dcol, (const double*)hv, (const double*)dv, (const double*)x,
+                                                   (const double*)prev, (double*)out, alpha, beta, n, B, j, M,
+                                                   hset ? (double*)hset->p : nullptr, dset ? (double*)dset->p : nullptr,
+                                                   s),
+                      "conductivity step");
+            };
+            dual(dH, nullptr, X0.p, nullptr, X1.p, 1.0, 0, 1, &Bset, nullptr);
             void *cur = X1.p, *prv = X0.p;
             for (int m = 2; m < M; ++m) {
-                step(dH, cur, prv, prv, 2.0, 1);
-                scatter(prv, m, Bset);
+                dual(dH, nullptr, cur, prv, prv, 2.0, 1, m, &Bset, nullptr);
                 std::swap(cur, prv);
             }
-            // alpha_0 = -dxH e_t; a_0 = alpha_0, a_1 = H a_0, a_j = 2 H a_{j-1} - a_{j-2}; D_j = dxH a_j
+            // alpha_0 = -dxH e_t; a_1 = H a_0, a_j = 2 H a_{j-1} - a_{j-2}.  One pass over a_{j-1}
+            // forms both a_j and D_{j-1} = dxH a_{j-1} (H and dxH share the pattern)
             check(leo2d::launch_init_targets(X0.p, leo2d::VEC_REAL_F64, n, B, (const int32_t*)dtg.p, s), "init");
-            step(dX, X0.p, X0.p, X1.p, -1.0, 0);
-            step(dX, X1.p, X1.p, Y.p, 1.0, 0);
-            scatter(Y.p, 0, Dset);
-            if (M > 1) {
-                step(dH, X1.p, X1.p, X0.p, 1.0, 0);
-                step(dX, X0.p, X0.p, Y.p, 1.0, 0);
-                scatter(Y.p, 1, Dset);
-            }
-            cur = X0.p;
-            prv = X1.p;
-            for (int j = 2; j < M; ++j) {
-                step(dH, cur, prv, prv, 2.0, 1);
-                step(dX, prv, prv, Y.p, 1.0, 0);
-                scatter(Y.p, j, Dset);
+            dual(dX, nullptr, X0.p, nullptr, X1.p, -1.0, 0, 0, nullptr, nullptr);
+            cur = X1.p;      // a_0
+            prv = X0.p;
+            for (int j = 1; j < M; ++j) {
+                dual(dH, dX, cur, j == 1 ? nullptr : prv, prv, j == 1 ? 1.0 : 2.0, j == 1 ? 0 : 1, j - 1, nullptr, &Dset);
                 std::swap(cur, prv);
             }
+            dual(nullptr, dX, cur, nullptr, nullptr, 1.0, 0, M - 1, nullptr, &Dset);
             check(hipEventRecord(e1, s), "event");
             // C_b[j][m] = sum_i D_b[j][i] Bset_b[m][i]: column-major (M x M) = Bset_b^T (M x n) Dset_b (n x M)
             const double one = 1.0, zero = 0.0;

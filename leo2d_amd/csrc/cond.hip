@@ -5,10 +5,10 @@
 // The reference builds beta_m = T_m(H) e_t (m < M), walks a_n = T_n(H) alpha_0
 // with alpha_0 = -dxH e_t, and takes C[n][m] = (dxH a_n) . beta_m one dot
 // product at a time: M^2 vector dots per target.  Here both vector sets are
-// produced by the fused Chebyshev step kernels for a block of targets at once
-// and stored as [target][m][row]; C for every target is then one strided
-// batched FP64 GEMM (rocBLAS, MFMA), and the 2D Jackson/DCT-III transform is
-// two more GEMMs against the cosine matrix.
+// produced for a block of targets at once, one kernel per Chebyshev index
+// writing straight into the [target][m][row] sets; C for every target is then
+// an FP64 GEMM (rocBLAS, MFMA) split into row chunks, and the 2D
+// Jackson/DCT-III transform is two more GEMMs against the cosine matrix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +78,55 @@ __global__ __launch_bounds__(kCondThreads) void chunk_sum_kernel(const double* _
     }
 }
 
+// One Chebyshev index of the conductivity recursion for a block of B real
+// vectors x = [n][B] (thread per (row, vector), rows in stored CSR order):
+//   yh = alpha H x - (beta ? prev : 0)  -> out   [n][B] (the recursion; out may alias prev)
+//   yd = dxH x                           -> dset[b][j][row] (written straight into the set)
+// H and dxH share the pattern, so each entry's column and the gathered x are
+// read once for both products.  With hx == nullptr only the dxH product is
+// formed; with dx == nullptr only the H product, and hset (when given)
+// receives yh as vector j of the set.
+template <bool HAS_H, bool HAS_D>
+__global__ __launch_bounds__(kCondThreads) void dual_step_kernel(const int64_t* __restrict__ rp,
+                                                                 const int32_t* __restrict__ col,
+                                                                 const double* __restrict__ hv,
+                                                                 const double* __restrict__ dv, const double* x,
+                                                                 const double* prev, double* out, double alpha, int beta,
+                                                                 int n, int B, int j, int M, double* __restrict__ hset,
+                                                                 double* __restrict__ dset)
+{
+    constexpr int U = 8;   // entries whose gathers are in flight together
+    const size_t total = (size_t)n * B;
+    for (size_t t = blockIdx.x * (size_t)kCondThreads + threadIdx.x; t < total; t += (size_t)gridDim.x * kCondThreads) {
+        const int row = (int)(t / B);
+        const int b = (int)(t % B);
+        double ah = 0.0, ad = 0.0;
+        int64_t q = rp[row];
+        const int64_t e = rp[row + 1];
+        for (; q + U <= e; q += U) {
+            double xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[(size_t)col[q + u] * B + b];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (HAS_H) ah += hv[q + u] * xv[u];
+                if (HAS_D) ad += dv[q + u] * xv[u];
+            }
+        }
+        for (; q < e; ++q) {
+            const double xv = x[(size_t)col[q] * B + b];
+            if (HAS_H) ah += hv[q] * xv;
+            if (HAS_D) ad += dv[q] * xv;
+        }
+        if (HAS_H) {
+            const double y = alpha * ah - (beta ? prev[t] : 0.0);
+            out[t] = y;
+            if (hset) hset[((size_t)b * M + j) * n + row] = y;
+        }
+        if (HAS_D) dset[((size_t)b * M + j) * n + row] = ad;
+    }
+}
+
 dim3 grid_for(size_t work) { return dim3((unsigned)std::min<size_t>(4096, (work + kCondThreads - 1) / kCondThreads)); }
 
 }  // namespace
@@ -91,6 +140,26 @@ hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, do
 hipError_t launch_cond_chunk_sum(const double* part, int S, size_t mm, int B, double* C, hipStream_t s)
 {
     hipLaunchKernelGGL(chunk_sum_kernel, grid_for((size_t)B * mm), dim3(kCondThreads), 0, s, part, S, mm, B, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_cond_dual_step(const int64_t* rp, const int32_t* col, const double* hv, const double* dv,
+                                 const double* x, const double* prev, double* out, double alpha, int beta, int n, int B,
+                                 int j, int M, double* hset, double* dset, hipStream_t s)
+{
+    const size_t work = (size_t)n * B;
+    const dim3 g((unsigned)std::min<size_t>(8192, (work + kCondThreads - 1) / kCondThreads));
+    if (hv && dv)
+        hipLaunchKernelGGL((dual_step_kernel<true, true>), g, dim3(kCondThreads), 0, s, rp, col, hv, dv, x, prev, out,
+                           alpha, beta, n, B, j, M, hset, dset);
+    else if (hv)
+        hipLaunchKernelGGL((dual_step_kernel<true, false>), g, dim3(kCondThreads), 0, s, rp, col, hv, dv, x, prev, out,
+                           alpha, beta, n, B, j, M, hset, dset);
+    else if (dv)
+        hipLaunchKernelGGL((dual_step_kernel<false, true>), g, dim3(kCondThreads), 0, s, rp, col, hv, dv, x, prev, out,
+                           alpha, beta, n, B, j, M, hset, dset);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -144,6 +144,11 @@ hipError_t scan_exclusive_i64(const int64_t* cnt, int64_t* out, int n, hipStream
 // KPM conductivity (cond.hip): blk [n][B] -> set[b][m][n]; X = 2 g_i g_j C
 // (Jackson g / sqrt 2); column-major DCT-III matrix P(k, i) = w_i cos(pi i (k+1/2)/M)
 hipError_t launch_cond_scatter(const double* blk, int n, int B, int m, int M, double* set, hipStream_t s);
+// one index of the conductivity recursion on a real [n][B] block: out = alpha H x - (beta ? prev : 0)
+// (hset[b][j][.] = out when hset != nullptr) and dset[b][j][.] = dxH x; hv or dv may be nullptr
+hipError_t launch_cond_dual_step(const int64_t* rp, const int32_t* col, const double* hv, const double* dv,
+                                 const double* x, const double* prev, double* out, double alpha, int beta, int n, int B,
+                                 int j, int M, double* hset, double* dset, hipStream_t s);
 hipError_t launch_cond_weight(const double* C, int M, int T, double* X, hipStream_t s);
 // C[b] = sum_c part[b][c] (c = 0 .. S-1, fixed order), blocks of mm values
 hipError_t launch_cond_chunk_sum(const double* part, int S, size_t mm, int B, double* C, hipStream_t s);
