@@ -1624,8 +1624,15 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
                 }
             }
         }
-        std::ofstream f(prefix + ".cheb");
-        f << leo2d::cheb_text(geo.deck, jobs, rho_all, M);
+        {
+            std::ofstream f(prefix + ".cheb");
+            f << leo2d::cheb_text(geo.deck, jobs, rho_all, M);
+        }
+        // the job list's Bloch vectors (locality.cpp:2785-2797)
+        if (o.geti("k_sampling") == 1) {
+            std::ofstream f(prefix + ".kpts");
+            f << leo2d::kpts_text(jobs);
+        }
         return LEO2D_OK;
     });
 }

@@ -205,6 +205,149 @@ std::vector<std::vector<double>> reciprocal(const std::vector<std::vector<double
     return b;
 }
 
+using KVec = std::array<double, 2>;
+
+// Locality::getReciprocal(int s) (locality.cpp:7168-7213): sheet s's lattice turned
+// by its angle (and strained by u_ij on a periodic cell with UNIFORM_STRAIN = 1)
+std::vector<std::vector<double>> sheet_reciprocal(const Deck& d, int s)
+{
+    const Params& o = d.opts;
+    const double A = 2.4768;
+    const double uc[2][2] = {{1. * A, 0.}, {.5 * A, 0.866025403784438646764 * A}};
+    const double th = d.angles[s];
+    std::vector<std::vector<double>> a(2, std::vector<double>(2));
+    for (int i = 0; i < 2; ++i) {
+        a[i][0] = uc[i][0] * std::cos(th) - uc[i][1] * std::sin(th);
+        a[i][1] = uc[i][0] * std::sin(th) + uc[i][1] * std::cos(th);
+    }
+    if (o.geti("boundary_condition") == 1 && o.geti("uniform_strain") == 1) {
+        const auto& u = o.dm.at("u_ij");
+        for (auto& t : a) {
+            const double x = t[0], y = t[1];
+            t[0] = (1.0 + u[0][0]) * x + u[0][1] * y;
+            t[1] = u[1][0] * x + (1.0 + u[1][1]) * y;
+        }
+    }
+    return reciprocal(a);
+}
+
+// Piecewise-linear walk over the corners v[0] -> v[1] -> ... with parameter
+// c = span i / nk: leg s covers s <= c <= s + 1 and the point is
+// (s + 1 - c) v[s] + (c - s) v[s + 1]  (K_TYPE 1-3, locality.cpp:1760-1778,
+// 1864-1884, 1946-1970)
+std::vector<KVec> corner_walk(const std::vector<KVec>& v, double span, int nk)
+{
+    std::vector<KVec> pts;
+    const int legs = (int)v.size() - 1;
+    for (int i = 0; i < nk; ++i) {
+        const double c = (span / (double)nk) * i;
+        int s = legs - 1;
+        for (int q = 0; q < legs - 1; ++q)
+            if (c <= q + 1) { s = q; break; }
+        const double hi = s + 1.0, lo = s;
+        pts.push_back({(hi - c) * v[s][0] + (c - lo) * v[s + 1][0], (hi - c) * v[s][1] + (c - lo) * v[s + 1][1]});
+    }
+    return pts;
+}
+
+// The high-symmetry path of K_TYPE 1-5 for one base job (locality.cpp:1655-2249).
+// Types 4 and 5 step along lines given by slope and intercept; their arithmetic
+// keeps the reference's operation order so the k vectors agree bit for bit.
+std::vector<KVec> k_path(const Deck& d)
+{
+    const Params& o = d.opts;
+    const long long kt = o.geti("k_type");
+    const int nk = (int)o.geti("num_k1");
+    const int ns = (int)d.sheets.size();
+    const KVec gam = {0.0, 0.0};
+    if (kt == 1 || kt == 3) {
+        // supercell zone: K = b2 turned by -30 deg over 2 cos 30, M = b2 / 2
+        if (kt == 1 && ns < 2) throw DeckError("K_TYPE = 1 (twisted zone) needs two or more sheets");
+        const auto b = reciprocal(o.dm.at("supercell"));
+        const double f = 1.0 / (2.0 * std::cos(M_PI / 6));
+        const KVec kp = {f * (std::cos(-M_PI / 6) * b[1][0] - std::sin(-M_PI / 6) * b[1][1]),
+                         f * (std::sin(-M_PI / 6) * b[1][0] + std::cos(-M_PI / 6) * b[1][1])};
+        const KVec mp = {b[1][0] / 2.0, b[1][1] / 2.0};
+        if (kt == 1) return corner_walk({kp, gam, mp, kp}, 3.0, nk);
+        const KVec kq = {std::cos(-M_PI / 3) * kp[0] - std::sin(-M_PI / 3) * kp[1],
+                         std::sin(-M_PI / 3) * kp[0] + std::cos(-M_PI / 3) * kp[1]};
+        return corner_walk({gam, mp, kp, kq, gam}, 4.0, nk);
+    }
+    if (kt == 2) {
+        // zone of sheet 1: M = b1 / 2, K = M turned by 30 deg over cos 30
+        const auto b = sheet_reciprocal(d, 0);
+        const double h = (M_PI / 3.0) / 2.0;
+        const KVec mp = {b[0][0] / 2.0, b[0][1] / 2.0};
+        const KVec kp = {(mp[0] * std::cos(h) - mp[1] * std::sin(h)) / std::cos(h),
+                         (mp[0] * std::sin(h) + mp[1] * std::cos(h)) / std::cos(h)};
+        return corner_walk({gam, mp, kp, gam}, 3.0, nk);
+    }
+    std::vector<KVec> pts;
+    if (kt == 4) {
+        // (M, N) cell zone: Gamma -> K -> M -> Gamma, nk points per line
+        if (o.geti("supercell_type") != 1) throw DeckError("K_TYPE = 4 needs SUPERCELL_M_N");
+        const int M = (int)o.geti("m_supercell"), N = (int)o.geti("n_supercell");
+        const int A[2][2] = {{M > N ? N : M, M > N ? M : N}, {M > N ? -M : -N, M + N}};
+        const double a = 2.4768;
+        const double uc[2][2] = {{1. * a, 0.}, {.5 * a, 0.866025403784438646764 * a}};
+        std::vector<std::vector<double>> sc(2, std::vector<double>(2));
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 2; ++c) sc[r][c] = A[r][0] * uc[0][c] + A[r][1] * uc[1][c];
+        const auto bs = reciprocal(sc);
+        const double r3 = std::sqrt(3), al = M_PI / 6;
+        const double rot[2][2] = {{std::cos(al), std::sin(al)}, {-std::sin(al), std::cos(al)}};
+        const KVec mp = {0.5 * (bs[0][0] + bs[1][0]), 0.5 * (bs[0][1] + bs[1][1])};
+        KVec kp = {0.0, 0.0};
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) kp[i] = kp[i] + rot[i][j] * bs[1][j] / r3;
+        const double s1 = (kp[1] - gam[1]) / (kp[0] - gam[0]);
+        const double s2 = (mp[1] - kp[1]) / (mp[0] - kp[0]);
+        const double s3 = (gam[1] - mp[1]) / (gam[0] - mp[0]);
+        const double c1 = (kp[0] * gam[1] - gam[0] * kp[1]) / (kp[0] - gam[0]);
+        const double c3 = (gam[0] * mp[1] - mp[0] * gam[1]) / (gam[0] - mp[0]);
+        const double l1 = kp[0] - gam[0], l2 = mp[0] - kp[0], l3 = gam[0] - mp[0];
+        for (int i = 0; i < 3 * nk; ++i) {
+            if (i < nk) {
+                pts.push_back({gam[0] + i * l1 / nk, gam[1] + s1 * i * l1 / nk + c1});
+            } else if (i < 2 * nk) {
+                const int t = i - nk;
+                pts.push_back({kp[0] + t * l2 / nk, kp[1] + s2 * t * l2 / nk});
+            } else {
+                const int t = i - 2 * nk;
+                if (mp[0] == mp[1]) pts.push_back({mp[0], mp[1] + t * (gam[1] - mp[1]) / nk});
+                else pts.push_back({mp[0] + t * l3 / nk, mp[1] + s3 * t * l3 / nk + c3});
+            }
+        }
+        return pts;
+    }
+    if (kt == 5) {
+        // K points of sheets 1, 2, 3 (K_s = (2 b1 + b2) / 3 of each sheet), joined by lines
+        if (ns < 3) throw DeckError("K_TYPE = 5 needs three sheets");
+        KVec K[3];
+        for (int s = 0; s < 3; ++s) {
+            const auto b = sheet_reciprocal(d, s);
+            for (int i = 0; i < 2; ++i) K[s][i] = 1.0 / 3.0 * (2.0 * b[0][i] + b[1][i]);
+        }
+        const double s1 = (K[1][1] - K[0][1]) / (K[1][0] - K[0][0]);
+        const double s2 = (K[2][1] - K[1][1]) / (K[2][0] - K[1][0]);
+        const double m1 = K[1][1] - s1 * K[1][0], m2 = K[2][1] - s2 * K[2][0];
+        const double l1 = K[1][0] - K[0][0], l2 = K[2][0] - K[1][0];
+        for (int i = 0; i < 2 * nk + 1; ++i) {
+            if (i < nk) {
+                const double x = K[0][0] + i * l1 / nk;
+                pts.push_back({x, s1 * x + m1});
+            } else if (i < 2 * nk) {
+                const double x = K[1][0] + (i - nk) * l2 / nk;
+                pts.push_back({x, s2 * x + m2});
+            } else {
+                pts.push_back(K[2]);
+            }
+        }
+        return pts;
+    }
+    throw DeckError("K_TYPE " + std::to_string(kt) + " is not a k-point scheme");
+}
+
 }  // namespace
 
 std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
@@ -298,8 +441,20 @@ std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
     } else {
         throw DeckError("SOLVER_TYPE " + std::to_string(st) + " is not on this path (SQ and LC are)");
     }
-    if (o.geti("k_sampling") == 1) {
-        if (o.geti("k_type") != 0) throw DeckError("only K_TYPE = 0 is on this path");
+    if (o.geti("k_sampling") == 1 && o.geti("k_type") != 0) {
+        if (g.bc != 1) throw DeckError("K_SAMPLING needs BOUNDARY_CONDITION = 1");
+        const std::vector<KVec> pts = k_path(d);
+        std::vector<leo2d_job> kj;
+        for (const leo2d_job& base : jobs)
+            for (const KVec& kv : pts) {
+                leo2d_job j = base;
+                j.job_id = (int)kj.size() + 1;
+                j.k[0] = kv[0];
+                j.k[1] = kv[1];
+                kj.push_back(j);
+            }
+        jobs = kj;
+    } else if (o.geti("k_sampling") == 1) {
         if (g.bc != 1) throw DeckError("K_SAMPLING needs BOUNDARY_CONDITION = 1");
         const int nk1 = (int)o.geti("num_k1"), nk2 = (int)o.geti("num_k2");
         const auto b = reciprocal(o.dm.at("supercell"));
@@ -430,6 +585,21 @@ std::string matrix_dat_text(int n, const int64_t* rp, const int32_t* col, const 
                 out += b;
             }
         }
+    return out;
+}
+
+std::string kpts_text(const std::vector<leo2d_job>& jobs)
+{
+    // Param_tools::saveKpts (param_tools.cpp:579-602): fixed with 10 decimals, a space
+    // before a component that is >= 0, two tabs between the components, " \n" after
+    std::string out;
+    char b[128];
+    for (const leo2d_job& j : jobs) {
+        if (j.k_sampling != 1) continue;
+        std::snprintf(b, sizeof b, "%s%.10f\t\t%s%.10f \n", j.k[0] >= 0 ? " " : "", j.k[0], j.k[1] >= 0 ? " " : "",
+                      j.k[1]);
+        out += b;
+    }
     return out;
 }
 

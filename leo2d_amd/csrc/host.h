@@ -83,6 +83,8 @@ std::vector<int> target_list(const Deck& d, const Geom& g);
 std::string cheb_text(const Deck& d, const std::vector<leo2d_job>& jobs, const std::vector<std::vector<double>>& rho,
                       int M);
 std::string matrix_dat_text(int n, const int64_t* rp, const int32_t* col, const double* val, int is_complex);
+// .kpts: one line per k-sampled job with its Bloch vector
+std::string kpts_text(const std::vector<leo2d_job>& jobs);
 // MATRIX_POS_SAVE writers: _pos.dat (job positions, sheet, orbital) and the pair-position lists
 std::string pos_dat_text(int n, const double* i2pos, const int32_t* i2g);
 std::string pair_pos_text(int64_t np, const int32_t* ij, const double* i2pos);

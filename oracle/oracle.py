@@ -7,10 +7,11 @@ Pieces restated here (the numeric core is leo2d_oracle.c):
   * deck parsing          app/src/main.cpp:75-937 (token rules and quirks)
   * Job_params defaults   src/params/job_params.cpp:18-156
   * supercell setup       src/locality.cpp:131-400 (types 1 and 2)
-  * job generation        src/locality.cpp:1222-1388 (SQ, LC), 1615-1652 (K_TYPE 0)
+  * job generation        src/locality.cpp:1222-1388 (SQ, LC), 1615-2249 (K_TYPE 0-5)
   * targets               src/geom/hstruct.cpp:1049-1080
   * .cheb writer          src/params/param_tools.cpp:19-285, 467-577
   * _matrix.dat writer    src/locality.cpp:4366-4383 (real), 5213-5228 (complex)
+  * .kpts writer          src/params/param_tools.cpp:579-602
 """
 from __future__ import annotations
 
@@ -584,9 +585,166 @@ def reciprocal(a):
     return b
 
 
+def sheet_reciprocal(deck: Deck, s: int):
+    """Locality::getReciprocal(int s) (locality.cpp:7168-7213): sheet s's lattice
+    rotated by its angle, strained by u_ij when BOUNDARY_CONDITION = 1 and
+    UNIFORM_STRAIN = 1, then reciprocal(a)."""
+    o = deck.opts
+    th = deck.angles[s]
+    a = [[LATTICE[i][0] * math.cos(th) - LATTICE[i][1] * math.sin(th),
+          LATTICE[i][0] * math.sin(th) + LATTICE[i][1] * math.cos(th)] for i in range(2)]
+    if o["boundary_condition"] == 1 and o["uniform_strain"] == 1:
+        u = o["u_ij"]
+        a = [[(1.0 + u[0][0]) * t[0] + u[0][1] * t[1], u[1][0] * t[0] + (1.0 + u[1][1]) * t[1]] for t in a]
+    return reciprocal(a)
+
+
+def _fdiv(x: float, y: float) -> float:
+    """x / y with IEEE semantics (inf / nan instead of ZeroDivisionError), as the
+    reference's double division gives."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return float(np.float64(x) / np.float64(y))
+
+
+def k_path(deck: Deck) -> list:
+    """k vectors of one base job for K_TYPE 1-5 (locality.cpp:1655-2249), in job order.
+
+    1: K -> Gamma -> M -> K of the supercell zone, c = 3 i / num_k1 (1655-1796)
+    2: Gamma -> M -> K -> Gamma of sheet 1's zone (1799-1914)
+    3: Gamma -> M -> K1 -> K2 -> Gamma of the supercell zone, c = 4 i / num_k1 (1916-1988)
+    4: Gamma -> K -> M -> Gamma of the (M, N) cell, num_k1 points per line (1990-2163)
+    5: K of sheet 1 -> K of sheet 2 -> K of sheet 3, 2 num_k1 + 1 points (2165-2249)"""
+    o = deck.opts
+    kt = o["k_type"]
+    nk = o["num_k1"]
+    ns = len(deck.sdata)
+    pts = []
+    if kt in (1, 3):
+        if kt == 1 and ns < 2:
+            raise DeckError("K_TYPE = 1 (twisted zone) needs two or more sheets")
+        b = reciprocal(o["supercell"])
+        g = (0.0, 0.0)
+        k1 = ((1.0 / (2.0 * math.cos(math.pi / 6))) * (math.cos(-math.pi / 6) * b[1][0] - math.sin(-math.pi / 6) * b[1][1]),
+              (1.0 / (2.0 * math.cos(math.pi / 6))) * (math.sin(-math.pi / 6) * b[1][0] + math.cos(-math.pi / 6) * b[1][1]))
+        m = (b[1][0] / 2.0, b[1][1] / 2.0)
+        if kt == 1:
+            legs = (k1, g, m, k1)
+            span = 3.0
+        else:
+            k2 = (math.cos(-math.pi / 3) * k1[0] - math.sin(-math.pi / 3) * k1[1],
+                  math.sin(-math.pi / 3) * k1[0] + math.cos(-math.pi / 3) * k1[1])
+            legs = (g, m, k1, k2, g)
+            span = 4.0
+        for i in range(nk):
+            c = (span / float(nk)) * i
+            seg = len(legs) - 2
+            for s in range(len(legs) - 2):
+                if c <= s + 1:
+                    seg = s
+                    break
+            p, q = legs[seg], legs[seg + 1]
+            hi = float(seg + 1)
+            lo = float(seg)
+            pts.append([(hi - c) * p[0] + (c - lo) * q[0], (hi - c) * p[1] + (c - lo) * q[1]])
+    elif kt == 2:
+        b1 = sheet_reciprocal(deck, 0)
+        phi = math.pi / 3.0
+        g = (0.0, 0.0)
+        m = (b1[0][0] / 2.0, b1[0][1] / 2.0)
+        k = ((m[0] * math.cos(phi / 2.0) - m[1] * math.sin(phi / 2.0)) / math.cos(phi / 2.0),
+             (m[0] * math.sin(phi / 2.0) + m[1] * math.cos(phi / 2.0)) / math.cos(phi / 2.0))
+        legs = (g, m, k, g)
+        for i in range(nk):
+            c = (3.0 / float(nk)) * i
+            seg = 0 if c <= 1 else (1 if c <= 2 else 2)
+            p, q = legs[seg], legs[seg + 1]
+            pts.append([(seg + 1.0 - c) * p[0] + (c - seg) * q[0], (seg + 1.0 - c) * p[1] + (c - seg) * q[1]])
+    elif kt == 4:
+        if o.get("supercell_type") != 1:
+            raise DeckError("K_TYPE = 4 needs SUPERCELL_M_N")
+        M, N = o["m_supercell"], o["n_supercell"]
+        groups = [0, 1] if ns < 3 else o["sc_groups"]
+        if len(groups) == 2:
+            A = ((N, M), (-M, M + N)) if M > N else ((M, N), (-N, M + N))
+        else:
+            t = M * M + M * N + N * N
+            A = ((0, t), (-t, t))
+        uc = LATTICE
+        sc = [[A[0][0] * uc[0][0] + A[0][1] * uc[1][0], A[0][0] * uc[0][1] + A[0][1] * uc[1][1]],
+              [A[1][0] * uc[0][0] + A[1][1] * uc[1][0], A[1][0] * uc[0][1] + A[1][1] * uc[1][1]]]
+        bs = reciprocal(sc)
+        r3 = math.sqrt(3)
+        al = math.pi / 6
+        rot = ((math.cos(al), math.sin(al)), (-math.sin(al), math.cos(al)))
+        m = [0.5 * (bs[0][i] + bs[1][i]) for i in range(2)]
+        k = [0.0, 0.0]
+        for i in range(2):
+            for j in range(2):
+                k[i] = k[i] + rot[i][j] * bs[1][j] / r3
+        g = (0.0, 0.0)
+        s1 = _fdiv(k[1] - g[1], k[0] - g[0])
+        s2 = _fdiv(m[1] - k[1], m[0] - k[0])
+        s3 = _fdiv(g[1] - m[1], g[0] - m[0])
+        b1 = _fdiv(k[0] * g[1] - g[0] * k[1], k[0] - g[0])
+        b3 = _fdiv(g[0] * m[1] - m[0] * g[1], g[0] - m[0])
+        l1 = k[0] - g[0]
+        l2 = m[0] - k[0]
+        l3 = g[0] - m[0]
+        for i in range(3 * nk):
+            if i < nk:
+                pts.append([g[0] + i * l1 / nk, g[1] + s1 * i * l1 / nk + b1])
+            elif i < 2 * nk:
+                n = i - nk
+                pts.append([k[0] + n * l2 / nk, k[1] + s2 * n * l2 / nk])
+            else:
+                n = i - 2 * nk
+                if m[0] == m[1]:
+                    ln = g[1] - m[1]
+                    pts.append([m[0], m[1] + n * ln / nk])
+                else:
+                    pts.append([m[0] + n * l3 / nk, m[1] + s3 * n * l3 / nk + b3])
+    elif kt == 5:
+        if ns < 3:
+            raise DeckError("K_TYPE = 5 needs three sheets")
+        bl = [sheet_reciprocal(deck, s) for s in range(3)]
+        K = [[1.0 / 3.0 * (2.0 * b[0][i] + b[1][i]) for i in range(2)] for b in bl]
+        s1 = _fdiv(K[1][1] - K[0][1], K[1][0] - K[0][0])
+        s2 = _fdiv(K[2][1] - K[1][1], K[2][0] - K[1][0])
+        m1 = K[1][1] - s1 * K[1][0]
+        m2 = K[2][1] - s2 * K[2][0]
+        l1 = K[1][0] - K[0][0]
+        l2 = K[2][0] - K[1][0]
+        for i in range(2 * nk + 1):
+            if i < nk:
+                x = K[0][0] + i * l1 / nk
+                pts.append([x, s1 * x + m1])
+            elif i < 2 * nk:
+                x = K[1][0] + (i - nk) * l2 / nk
+                pts.append([x, s2 * x + m2])
+            else:
+                pts.append([K[2][0], K[2][1]])
+    else:
+        raise DeckError(f"K_TYPE {kt} is not a k-point scheme")
+    return pts
+
+
+def kpts_text(jobs: list) -> str:
+    """Param_tools::saveKpts (param_tools.cpp:579-602) over the job list: fixed,
+    10 decimals, a leading space on non-negative values, two tabs between the
+    components and ' \\n' after the second."""
+    out = []
+    for job in jobs:
+        if "k_vec" not in job:
+            continue
+        k0, k1 = job["k_vec"]
+        out.append((" " if k0 >= 0 else "") + "%.10f" % k0 + "\t\t" + (" " if k1 >= 0 else "") + "%.10f" % k1 + " \n")
+    return "".join(out)
+
+
 def build_jobs(deck: Deck, targets: list) -> list:
     """rootChebSolve job list: SQ via recursiveShiftCalc (locality.cpp:1222-1269,
-    1308-1327), LC (1330-1388), then the K_TYPE 0 expansion (1615-1652)."""
+    1308-1327), LC (1330-1388), then the k-point expansion: the K_TYPE 0 grid
+    (1615-1652) or the K_TYPE 1-5 paths of k_path (1655-2249)."""
     o = deck.opts
     ns = len(deck.sdata)
     jobs = []
@@ -628,9 +786,19 @@ def build_jobs(deck: Deck, targets: list) -> list:
                          "target_list": list(targets)})
     else:
         raise DeckError(f"solver_type {st} is outside this path")
-    if o.get("k_sampling", 0) == 1:
-        if o.get("k_type", 0) != 0:
-            raise DeckError("only K_TYPE = 0 is on this path")
+    if o.get("k_sampling", 0) == 1 and o["boundary_condition"] != 1:
+        raise DeckError("K_SAMPLING needs BOUNDARY_CONDITION = 1")
+    if o.get("k_sampling", 0) == 1 and o.get("k_type", 0) != 0:
+        pts = k_path(deck)
+        kj = []
+        for job in jobs:
+            for kv in pts:
+                nj = dict(job)
+                nj["jobID"] = len(kj) + 1
+                nj["k_vec"] = list(kv)
+                kj.append(nj)
+        jobs = kj
+    elif o.get("k_sampling", 0) == 1:
         nk1, nk2 = o["num_k1"], o["num_k2"]
         b = reciprocal(o["supercell"])
         kj = []

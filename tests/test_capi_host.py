@@ -32,7 +32,8 @@ def test_exports_every_declared_symbol():
 
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
-             "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in"]
+             "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
+             "ttlg_grid_k5.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -78,6 +79,63 @@ def test_jobs_match_oracle(oracle_mod, name):
         for s in range(g.info.num_sheets):
             for a in range(2):
                 assert tuple(j.sheet_strain[s][a]) == tuple(oj_struct.sheet_strain[s][a])
+
+
+def _k_variant(name, k_type, nk, *edits):
+    text = re.sub(r"K_TYPE = \d+", "K_TYPE = %d" % k_type, deck_text(name))
+    text = re.sub(r"K_GRID = \d+ \d+", "K_GRID = %d 1" % nk, text)
+    for a, b in edits:
+        text = text.replace(a, b)
+    return text
+
+
+@pytest.mark.parametrize("name,k_type,nk,edits", [
+    ("tblg_sc_7_6_k.in", 1, 7, ()),
+    ("tblg_sc_7_6_k.in", 2, 8, ()),
+    ("tblg_sc_7_6_k.in", 3, 9, ()),
+    ("tblg_sc_7_6_k.in", 4, 5, ()),
+    ("tblg_sc_7_6_k.in", 4, 3, (("SUPERCELL_M_N = 7 6", "SUPERCELL_M_N = 6 7"),)),
+    ("tblg_grid_3x3.in", 1, 4, ()),
+    ("tblg_grid_3x3.in", 3, 6, ()),
+    ("mlg_strain_k2.in", 2, 1, ()),
+    ("mlg_strain_k2.in", 2, 5, (("0.00 0.05 0.05 0.00", "0.01 0.05 -0.02 0.03"),)),
+    ("ttlg_grid_k5.in", 5, 1, ()),
+    ("ttlg_grid_k5.in", 5, 6, ()),
+    ("ttlg_grid_k5.in", 2, 4, ()),
+])
+def test_k_paths_match_oracle(oracle_mod, name, k_type, nk, edits):
+    """K_TYPE 1-5 (locality.cpp:1655-2249): the library's job list has the oracle's
+    length and order and bit-identical Bloch vectors."""
+    text = _k_variant(name, k_type, nk, *edits)
+    jobs = L.Geometry(L.Deck(text)).jobs()
+    od = oracle_mod.parse_deck(text)
+    oracle_mod.setup_supercell(od)
+    og = oracle_mod.Geometry(oracle_mod.geom_desc(od))
+    ojobs = oracle_mod.build_jobs(od, oracle_mod.target_list(od, og))
+    assert len(jobs) == len(ojobs)
+    assert len(ojobs) % len(oracle_mod.k_path(od)) == 0
+    for j, oj in zip(jobs, ojobs):
+        assert j.job_id == oj["jobID"]
+        assert (j.k[0], j.k[1]) == tuple(oj["k_vec"])
+        assert tuple(j.shift[len(od.sdata) - 1][:2]) == tuple(oj["shifts"][-1][:2])
+
+
+@pytest.mark.parametrize("name,k_type,edits", [
+    ("mlg_strain_k2.in", 1, ()),                                             # twisted zone, one sheet
+    ("tblg_sc_7_6_k.in", 5, ()),                                             # three Dirac points, two sheets
+    ("tblg_grid_3x3.in", 4, ()),                                             # (M, N) path on a grid cell
+    ("tblg_sc_7_6_k.in", 6, ()),                                             # no such scheme
+    ("tblg_sc_7_6_k.in", 1, (("BOUNDARY_CONDITION = 1", "BOUNDARY_CONDITION = 0"),)),
+])
+def test_k_path_errors(oracle_mod, name, k_type, edits):
+    text = _k_variant(name, k_type, 4, *edits)
+    od = oracle_mod.parse_deck(text)
+    oracle_mod.setup_supercell(od)
+    with pytest.raises(oracle_mod.DeckError):
+        oracle_mod.build_jobs(od, [0, 1])
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(text)).jobs()
+    assert e.value.code == L.ERR_DECK
 
 
 def test_commensurate_cell_count():

@@ -185,7 +185,8 @@ def _cheb_numbers(text):
 
 
 @pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in",
-                                  "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose"])
+                                  "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose",
+                                  "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in"])
 def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     name, _, variant = name.partition(":")
     text = deck_text(name)
@@ -201,6 +202,11 @@ def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     wn, ws = _cheb_numbers(want)
     assert gs == ws                                   # identical layout and integers
     assert np.max(np.abs(np.array(gn) - np.array(wn))) <= 1.0000001e-6   # 6-decimal rounding
+    if od.opts["k_sampling"] == 1:
+        # Bloch vector of every job, byte for byte (param_tools.cpp:579-602)
+        assert open(prefix + ".kpts").read() == oracle_mod.kpts_text(jobs)
+    else:
+        assert not os.path.exists(prefix + ".kpts")
     if od.opts["matrix_save"] > 0:
         rp, ci, v = og.assemble(oracle_mod.make_job(od, jobs[-1]))
         assert os.path.exists(prefix + "_matrix.dat")

@@ -592,3 +592,151 @@ def test_cond_transform_against_scipy_dctn(oracle_mod):
         g = ((M - i) * np.cos(np.pi * i / M) + np.sin(np.pi * i / M) / np.tan(np.pi / M)) / (np.sqrt(2.0) * M)
         want = dctn(2.0 * np.outer(g, g) * Cm, type=3)
         assert np.max(np.abs(out - want)) <= 1e-12 * np.max(np.abs(want)), M
+
+
+# ---------------------------------------------------------------- k-point paths
+DECKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks")
+
+
+def _k_deck(orc, name, k_type, nk, *edits):
+    text = open(os.path.join(DECKS, name)).read()
+    text = re.sub(r"K_TYPE = \d+", "K_TYPE = %d" % k_type, text)
+    text = re.sub(r"K_GRID = \d+ \d+", "K_GRID = %d 1" % nk, text)
+    for a, b in edits:
+        text = text.replace(a, b)
+    od = orc.parse_deck(text)
+    orc.setup_supercell(od)
+    return od
+
+
+def _recip(a):
+    """Rows b_i with b_i . a_j = 2 pi delta_ij."""
+    return 2 * math.pi * np.linalg.inv(np.array(a, dtype=float)).T
+
+
+def _rot(v, deg):
+    t = math.radians(deg)
+    return np.array([math.cos(t) * v[0] - math.sin(t) * v[1], math.sin(t) * v[0] + math.cos(t) * v[1]])
+
+
+def _corner(p, q):
+    """Circumcentre of the triangle 0, p, q: the x with 2 x.p = |p|^2 and 2 x.q = |q|^2."""
+    return np.linalg.solve(2.0 * np.array([p, q]), [p @ p, q @ q])
+
+
+def _walk(corners, span, nk):
+    """Points at c = span i / nk along corners[0] -> corners[1] -> ..., one leg per unit of c."""
+    out = []
+    for i in range(nk):
+        c = span * i / nk
+        s = min(int(math.floor(c)), len(corners) - 2)
+        out.append(corners[s] + (c - s) * (corners[s + 1] - corners[s]))
+    return np.array(out)
+
+
+def _is_zone_corner(K, b):
+    """K is a vertex of the Wigner-Seitz cell of the lattice spanned by b: three
+    lattice points (the origin one of them) at the same, smallest distance."""
+    d = sorted(np.hypot(*(K - (n1 * b[0] + n2 * b[1])))
+               for n1 in range(-3, 4) for n2 in range(-3, 4))
+    return abs(d[0] - np.hypot(*K)) < 1e-12 * d[0] and abs(d[2] - d[0]) < 1e-12 * d[0] and d[3] > d[0] * 1.01
+
+
+def test_reciprocal_against_matrix_inverse(oracle_mod):
+    """reciprocal(a) and the sheet lattice's reciprocal (turned, uniformly strained)
+    satisfy b_i . a_j = 2 pi delta_ij."""
+    od = _k_deck(oracle_mod, "tblg_sc_7_6_k.in", 0, 2)
+    assert np.allclose(oracle_mod.reciprocal(od.opts["supercell"]), _recip(od.opts["supercell"]), rtol=0, atol=1e-14)
+    od = _k_deck(oracle_mod, "mlg_strain_k2.in", 2, 9, ("0.00 0.05 0.05 0.00", "0.01 0.05 -0.02 0.03"))
+    u = np.array(od.opts["u_ij"])
+    assert u[0][1] != u[1][0]
+    a = np.array(oracle_mod.LATTICE) @ (np.eye(2) + u).T          # a_i -> (1 + u) a_i
+    assert np.allclose(oracle_mod.sheet_reciprocal(od, 0), _recip(a), rtol=0, atol=1e-14)
+    od = _k_deck(oracle_mod, "ttlg_grid_k5.in", 5, 4)
+    a = np.array([_rot(v, 2.1) for v in oracle_mod.LATTICE])
+    assert np.allclose(oracle_mod.sheet_reciprocal(od, 2), _recip(a), rtol=0, atol=1e-14)
+
+
+def _path(orc, od):
+    return np.array(orc.k_path(od))
+
+
+@pytest.mark.parametrize("nk", [6, 7])
+def test_k_type1_supercell_path(oracle_mod, nk):
+    """K -> Gamma -> M -> K of the (7, 6) cell's zone: K is the zone corner next to
+    M = b2 / 2 on the clockwise side, M the midpoint of the edge facing b2."""
+    od = _k_deck(oracle_mod, "tblg_sc_7_6_k.in", 1, nk)
+    b = _recip(od.opts["supercell"])
+    K = _corner(b[1], _rot(b[1], -60))
+    assert _is_zone_corner(K, b)
+    want = _walk([K, np.zeros(2), b[1] / 2, K], 3.0, nk)
+    got = _path(oracle_mod, od)
+    assert got.shape == (nk, 2)
+    assert np.allclose(got, want, rtol=0, atol=1e-12 * np.hypot(*b[1]))
+
+
+def test_k_type2_monolayer_path(oracle_mod):
+    """Gamma -> M -> K -> Gamma of graphene's own zone: M = b1 / 2 and K the corner
+    at 4 pi / (3 a) (textbook), i.e. K = (4 pi / 3 a, 0) for the unturned sheet."""
+    od = _k_deck(oracle_mod, "mlg_strain_k2.in", 2, 9, ("UNIFORM_STRAIN = 1", "UNIFORM_STRAIN = 0"))
+    b = _recip(oracle_mod.LATTICE)
+    K = _corner(b[0], _rot(b[0], 60))
+    assert _is_zone_corner(K, b)
+    assert np.allclose(K, [4 * math.pi / (3 * oracle_mod.GR_A), 0.0], rtol=0, atol=1e-14)
+    got = _path(oracle_mod, od)
+    assert np.allclose(got, _walk([np.zeros(2), b[0] / 2, K, np.zeros(2)], 3.0, 9), rtol=0, atol=1e-13)
+
+
+def test_k_type3_supercell_path(oracle_mod):
+    """Gamma -> M -> K -> K' -> Gamma: K' is the next zone corner clockwise."""
+    od = _k_deck(oracle_mod, "tblg_sc_7_6_k.in", 3, 8)
+    b = _recip(od.opts["supercell"])
+    K1 = _corner(b[1], _rot(b[1], -60))
+    K2 = _corner(_rot(b[1], -60), _rot(b[1], -120))
+    assert _is_zone_corner(K1, b) and _is_zone_corner(K2, b)
+    want = _walk([np.zeros(2), b[1] / 2, K1, K2, np.zeros(2)], 4.0, 8)
+    assert np.allclose(_path(oracle_mod, od), want, rtol=0, atol=1e-12 * np.hypot(*b[1]))
+
+
+@pytest.mark.parametrize("mn", ["7 6", "6 7"])
+def test_k_type4_cell_path(oracle_mod, mn):
+    """Gamma -> K -> M -> Gamma of the (M, N) cell built on the unturned sheet
+    lattice, nk points per line from each line's start."""
+    nk = 5
+    od = _k_deck(oracle_mod, "tblg_sc_7_6_k.in", 4, nk, ("SUPERCELL_M_N = 7 6", "SUPERCELL_M_N = " + mn))
+    M, N = (int(x) for x in mn.split())
+    A = np.array([[N, M], [-M, M + N]] if M > N else [[M, N], [-N, M + N]], dtype=float)
+    b = _recip(A @ np.array(oracle_mod.LATTICE))
+    K = _corner(b[1], _rot(b[1], -60))
+    Mp = (b[0] + b[1]) / 2
+    assert _is_zone_corner(K, b)
+    corners = [np.zeros(2), K, Mp, np.zeros(2)]
+    want = np.array([corners[i // nk] + (i % nk) / nk * (corners[i // nk + 1] - corners[i // nk])
+                     for i in range(3 * nk)])
+    assert np.allclose(_path(oracle_mod, od), want, rtol=0, atol=1e-12 * np.hypot(*b[1]))
+
+
+def test_k_type5_trilayer_dirac_points(oracle_mod):
+    """K of sheet 1 -> K of sheet 2 -> K of sheet 3: each K is a corner of its turned
+    sheet's zone at 4 pi / (3 a); 2 nk points on the two lines, then K3."""
+    nk = 4
+    od = _k_deck(oracle_mod, "ttlg_grid_k5.in", 5, nk)
+    Ks = []
+    for s in range(3):
+        b = _recip([_rot(v, math.degrees(od.angles[s])) for v in oracle_mod.LATTICE])
+        K = _corner(b[0], b[0] + b[1])
+        assert _is_zone_corner(K, b)
+        assert abs(np.hypot(*K) - 4 * math.pi / (3 * oracle_mod.GR_A)) < 1e-14
+        Ks.append(K)
+    want = [Ks[0] + i / nk * (Ks[1] - Ks[0]) for i in range(nk)]
+    want += [Ks[1] + i / nk * (Ks[2] - Ks[1]) for i in range(nk)] + [Ks[2]]
+    assert np.allclose(_path(oracle_mod, od), np.array(want), rtol=0, atol=1e-13)
+
+
+def test_kpts_text_format(oracle_mod):
+    """saveKpts: std::fixed, precision 10, a space before a value >= 0 (so before
+    -0.0 too), two tabs, ' \\n'; jobs without a Bloch vector write nothing."""
+    jobs = [{"k_vec": [0.5, -0.25]}, {"k_vec": [-0.0, 1e-17]}, {"k_vec": [-1e-12, 3.0]}, {"jobID": 4}]
+    assert oracle_mod.kpts_text(jobs) == (" 0.5000000000\t\t-0.2500000000 \n"
+                                          " -0.0000000000\t\t 0.0000000000 \n"
+                                          "-0.0000000000\t\t 3.0000000000 \n")
