@@ -296,13 +296,16 @@ int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, in
                        float* spread);
 
 /* ---- KPM conductivity (OBSERVABLE_TYPE = 1) -------------------------------- */
-/* leo2d_assemble plus the x-velocity matrix dxH on H's pattern: slot values
- * (x_row + image shift - x_col) t summed like H's (locality.cpp:4182, 4299).
- * Real H only (the reference's conductivity path is real; a complex job
- * returns LEO2D_ERR_ARG). */
+/* leo2d_assemble plus the velocity matrices dxH and dyH on H's pattern: slot
+ * values (x_row + image shift - x_col) t and (y_row + image shift - y_col) t,
+ * summed like H's, with t the real coupling or, for a complex H, the phased
+ * one (generateRealH / generateCpxH, locality.cpp:4182-4183, 4299-4300,
+ * 4939-4940, 5113-5114). */
 int leo2d_assemble_velocity(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz);
-/* Copy dxH's values (same pattern and order as leo2d_csr_copy). */
-int leo2d_velocity_copy(leo2d_handle* h, double* val_dx);
+/* Copy dxH's and dyH's values (same pattern and order as leo2d_csr_copy; nnz
+ * doubles each for a real H, 2 nnz interleaved for a complex one).  Either
+ * pointer may be NULL. */
+int leo2d_velocity_copy(leo2d_handle* h, double* val_dx, double* val_dy);
 
 typedef struct {
     int32_t M;                 /* moments per direction: C is M x M per target */

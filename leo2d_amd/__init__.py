@@ -176,7 +176,7 @@ def lib() -> C.CDLL:
         L.leo2d_csr_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
         L.leo2d_csr_copy.argtypes = [vp, vp, vp, vp]
         L.leo2d_assemble_velocity.argtypes = [vp, C.POINTER(Job), C.c_int32, C.POINTER(C.c_int64)]
-        L.leo2d_velocity_copy.argtypes = [vp, vp]
+        L.leo2d_velocity_copy.argtypes = [vp, vp, vp]
         L.leo2d_cond_moments.argtypes = [vp, C.POINTER(CondParams), vp]
         L.leo2d_cond_transform.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
         L.leo2d_cond_last_timing.argtypes = [vp, C.POINTER(CondTiming)]
@@ -326,16 +326,22 @@ class Device:
         return nnz.value
 
     def assemble_velocity(self, job: Job, pattern: int = PATTERN_REFERENCE) -> int:
-        """H and the x-velocity matrix dxH on its pattern (real H only)."""
+        """H and the velocity matrices dxH, dyH on its pattern (real or complex like H)."""
         nnz = C.c_int64()
         self._check(lib().leo2d_assemble_velocity(self.h, C.byref(job), pattern, C.byref(nnz)))
         return nnz.value
 
-    def velocity_copy(self):
-        _, nnz, _ = self.csr_info()
-        v = np.zeros(max(nnz, 1), dtype=np.float64)
-        self._check(lib().leo2d_velocity_copy(self.h, _p(v)))
-        return v[:nnz]
+    def velocity_copy(self, with_dy: bool = False):
+        """dxH (and dyH) values; complex128 arrays for a complex H."""
+        _, nnz, cpx = self.csr_info()
+        w = 2 if cpx else 1
+        dx = np.zeros(w * max(nnz, 1), dtype=np.float64)
+        dy = np.zeros(w * max(nnz, 1), dtype=np.float64)
+        self._check(lib().leo2d_velocity_copy(self.h, _p(dx), _p(dy) if with_dy else None))
+        dx, dy = dx[: w * nnz], dy[: w * nnz]
+        if cpx:
+            dx, dy = dx.view(np.complex128), dy.view(np.complex128)
+        return (dx, dy) if with_dy else dx
 
     def cond_moments(self, M, targets, block=0):
         """KPM conductivity moments C[t][n][m] (computeCondKPM)."""

@@ -54,7 +54,7 @@ hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* 
 hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipStream_t s);
 hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
                       const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
-                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, hipStream_t s);
+                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, hipStream_t s);
 hipError_t prune_count(int n, const int64_t* slot_rp, const uint8_t* skeep, int64_t* cnt, int64_t* out_rp,
                        hipStream_t s);
 hipError_t prune_compact(int n, int is_complex, const int64_t* slot_rp, const uint8_t* skeep, const int32_t* scol,

@@ -320,7 +320,7 @@ template <bool CPX>
 __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, const double* i2pos, const int64_t* pair_rp,
                                                         const int32_t* cnt_intra, const int32_t* pcol,
                                                         const int32_t* psc, const int64_t* slot_rp, int32_t* scol,
-                                                        double* sval, uint8_t* skeep, double* sdx)
+                                                        double* sval, uint8_t* skeep, double* sdx, double* sdy)
 {
     const int k_i = blockIdx.x * kThreads + threadIdx.x;
     if (k_i >= g.n) return;
@@ -332,7 +332,9 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
     const int ih = g.i2g[4 * (size_t)k_i + 0], jh = g.i2g[4 * (size_t)k_i + 1], l0 = g.i2g[4 * (size_t)k_i + 2];
     const double xi = i2pos[3 * (size_t)k_i], yi = i2pos[3 * (size_t)k_i + 1], zi = i2pos[3 * (size_t)k_i + 2];
     const double resc = j.energy_rescale, eshift = j.energy_shift;
-    double ar = 0.0, ai = 0.0, adx = 0.0;   // adx: x-velocity slot (real H, sdx != nullptr)
+    double ar = 0.0, ai = 0.0;
+    // velocity slots (sdx != nullptr): (x1 - x2) t and (y1 - y2) t, real or complex like H
+    double axr = 0.0, axi = 0.0, ayr = 0.0, ayi = 0.0;
     bool keep = false;
     for (int64_t q = q0; q < q1; ++q) {
         const bool intra = q < qi;
@@ -377,7 +379,8 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
         if (!CPX) {
             if (t != 0) {
                 ar += t;
-                adx += (x1 - x2) * t;   // (delta_x) t, locality.cpp:4182, 4299
+                axr += (x1 - x2) * t;   // locality.cpp:4182-4183, 4299-4300
+                ayr += (y1 - y2) * t;
             }
         } else {
             // peierlsPhase (locality.cpp:7157-7161) and k.(r2 - r1) (locality.cpp:4896-4905)
@@ -389,6 +392,10 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
             if (tr != 0 || ti != 0) {
                 ar += tr;
                 ai += ti;
+                axr += (x1 - x2) * tr;   // locality.cpp:4939-4940, 5113-5114
+                axi += (x1 - x2) * ti;
+                ayr += (y1 - y2) * tr;
+                ayi += (y1 - y2) * ti;
             }
         }
         keep = keep || support;
@@ -398,15 +405,24 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
             if (CPX) {
                 sval[2 * slot] = ar;
                 sval[2 * slot + 1] = ai;
+                if (sdx) {
+                    sdx[2 * slot] = axr;
+                    sdx[2 * slot + 1] = axi;
+                    sdy[2 * slot] = ayr;
+                    sdy[2 * slot + 1] = ayi;
+                }
             } else {
                 sval[slot] = ar;
-                if (sdx) sdx[slot] = adx;
+                if (sdx) {
+                    sdx[slot] = axr;
+                    sdy[slot] = ayr;
+                }
             }
             skeep[slot] = keep ? 1 : 0;
             ++slot;
             ar = 0.0;
             ai = 0.0;
-            adx = 0.0;
+            axr = axi = ayr = ayi = 0.0;
             keep = false;
         }
     }
@@ -487,15 +503,15 @@ hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipSt
 
 hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
                       const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
-                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, hipStream_t s)
+                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, hipStream_t s)
 {
+    if ((sdx == nullptr) != (sdy == nullptr)) return hipErrorInvalidValue;   // both velocity arrays or neither
     if (j.complex_matrix) {
-        if (sdx) return hipErrorInvalidValue;   // velocity slots are filled for a real H only
         hipLaunchKernelGGL(fill_kernel<true>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep, nullptr);
+                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy);
     } else {
         hipLaunchKernelGGL(fill_kernel<false>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep, sdx);
+                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy);
     }
     return hipGetLastError();
 }

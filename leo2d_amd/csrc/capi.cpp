@@ -101,7 +101,7 @@ struct leo2d_handle {
     DevBuf rp, col, val, val32, i2pos;
     bool val32_valid = false;
     // x-velocity matrix dxH on H's pattern and row order (leo2d_assemble_velocity)
-    DevBuf val_dx;
+    DevBuf val_dx, val_dy;     // velocity matrices on H's pattern, real or complex like H
     bool has_dx = false;
     rocblas_handle blas = nullptr;   // created on the first conductivity call, bound to the stream
     struct CondTiming {
@@ -956,7 +956,6 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
 {
     if (!h || !job || (pattern != LEO2D_PATTERN_REFERENCE && pattern != LEO2D_PATTERN_PRUNED)) return LEO2D_ERR_ARG;
     return guarded(h, [&]() {
-        if (velocity && job->complex_matrix) throw ArgError("the velocity matrix is built for a real H only");
         if (!h->has_geom) throw StateError("leo2d_assemble before leo2d_set_geometry");
         if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
             throw ArgError("energy_rescale must be finite and non-zero");
@@ -972,20 +971,24 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
         if (!h->i2pos.p || h->i2pos.bytes < sizeof(double) * 3 * (size_t)n)
             dalloc(h->i2pos, sizeof(double) * 3 * (size_t)n, "job positions");
         check(leo2d::job_positions(h->gdev, jd, (double*)h->i2pos.p, s), "job positions");
-        DevBuf scol, sval, skeep, sdx;
+        DevBuf scol, sval, skeep, sdx, sdy;
         dalloc(scol, sizeof(int32_t) * (size_t)h->n_slots, "slot columns");
         dalloc(sval, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "slot values");
         dalloc(skeep, (size_t)h->n_slots + 1, "slot flags");
-        if (velocity) dalloc(sdx, sizeof(double) * (size_t)h->n_slots, "velocity slots");
+        if (velocity) {
+            dalloc(sdx, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "x-velocity slots");
+            dalloc(sdy, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "y-velocity slots");
+        }
         check(leo2d::fill_slots(h->gdev, jd, (const double*)h->i2pos.p, (const int64_t*)h->pair_rp.p,
                                 (const int32_t*)h->cnt_intra.p, (const int32_t*)h->pcol.p, (const int32_t*)h->psc.p,
                                 (const int64_t*)h->slot_rp.p, (int32_t*)scol.p, (double*)sval.p, (uint8_t*)skeep.p,
-                                velocity ? (double*)sdx.p : nullptr, s),
+                                velocity ? (double*)sdx.p : nullptr, velocity ? (double*)sdy.p : nullptr, s),
               "hamiltonian fill");
         h->rp.release();
         h->col.release();
         h->val.release();
         h->val_dx.release();
+        h->val_dy.release();
         if (pattern == LEO2D_PATTERN_REFERENCE) {
             dalloc(h->rp, sizeof(int64_t) * (n + 1), "row pointers");
             check(hipMemcpyAsync(h->rp.p, h->slot_rp.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s), "rp");
@@ -993,7 +996,10 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
             h->val = std::move(sval);
             skeep.release();
             set_csr_owned(h, n, h->n_slots, cpx);
-            if (velocity) h->val_dx = std::move(sdx);
+            if (velocity) {
+                h->val_dx = std::move(sdx);
+                h->val_dy = std::move(sdy);
+            }
         } else {
             DevBuf cnt;
             dalloc(cnt, sizeof(int64_t) * n, "prune counts");
@@ -1012,11 +1018,16 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
                   "prune compact");
             if (velocity) {
                 // the same compaction for the velocity slots (columns rewritten identically)
-                dalloc(h->val_dx, sizeof(double) * (size_t)std::max<int64_t>(kept, 1), "velocity values");
-                check(leo2d::prune_compact(n, 0, (const int64_t*)h->slot_rp.p, (const uint8_t*)skeep.p,
-                                           (const int32_t*)scol.p, (const double*)sdx.p, (const int64_t*)h->rp.p,
-                                           (int32_t*)h->col.p, (double*)h->val_dx.p, s),
-                      "prune compact velocity");
+                DevBuf* dst[2] = {&h->val_dx, &h->val_dy};
+                const DevBuf* src[2] = {&sdx, &sdy};
+                for (int c = 0; c < 2; ++c) {
+                    dalloc(*dst[c], sizeof(double) * (cpx ? 2 : 1) * (size_t)std::max<int64_t>(kept, 1),
+                           "velocity values");
+                    check(leo2d::prune_compact(n, cpx, (const int64_t*)h->slot_rp.p, (const uint8_t*)skeep.p,
+                                               (const int32_t*)scol.p, (const double*)src[c]->p,
+                                               (const int64_t*)h->rp.p, (int32_t*)h->col.p, (double*)dst[c]->p, s),
+                          "prune compact velocity");
+                }
             }
             check(hipStreamSynchronize(s), "prune compact");
             cnt.release();
@@ -1024,6 +1035,7 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
             sval.release();
             skeep.release();
             sdx.release();
+            sdy.release();
             set_csr_owned(h, n, kept, cpx);
         }
         check(hipStreamSynchronize(s), "assemble");
@@ -1045,14 +1057,14 @@ int leo2d_assemble_velocity(leo2d_handle* h, const leo2d_job* job, int32_t patte
     return assemble_impl(h, job, pattern, nnz_out, true);
 }
 
-int leo2d_velocity_copy(leo2d_handle* h, double* val_dx)
+int leo2d_velocity_copy(leo2d_handle* h, double* val_dx, double* val_dy)
 {
-    if (!h || !val_dx) return LEO2D_ERR_ARG;
+    if (!h || (!val_dx && !val_dy)) return LEO2D_ERR_ARG;
     return guarded(h, [&]() {
         if (!h->has_csr || !h->has_dx) throw StateError("no velocity matrix: use leo2d_assemble_velocity");
-        if (h->nnz)
-            check(hipMemcpy(val_dx, h->val_dx.p, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost),
-                  "velocity copy");
+        const size_t bytes = sizeof(double) * (h->is_complex ? 2 : 1) * (size_t)h->nnz;
+        if (h->nnz && val_dx) check(hipMemcpy(val_dx, h->val_dx.p, bytes, hipMemcpyDeviceToHost), "velocity copy");
+        if (h->nnz && val_dy) check(hipMemcpy(val_dy, h->val_dy.p, bytes, hipMemcpyDeviceToHost), "velocity copy");
         return LEO2D_OK;
     });
 }
@@ -1545,7 +1557,9 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
                 rho_all.push_back(cm);
                 continue;
             }
-            rc = leo2d_assemble(h, &job, pattern, &nnz);
+            // the last job's velocity matrices feed the MATRIX_SAVE = 2 dumps (H is the same either way)
+            if (q + 1 == jobs.size() && o.geti("matrix_save") > 1) rc = leo2d_assemble_velocity(h, &job, pattern, &nnz);
+            else rc = leo2d_assemble(h, &job, pattern, &nnz);
             if (rc != LEO2D_OK) return rc;
             leo2d_kpm_params kp{};
             kp.M = M;
@@ -1600,8 +1614,25 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
             std::vector<double> vv((size_t)z * (cpx ? 2 : 1));
             rc = leo2d_csr_copy(h, rp.data(), ci.data(), vv.data());
             if (rc != LEO2D_OK) return rc;
-            std::ofstream f(prefix + "_matrix.dat");
-            f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vv.data(), cpx);
+            {
+                std::ofstream f(prefix + "_matrix.dat");
+                f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vv.data(), cpx);
+            }
+            if (o.geti("matrix_save") > 1) {
+                // locality.cpp:4385-4416 (real: dxH and dyH), 5230-5244 (complex: dxH only),
+                // written like _matrix.dat
+                std::vector<double> vx(vv.size()), vy(vv.size());
+                rc = leo2d_velocity_copy(h, vx.data(), vy.data());
+                if (rc != LEO2D_OK) return rc;
+                {
+                    std::ofstream f(prefix + "_dxH_matrix.dat");
+                    f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vx.data(), cpx);
+                }
+                if (!cpx) {
+                    std::ofstream f(prefix + "_dyH_matrix.dat");
+                    f << leo2d::matrix_dat_text(nn, rp.data(), ci.data(), vy.data(), cpx);
+                }
+            }
         }
         if (!jobs.empty() && o.geti("matrix_pos_save") > 0) {
             const int nn = geo.g.n;
@@ -1935,6 +1966,7 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
     if (!h || !p || !C) return LEO2D_ERR_ARG;
     return guarded(h, [&]() {
         if (!h->has_csr || !h->has_dx) throw StateError("leo2d_cond_moments needs leo2d_assemble_velocity");
+        if (h->is_complex) throw ArgError("the conductivity recursion takes a real H");
         const int M = p->M, R = p->R, n = h->n;
         if (M < 2) throw ArgError("M must be at least 2");
         if (R <= 0 || !p->targets) throw ArgError("conductivity needs R >= 1 targets");

@@ -547,7 +547,11 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     int32_t* ci = (int32_t*)malloc(sizeof(int32_t) * (size_t)(maxnnz + 1));
     double* v = (double*)calloc((size_t)(cpx ? 2 : 1) * (size_t)(maxnnz + 1), sizeof(double));
     /* dxH slots (locality.cpp:4182, 4299): (x1 - x2) t accumulated with H */
-    double* vdx = cpx ? NULL : (double*)calloc((size_t)(maxnnz + 1), sizeof(double));
+    /* velocity slots (locality.cpp:4182-4183, 4299-4300 real; 4939-4940, 5113-5114 complex):
+     * dxH, dyH accumulate (x1 - x2) t and (y1 - y2) t on H's slots */
+    const int w = cpx ? 2 : 1;
+    double* vdx = (double*)calloc((size_t)w * (size_t)(maxnnz + 1), sizeof(double));
+    double* vdy = (double*)calloc((size_t)w * (size_t)(maxnnz + 1), sizeof(double));
     char* keep = (char*)calloc((size_t)(maxnnz + 1), 1);
     int64_t ic = 0, xc = 0, input_counter = 0;
     const double resc = job->energy_rescale, eshift = job->energy_shift;
@@ -605,6 +609,7 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                     if (t != 0) {
                         v[input_counter] += t;
                         vdx[input_counter] += (x1 - x2) * t;
+                        vdy[input_counter] += (y1 - y2) * t;
                     }
                 } else {
                     /* peierlsPhase (locality.cpp:7157-7161) + k.(r2 - r1) */
@@ -614,6 +619,10 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                     if (tr != 0 || ti != 0) {
                         v[2 * input_counter] += tr;
                         v[2 * input_counter + 1] += ti;
+                        vdx[2 * input_counter] += (x1 - x2) * tr;
+                        vdx[2 * input_counter + 1] += (x1 - x2) * ti;
+                        vdy[2 * input_counter] += (y1 - y2) * tr;
+                        vdy[2 * input_counter + 1] += (y1 - y2) * ti;
                     }
                 }
                 if (support) keep[input_counter] = 1;
@@ -629,19 +638,22 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     rp[n] = input_counter;
     free(i2pos);
     if (job->pruned) {
-        int64_t w = 0;
+        int64_t wr = 0;
         for (int r = 0; r < n; ++r) {
             int64_t b = rp[r], e = rp[r + 1];
-            rp[r] = w;
+            rp[r] = wr;
             for (int64_t q = b; q < e; ++q) {
                 if (!keep[q]) continue;
-                ci[w] = ci[q];
-                if (cpx) { v[2 * w] = v[2 * q]; v[2 * w + 1] = v[2 * q + 1]; }
-                else { v[w] = v[q]; vdx[w] = vdx[q]; }
-                ++w;
+                ci[wr] = ci[q];
+                for (int c = 0; c < w; ++c) {
+                    v[w * wr + c] = v[w * q + c];
+                    vdx[w * wr + c] = vdx[w * q + c];
+                    vdy[w * wr + c] = vdy[w * q + c];
+                }
+                ++wr;
             }
         }
-        rp[n] = w;
+        rp[n] = wr;
     }
     free(keep);
     out->n = n;
@@ -651,13 +663,14 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     out->val = v;
     out->is_complex = cpx;
     out->val_dx = vdx;
+    out->val_dy = vdy;
     return 0;
 }
 
 void orc_csr_free(orc_csr* h)
 {
-    free(h->row_ptr); free(h->col); free(h->val); free(h->val_dx);
-    h->row_ptr = NULL; h->col = NULL; h->val = NULL; h->val_dx = NULL;
+    free(h->row_ptr); free(h->col); free(h->val); free(h->val_dx); free(h->val_dy);
+    h->row_ptr = NULL; h->col = NULL; h->val = NULL; h->val_dx = NULL; h->val_dy = NULL;
 }
 
 /* ------------------------------------------------------------------ */

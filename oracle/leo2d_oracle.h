@@ -118,7 +118,8 @@ typedef struct {
     int32_t* col;       /* [nnz] */
     double* val;        /* [nnz] real or [2 nnz] complex interleaved */
     int is_complex;
-    double* val_dx;     /* [nnz] x-velocity matrix dxH on the same pattern (real H only; NULL if complex) */
+    double* val_dx;     /* x-velocity matrix dxH on H's pattern, laid out like val */
+    double* val_dy;     /* y-velocity matrix dyH, likewise */
 } orc_csr;
 
 void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos /*[3n]*/);

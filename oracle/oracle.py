@@ -112,6 +112,7 @@ class Csr(C.Structure):
         ("val", C.POINTER(C.c_double)),
         ("is_complex", C.c_int),
         ("val_dx", C.POINTER(C.c_double)),
+        ("val_dy", C.POINTER(C.c_double)),
     ]
 
 
@@ -526,23 +527,31 @@ class Geometry:
         lib().orc_pairs_copy(p, which, _p(ij), _p(sc), _p(t))
         return ij.reshape(-1, 2), sc.reshape(-1, 2), t
 
-    def assemble(self, job: Job, with_dx: bool = False):
-        """CSR of H (rp, ci, val); with_dx also returns the x-velocity matrix
-        dxH on the same pattern (real H only, locality.cpp:4182,4299)."""
+    def assemble(self, job: Job, with_dx: bool = False, with_dy: bool = False):
+        """CSR of H (rp, ci, val); with_dx / with_dy also return the velocity
+        matrices dxH / dyH on the same pattern, real or complex like H
+        (locality.cpp:4182-4183, 4299-4300, 4939-4940, 5113-5114)."""
         csr = Csr()
         lib().orc_assemble(self.h, self.pairs(), C.byref(job), C.byref(csr))
         n, nnz = csr.n, csr.nnz
         rp = np.ctypeslib.as_array(csr.row_ptr, shape=(n + 1,)).copy()
         ci = np.ctypeslib.as_array(csr.col, shape=(max(nnz, 1),))[:nnz].copy()
         w = 2 if csr.is_complex else 1
-        v = np.ctypeslib.as_array(csr.val, shape=(max(w * nnz, 1),))[: w * nnz].copy()
-        dx = None
-        if csr.is_complex:
-            v = v.view(np.complex128)
-        elif with_dx:
-            dx = np.ctypeslib.as_array(csr.val_dx, shape=(max(nnz, 1),))[:nnz].copy()
+
+        def take(ptr):
+            a = np.ctypeslib.as_array(ptr, shape=(max(w * nnz, 1),))[: w * nnz].copy()
+            return a.view(np.complex128) if csr.is_complex else a
+
+        v = take(csr.val)
+        dx = take(csr.val_dx)
+        dy = take(csr.val_dy)
         lib().orc_csr_free(C.byref(csr))
-        return (rp, ci, v, dx) if with_dx else (rp, ci, v)
+        out = (rp, ci, v)
+        if with_dx:
+            out += (dx,)
+        if with_dy:
+            out += (dy,)
+        return out
 
     def job_positions(self, job: Job):
         out = np.zeros(3 * self.n, dtype=np.float64)

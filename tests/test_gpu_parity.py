@@ -186,13 +186,16 @@ def _cheb_numbers(text):
 
 @pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in",
                                   "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose",
-                                  "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in"])
+                                  "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in", "tblg_sc_7_6_k.in:msave2"])
 def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     name, _, variant = name.partition(":")
     text = deck_text(name)
     if variant == "verbose":
         # conductivity with VERBOSE_SAVE = 1: the reference writes the job headers only
         text = text.replace("VERBOSE_SAVE = 0", "VERBOSE_SAVE = 1")
+    if variant == "msave2":
+        # MATRIX_SAVE = 2 on a complex H(k): _dxH_matrix.dat as well, no _dyH_matrix.dat
+        text = text.replace("MATRIX_SAVE = 1", "MATRIX_SAVE = 2")
     prefix = str(tmp_path / "out")
     dev.run_deck(text, prefix)
     got = open(prefix + ".cheb").read()
@@ -208,16 +211,30 @@ def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     else:
         assert not os.path.exists(prefix + ".kpts")
     if od.opts["matrix_save"] > 0:
-        rp, ci, v = og.assemble(oracle_mod.make_job(od, jobs[-1]))
-        assert os.path.exists(prefix + "_matrix.dat")
-        gl = open(prefix + "_matrix.dat").read().splitlines()
-        wl = oracle_mod.matrix_dat_text(rp, ci, v).splitlines()
-        assert len(gl) == len(wl)
+        _check_matrix_files(oracle_mod, od, og, jobs[-1], prefix)
+
+
+def _check_matrix_files(orc, od, og, job, prefix):
+    """_matrix.dat of the last job and, with MATRIX_SAVE = 2, its velocity dumps:
+    _dxH_matrix.dat and (real H) _dyH_matrix.dat (locality.cpp:4366-4416, 5213-5244)."""
+    rp, ci, v, dx, dy = og.assemble(orc.make_job(od, job), with_dx=True, with_dy=True)
+    files = [("_matrix.dat", v)]
+    if od.opts["matrix_save"] > 1:
+        files.append(("_dxH_matrix.dat", dx))
+        if not np.iscomplexobj(v):
+            files.append(("_dyH_matrix.dat", dy))
+        else:
+            assert not os.path.exists(prefix + "_dyH_matrix.dat")
+    for ext, vals in files:
+        assert os.path.exists(prefix + ext), ext
+        gl = open(prefix + ext).read().splitlines()
+        wl = orc.matrix_dat_text(rp, ci, vals).splitlines()
+        assert len(gl) == len(wl), ext
         for a, b in zip(gl, wl):
             ga = [float(x) for x in a.split(",")]
             wa = [float(x) for x in b.split(",")]
             assert ga[:2] == wa[:2]
-            assert np.allclose(ga[2:], wa[2:], rtol=1e-5, atol=1e-6)
+            assert np.allclose(ga[2:], wa[2:], rtol=1e-5, atol=1e-6), ext
 
 
 def test_run_deck_position_files_match_oracle(dev, oracle_mod, tmp_path):
@@ -234,7 +251,8 @@ def test_run_deck_position_files_match_oracle(dev, oracle_mod, tmp_path):
     for which, ext in ((0, "_intra_pos.dat"), (1, "_inter_pos.dat")):
         ij, _, _ = og.pair_arrays(which)
         assert open(prefix + ext).read() == oracle_mod.pair_pos_text(ij, pos), ext
-    assert os.path.exists(prefix + "_matrix.dat")
+    assert od.opts["matrix_save"] == 2
+    _check_matrix_files(oracle_mod, od, og, jobs[-1], prefix)
 
 
 def test_spectrum_and_nan_errors(dev):

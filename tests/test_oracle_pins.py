@@ -541,14 +541,41 @@ def test_velocity_matrix_closed_form(oracle_mod, periodic):
     orc = oracle_mod
     text = W.tblg_cell_deck(3, 2, M=16) if periodic else W.tblg_flake_deck(3.0, 12, M=16)
     deck, geom, jobs = _cond_setup(orc, text)
-    rp, ci, v, dx = geom.assemble(orc.make_job(deck, jobs[0], 1), with_dx=True)
-    assert dx is not None and dx.shape == v.shape
-    H, D = _dense(rp, ci, v), _dense(rp, ci, dx)
-    assert np.max(np.abs(D + D.T)) <= 1e-14 * np.max(np.abs(D))
-    assert np.count_nonzero(dx) > 0.9 * len(dx)
-    if not periodic:
-        x = geom.index_to_pos.reshape(-1, 3)[:, 0]
-        assert np.max(np.abs(D - (x[:, None] - x[None, :]) * H)) == 0.0
+    rp, ci, v, dx, dy = geom.assemble(orc.make_job(deck, jobs[0], 1), with_dx=True, with_dy=True)
+    assert dx.shape == v.shape and dy.shape == v.shape
+    H = _dense(rp, ci, v)
+    for c, vel in enumerate((dx, dy)):
+        D = _dense(rp, ci, vel)
+        assert np.max(np.abs(D + D.T)) <= 1e-14 * np.max(np.abs(D))
+        assert np.count_nonzero(vel) > 0.6 * len(vel)
+        if not periodic:
+            x = geom.index_to_pos.reshape(-1, 3)[:, c]
+            assert np.max(np.abs(D - (x[:, None] - x[None, :]) * H)) == 0.0
+
+
+def test_complex_velocity_is_the_k_derivative(oracle_mod):
+    """H(k)_ij = sum t e^{i k.(r_j - r_i - R)}, so i dH/dk_x = sum (x_i + R_x - x_j) t e^{i phi}
+    = dxH and likewise for y: the complex velocity slots (locality.cpp:4939-4940,
+    5113-5114) against central differences of the oracle's own H(k) on a periodic cell."""
+    from leo2d_amd import workloads as W
+
+    orc = oracle_mod
+    deck, geom, jobs = _cond_setup(orc, W.tblg_cell_deck(3, 2, M=16))
+    k0 = np.array([0.0317, -0.0211])
+
+    def hk(k, vel=False):
+        j = orc.make_job(deck, jobs[0], 0)
+        j.complex_matrix = 1
+        j.k[0], j.k[1] = k
+        out = geom.assemble(j, with_dx=vel, with_dy=vel)
+        return out if vel else out[2]
+
+    rp, ci, v, dx, dy = hk(k0, True)
+    assert np.iscomplexobj(dx) and np.max(np.abs(dx.imag)) > 0
+    h = 1e-5
+    for e, vel in ((np.array([h, 0.0]), dx), (np.array([0.0, h]), dy)):
+        deriv = (hk(k0 + e) - hk(k0 - e)) / (2 * h)
+        assert np.max(np.abs(1j * deriv - vel)) <= 1e-7 * np.max(np.abs(vel))
 
 
 def test_cond_against_dense_eigendecomposition(oracle_mod):
