@@ -53,7 +53,7 @@ EXPORTS = [
     "leo2d_batch_last_timing", "leo2d_mlp_features", "leo2d_mlp_create", "leo2d_mlp_destroy",
     "leo2d_mlp_last_error", "leo2d_mlp_num_params", "leo2d_mlp_get_params", "leo2d_mlp_set_params",
     "leo2d_mlp_forward", "leo2d_mlp_gradients", "leo2d_mlp_train", "leo2d_mlp_ensemble",
-    "leo2d_assemble_velocity", "leo2d_velocity_copy", "leo2d_cond_moments", "leo2d_cond_transform",
+    "leo2d_assemble_velocity", "leo2d_velocity_copy", "leo2d_set_relaxation", "leo2d_read_relaxation", "leo2d_cond_moments", "leo2d_cond_transform",
     "leo2d_cond_last_timing",
 ]
 MLP_FEATURES = 64
@@ -83,6 +83,7 @@ class Job(C.Structure):
         ("complex_matrix", C.c_int32), ("uniform_strain", C.c_int32), ("u", (C.c_double * 2) * 2),
         ("num_targets", C.c_int32), ("targets", C.c_int32 * MAX_TARGETS),
         ("sheet_strain_on", C.c_int32), ("sheet_strain", ((C.c_double * 2) * 2) * MAX_SHEETS),
+        ("strain_type", C.c_int32),
     ]
 
 
@@ -177,6 +178,8 @@ def lib() -> C.CDLL:
         L.leo2d_csr_copy.argtypes = [vp, vp, vp, vp]
         L.leo2d_assemble_velocity.argtypes = [vp, C.POINTER(Job), C.c_int32, C.POINTER(C.c_int64)]
         L.leo2d_velocity_copy.argtypes = [vp, vp, vp]
+        L.leo2d_set_relaxation.argtypes = [vp, C.c_int32, vp]
+        L.leo2d_read_relaxation.argtypes = [C.c_char_p, C.POINTER(C.c_int32), vp, C.POINTER(C.c_int64)]
         L.leo2d_cond_moments.argtypes = [vp, C.POINTER(CondParams), vp]
         L.leo2d_cond_transform.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
         L.leo2d_cond_last_timing.argtypes = [vp, C.POINTER(CondTiming)]
@@ -330,6 +333,11 @@ class Device:
         nnz = C.c_int64()
         self._check(lib().leo2d_assemble_velocity(self.h, C.byref(job), pattern, C.byref(nnz)))
         return nnz.value
+
+    def set_relaxation(self, max_k: int, coeffs=None):
+        """Fourier relaxation field for jobs with strain_type = 1 (coeffs [3][nc]); max_k < 0 clears it."""
+        c = None if coeffs is None else np.ascontiguousarray(coeffs, dtype=np.float64)
+        self._check(lib().leo2d_set_relaxation(self.h, max_k, _p(c) if c is not None else None))
 
     def velocity_copy(self, with_dy: bool = False):
         """dxH (and dyH) values; complex128 arrays for a complex H."""
@@ -487,6 +495,19 @@ class Device:
 
 
 # ---- learned surrogate ------------------------------------------------------
+def read_relaxation(path: str):
+    """max_k and the [3][nc] coefficients of a STRAIN_FILE, read by the library."""
+    mk, cnt = C.c_int32(), C.c_int64()
+    rc = lib().leo2d_read_relaxation(path.encode(), C.byref(mk), None, C.byref(cnt))
+    if rc != OK:
+        raise Leo2dError(rc, "cannot read " + path)
+    c = np.zeros(cnt.value, dtype=np.float64)
+    rc = lib().leo2d_read_relaxation(path.encode(), C.byref(mk), _p(c), C.byref(cnt))
+    if rc != OK:
+        raise Leo2dError(rc, "short relaxation file " + path)
+    return mk.value, c.reshape(3, -1)
+
+
 def mlp_features(cfg) -> np.ndarray:
     """[n][64] Fourier features of configurations [n][6] = (b1, b2, exx, eyy, exy, theta_deg)."""
     c = np.ascontiguousarray(np.atleast_2d(cfg), dtype=np.float64)

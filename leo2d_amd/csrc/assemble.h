@@ -46,15 +46,25 @@ struct JobDev {
     double u[2][2];
     int sheet_strain_on;
     double ss[LEO2D_MAX_SHEETS][2][2];   // per-sheet strain tensors
+    // STRAIN_TYPE = SUPERCELL: Fourier relaxation field (coefficients [3][nc] on the device)
+    int strain_type;
+    const double* relax;
+    int relax_max_k;
+    double rb[2][2];                     // reciprocal vectors of the supercell
+    double rot_cs[3][2];                 // cos, sin of r pi / 3, r = 0, 1, 2 (host libm)
+    double basis_cs[2];                  // cos, sin of the angle of b1 (host libm)
 };
 
 hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total, int64_t* pair_rp, hipStream_t s);
 hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* pcol, int32_t* psc,
                        int64_t* nslots, int64_t* slot_rp, hipStream_t s);
 hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipStream_t s);
+// each orbital's local twist from its nearest neighbour on positions i2pos
+hipError_t local_theta(const GeomDev& g, const double* i2pos, double* theta, hipStream_t s);
 hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
                       const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
-                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, hipStream_t s);
+                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, const double* ltheta,
+                      hipStream_t s);
 hipError_t prune_count(int n, const int64_t* slot_rp, const uint8_t* skeep, int64_t* cnt, int64_t* out_rp,
                        hipStream_t s);
 hipError_t prune_compact(int n, int is_complex, const int64_t* slot_rp, const uint8_t* skeep, const int32_t* scol,

@@ -300,6 +300,38 @@ __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j
     const double sh0 = j.shift[s][0], sh1 = j.shift[s][1];
     double x = g.i2p[3 * (size_t)i + 0] + sh0 * a[0] + sh1 * a[2];
     double y = g.i2p[3 * (size_t)i + 1] + sh0 * a[1] + sh1 * a[3];
+    double z = g.i2p[3 * (size_t)i + 2];
+    if (j.strain_type == 1) {
+        // relaxation field (locality.cpp:3657-3711; StrainCalc::fourierStrainDisp_sc and
+        // fourierStrainDisp, strain.cpp:290-311, 390-460): directions k = p b1 + q b2,
+        // q < max(p, 1), each with its three 60-degree turns (one for the constant term)
+        const int nc = j.relax_max_k * (j.relax_max_k + 1) / 2 + 1;
+        double ux = 0.0, uy = 0.0, uz = 0.0;
+        int idx = 0;
+        for (int p = 0; p < j.relax_max_k + 1; ++p) {
+            for (int q = 0; q < (p > 1 ? p : 1); ++q) {
+                const double ff = p == 0 ? 1.0 : 2.0;
+                const int rot_max = p == 0 ? 1 : 3;
+                const double kb0 = p * j.rb[0][0] + q * j.rb[1][0], kb1 = p * j.rb[0][1] + q * j.rb[1][1];
+                const double cx = j.relax[idx], cy = j.relax[nc + idx], cz = j.relax[2 * nc + idx];
+                for (int r = 0; r < rot_max; ++r) {
+                    const double c = j.rot_cs[r][0], sn = j.rot_cs[r][1];
+                    const double k0 = c * kb0 - sn * kb1, k1 = sn * kb0 + c * kb1;
+                    const double c0 = c * cx - sn * cy, c1 = sn * cx + c * cy;
+                    const double ph = k0 * x + k1 * y;
+                    ux = ux + ff * c0 * sin(ph);
+                    uy = uy + ff * c1 * sin(ph);
+                    uz = uz + ff * cz * cos(ph);
+                }
+                ++idx;
+            }
+        }
+        const double rx = j.basis_cs[0] * ux - j.basis_cs[1] * uy, ry = j.basis_cs[1] * ux + j.basis_cs[0] * uy;
+        const double sign = s == 1 ? -1.0 : 1.0;   // the second sheet moves the other way
+        x = x + sign * rx;
+        y = y + sign * ry;
+        z = z + sign * uz;
+    }
     if (j.sheet_strain_on == 1) {
         // per-sheet strain (open boundaries only), before the uniform strain
         const double tx = x, ty = y;
@@ -313,14 +345,44 @@ __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j
     }
     i2pos[3 * (size_t)i + 0] = x;
     i2pos[3 * (size_t)i + 1] = y;
-    i2pos[3 * (size_t)i + 2] = g.i2p[3 * (size_t)i + 2];
+    i2pos[3 * (size_t)i + 2] = z;
+}
+
+__global__ __launch_bounds__(kThreads) void local_theta_kernel(GeomDev g, const double* i2pos, double* theta)
+{
+    // Locality::getLocalTheta (locality.cpp:665-731): the first nearest neighbour
+    // among the orbitals at index offsets -3, -1, 1, 3 (isNearestNeighbor,
+    // locality.cpp:733-767: same sheet, the other orbital, the same cell or the cell
+    // one step back (from A) / forward (from B) in i or j); the bond angle less
+    // pi/6 (A) or pi/2 (B); the sheet's angle when none is found
+    const int k1 = blockIdx.x * kThreads + threadIdx.x;
+    if (k1 >= g.n) return;
+    const int* a = g.i2g + 4 * (size_t)k1;
+    const int step = a[2] == 0 ? -1 : 1;
+    int k2 = -1;
+    for (int dk = -3; dk < 4 && k2 < 0; dk += 2) {
+        const int t = k1 + dk;
+        if (t < 0 || t >= g.n) continue;
+        const int* c = g.i2g + 4 * (size_t)t;
+        if (c[3] != a[3] || c[2] == a[2]) continue;
+        const int di = c[0] - a[0], dj = c[1] - a[1];
+        if ((di == 0 && dj == 0) || (di == step && dj == 0) || (di == 0 && dj == step)) k2 = t;
+    }
+    double th = g.s[a[3]].angle;
+    if (k2 >= 0) {
+        const double dx = i2pos[3 * (size_t)k2] - i2pos[3 * (size_t)k1];
+        const double dy = i2pos[3 * (size_t)k2 + 1] - i2pos[3 * (size_t)k1 + 1];
+        th = atan2(dy, dx) - (a[2] == 0 ? 0.523598775598298873077 : 1.57079632679489661923);
+    }
+    theta[k1] = th;
 }
 
 template <bool CPX>
 __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, const double* i2pos, const int64_t* pair_rp,
                                                         const int32_t* cnt_intra, const int32_t* pcol,
                                                         const int32_t* psc, const int64_t* slot_rp, int32_t* scol,
-                                                        double* sval, uint8_t* skeep, double* sdx, double* sdy)
+                                                        double* sval, uint8_t* skeep, double* sdx, double* sdy,
+                                                        const double* ltheta)
 {
     const int k_i = blockIdx.x * kThreads + threadIdx.x;
     if (k_i >= g.n) return;
@@ -373,7 +435,10 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
         } else {
             const int s2 = g.i2g[4 * (size_t)new_k + 3];
             const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
-            t = interlayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], dx, dy, dz, g.s[s_i].angle, g.s[s2].angle) / resc;
+            // with a relaxation field, each orbital's local twist (locality.cpp:4261-4269, 5045-5052)
+            const double th1 = ltheta ? ltheta[k_i] : g.s[s_i].angle;
+            const double th2 = ltheta ? ltheta[new_k] : g.s[s2].angle;
+            t = interlayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], dx, dy, dz, th1, th2) / resc;
             support = inter_support(mat, dx, dy, dz);
         }
         if (!CPX) {
@@ -501,17 +566,24 @@ hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipSt
     return hipGetLastError();
 }
 
+hipError_t local_theta(const GeomDev& g, const double* i2pos, double* theta, hipStream_t s)
+{
+    hipLaunchKernelGGL(local_theta_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, i2pos, theta);
+    return hipGetLastError();
+}
+
 hipError_t fill_slots(const GeomDev& g, const JobDev& j, const double* i2pos, const int64_t* pair_rp,
                       const int32_t* cnt_intra, const int32_t* pcol, const int32_t* psc, const int64_t* slot_rp,
-                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, hipStream_t s)
+                      int32_t* scol, double* sval, uint8_t* skeep, double* sdx, double* sdy, const double* ltheta,
+                      hipStream_t s)
 {
     if ((sdx == nullptr) != (sdy == nullptr)) return hipErrorInvalidValue;   // both velocity arrays or neither
     if (j.complex_matrix) {
         hipLaunchKernelGGL(fill_kernel<true>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy);
+                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy, ltheta);
     } else {
         hipLaunchKernelGGL(fill_kernel<false>, rows_grid(g.n), dim3(kThreads), 0, s, g, j, i2pos, pair_rp, cnt_intra,
-                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy);
+                           pcol, psc, slot_rp, scol, sval, skeep, sdx, sdy, ltheta);
     }
     return hipGetLastError();
 }

@@ -103,6 +103,9 @@ struct leo2d_handle {
     // x-velocity matrix dxH on H's pattern and row order (leo2d_assemble_velocity)
     DevBuf val_dx, val_dy;     // velocity matrices on H's pattern, real or complex like H
     bool has_dx = false;
+    // Fourier relaxation field (leo2d_set_relaxation) and the per-orbital local twist
+    DevBuf relax, ltheta;
+    int relax_max_k = -1;
     rocblas_handle blas = nullptr;   // created on the first conductivity call, bound to the stream
     struct CondTiming {
         double recursion_ms = 0.0, gemm_ms = 0.0;
@@ -280,6 +283,20 @@ leo2d::JobDev job_dev(const leo2d::Geom& g, const leo2d_job& j)
     for (int s = 0; s < LEO2D_MAX_SHEETS; ++s)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) d.ss[s][a][b] = j.sheet_strain[s][a][b];
+    d.strain_type = j.strain_type;
+    if (j.strain_type == 1) {
+        // the relaxation field is periodic on the supercell (strain.cpp:390-460): its
+        // directions are the supercell's reciprocal vectors turned by r pi / 3
+        leo2d::reciprocal_vectors(g.gsc, d.rb);
+        for (int r = 0; r < 3; ++r) {
+            const double th = r * 3.14159265358979323846 / 3.0;
+            d.rot_cs[r][0] = std::cos(th);
+            d.rot_cs[r][1] = std::sin(th);
+        }
+        const double rot = std::atan2(d.rb[0][1], d.rb[0][0]);
+        d.basis_cs[0] = std::cos(rot);
+        d.basis_cs[1] = std::sin(rot);
+    }
     return d;
 }
 
@@ -960,6 +977,9 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
         if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
             throw ArgError("energy_rescale must be finite and non-zero");
         if (job->sheet_strain_on && h->geom.bc == 1) throw ArgError("per-sheet strain needs open boundaries");
+        if (job->strain_type != 0 && job->strain_type != 1) throw ArgError("strain_type must be 0 or 1");
+        if (job->strain_type == 1 && h->geom.bc != 1) throw ArgError("the relaxation field needs a periodic supercell");
+        if (job->strain_type == 1 && h->relax_max_k < 0) throw StateError("strain_type 1 before leo2d_set_relaxation");
         if (!h->has_pairs) {
             int rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
             if (rc != LEO2D_OK) return rc;
@@ -970,7 +990,18 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
         leo2d::JobDev jd = job_dev(h->geom, *job);
         if (!h->i2pos.p || h->i2pos.bytes < sizeof(double) * 3 * (size_t)n)
             dalloc(h->i2pos, sizeof(double) * 3 * (size_t)n, "job positions");
+        if (job->strain_type == 1) {
+            jd.relax = (const double*)h->relax.p;
+            jd.relax_max_k = h->relax_max_k;
+        }
         check(leo2d::job_positions(h->gdev, jd, (double*)h->i2pos.p, s), "job positions");
+        const double* lt = nullptr;
+        if (job->strain_type != 0) {
+            if (!h->ltheta.p || h->ltheta.bytes < sizeof(double) * (size_t)std::max(n, 1))
+                dalloc(h->ltheta, sizeof(double) * (size_t)std::max(n, 1), "local twist");
+            check(leo2d::local_theta(h->gdev, (const double*)h->i2pos.p, (double*)h->ltheta.p, s), "local twist");
+            lt = (const double*)h->ltheta.p;
+        }
         DevBuf scol, sval, skeep, sdx, sdy;
         dalloc(scol, sizeof(int32_t) * (size_t)h->n_slots, "slot columns");
         dalloc(sval, sizeof(double) * (cpx ? 2 : 1) * (size_t)h->n_slots, "slot values");
@@ -982,7 +1013,7 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
         check(leo2d::fill_slots(h->gdev, jd, (const double*)h->i2pos.p, (const int64_t*)h->pair_rp.p,
                                 (const int32_t*)h->cnt_intra.p, (const int32_t*)h->pcol.p, (const int32_t*)h->psc.p,
                                 (const int64_t*)h->slot_rp.p, (int32_t*)scol.p, (double*)sval.p, (uint8_t*)skeep.p,
-                                velocity ? (double*)sdx.p : nullptr, velocity ? (double*)sdy.p : nullptr, s),
+                                velocity ? (double*)sdx.p : nullptr, velocity ? (double*)sdy.p : nullptr, lt, s),
               "hamiltonian fill");
         h->rp.release();
         h->col.release();
@@ -1055,6 +1086,46 @@ int leo2d_assemble(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64
 int leo2d_assemble_velocity(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_t* nnz_out)
 {
     return assemble_impl(h, job, pattern, nnz_out, true);
+}
+
+int leo2d_set_relaxation(leo2d_handle* h, int32_t max_k, const double* coeffs)
+{
+    if (!h) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (max_k < 0) {
+            h->relax.release();
+            h->relax_max_k = -1;
+            return LEO2D_OK;
+        }
+        if (!coeffs) throw ArgError("relaxation coefficients missing");
+        if (max_k > 1024) throw ArgError("relaxation max_k above 1024");
+        const size_t count = 3 * ((size_t)max_k * (max_k + 1) / 2 + 1);
+        for (size_t q = 0; q < count; ++q)
+            if (!std::isfinite(coeffs[q])) throw NanError("relaxation coefficient is not finite");
+        check(hipStreamSynchronize(h->stream), "relaxation");
+        dalloc(h->relax, sizeof(double) * count, "relaxation coefficients");
+        check(hipMemcpy(h->relax.p, coeffs, sizeof(double) * count, hipMemcpyHostToDevice), "relaxation upload");
+        h->relax_max_k = max_k;
+        return LEO2D_OK;
+    });
+}
+
+int leo2d_read_relaxation(const char* path, int32_t* max_k, double* coeffs, int64_t* count)
+{
+    // StrainCalc::loadFourierConfigFile (strain.cpp:258-284): max_k, then the u_x, u_y and
+    // u_z rows of max_k (max_k + 1) / 2 + 1 values each, read as whitespace-separated numbers
+    if (!path || !max_k || !count) return LEO2D_ERR_ARG;
+    std::ifstream f(path);
+    if (!f) return LEO2D_ERR_DECK;
+    long long mk = -1;
+    if (!(f >> mk) || mk < 0 || mk > 1024) return LEO2D_ERR_DECK;
+    const int64_t n = 3 * (mk * (mk + 1) / 2 + 1);
+    *max_k = (int32_t)mk;
+    *count = n;
+    if (!coeffs) return LEO2D_OK;
+    for (int64_t q = 0; q < n; ++q)
+        if (!(f >> coeffs[q])) return LEO2D_ERR_DECK;
+    return LEO2D_OK;
 }
 
 int leo2d_velocity_copy(leo2d_handle* h, double* val_dx, double* val_dy)
@@ -1515,6 +1586,20 @@ int leo2d_run_deck(leo2d_handle* h, const char* deck_text, const char* out_prefi
         if (rc != LEO2D_OK) return rc;
         rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
         if (rc != LEO2D_OK) return rc;
+        if (o.geti("strain_type") == 1) {
+            // STRAIN_TYPE = SUPERCELL: StrainCalc::loadFourierConfigFile on STRAIN_FILE (locality.cpp:3593-3597)
+            auto sf = o.s.find("strain_file");
+            if (sf == o.s.end()) throw leo2d::DeckError("STRAIN_TYPE = SUPERCELL needs STRAIN_FILE");
+            int32_t mk = 0;
+            int64_t cnt = 0;
+            if (leo2d_read_relaxation(sf->second.c_str(), &mk, nullptr, &cnt) != LEO2D_OK)
+                throw leo2d::DeckError("cannot read STRAIN_FILE " + sf->second);
+            std::vector<double> coeffs((size_t)cnt);
+            if (leo2d_read_relaxation(sf->second.c_str(), &mk, coeffs.data(), &cnt) != LEO2D_OK)
+                throw leo2d::DeckError("STRAIN_FILE " + sf->second + " is short of coefficients");
+            rc = leo2d_set_relaxation(h, mk, coeffs.data());
+            if (rc != LEO2D_OK) return rc;
+        }
         const int M = (int)o.geti("poly_order");
         const bool trace = o.geti("kpm_trace") == 1;
         int samples = 0;

@@ -215,7 +215,14 @@ Deck parse_deck(const std::string& text)
                 const std::string v = val();
                 if (!v.empty() && v[0] == 'R') o.i["solver_space"] = 0;
                 else if (!v.empty() && v[0] == 'M') o.i["solver_space"] = 1;
-            } else if (key == "UNIFORM_STRAIN") o.i["uniform_strain"] = std::atoi(val().c_str());
+            } else if (key == "STRAIN_TYPE") {
+                // main.cpp:595-614: the value's first letter
+                const std::string v = val();
+                const std::string codes = "NSCRFPD";
+                const size_t c = v.empty() ? std::string::npos : codes.find(v[0]);
+                if (c != std::string::npos) o.i["strain_type"] = (long long)c;
+            } else if (key == "STRAIN_FILE") o.s["strain_file"] = val();
+            else if (key == "UNIFORM_STRAIN") o.i["uniform_strain"] = std::atoi(val().c_str());
             else if (key == "UNIFORM_STRAIN_MAT") {
                 tk.next();
                 std::vector<std::vector<double>> u(2, std::vector<double>(2, 0.0));

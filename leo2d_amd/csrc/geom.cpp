@@ -59,7 +59,11 @@ Geom build_geometry(const Deck& d)
     Geom g;
     const Params& o = d.opts;
     if (o.geti("solver_space") != 0) throw DeckError("momentum-space solver is not on this path");
-    if (o.geti("strain_type") != 0) throw DeckError("STRAIN_TYPE other than NONE is not on this path");
+    if (o.geti("strain_type") != 0 && o.geti("strain_type") != 1)
+        throw DeckError("STRAIN_TYPE " + std::to_string(o.geti("strain_type")) +
+                        " is not on this path (NONE and SUPERCELL are)");
+    if (o.geti("strain_type") == 1 && o.geti("boundary_condition") != 1)
+        throw DeckError("STRAIN_TYPE = SUPERCELL needs a periodic supercell");
     if (o.geti("mat_from_file") != 0) throw DeckError("MAT_FROM_FILE is not on this path");
     g.num_sheets = (int)d.sheets.size();
     g.bc = (int)o.geti("boundary_condition");
@@ -350,6 +354,13 @@ std::vector<KVec> k_path(const Deck& d)
 
 }  // namespace
 
+void reciprocal_vectors(const double a[2][2], double b[2][2])
+{
+    const auto r = reciprocal({{a[0][0], a[0][1]}, {a[1][0], a[1][1]}});
+    for (int i = 0; i < 2; ++i)
+        for (int k = 0; k < 2; ++k) b[i][k] = r[i][k];
+}
+
 std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
 {
     const Params& o = d.opts;
@@ -375,6 +386,7 @@ std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) proto.u[a][b] = it->second[a][b];
     }
+    proto.strain_type = (int)o.geti("strain_type");
     proto.num_targets = (int)targets.size();
     for (size_t t = 0; t < targets.size(); ++t) proto.targets[t] = targets[t];
     if (auto hs = o.dm.find("heterostrain"); hs != o.dm.end()) {

@@ -76,6 +76,8 @@ struct Geom {
 };
 
 Geom build_geometry(const Deck& d);
+// Locality::getReciprocal(a) (locality.cpp:7215-7262): b_i . a_j = 2 pi delta_ij
+void reciprocal_vectors(const double a[2][2], double b[2][2]);
 std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g);
 std::vector<int> target_list(const Deck& d, const Geom& g);
 

@@ -21,6 +21,10 @@
 #define N_PI_6 0.523598775598298873077
 #define N_SQRT3_2 0.866025403784438646764
 #define GR_ACC 1.429981146728865 /* a / sqrt(3), the unstrained nearest-neighbour bond (A) */
+/* tools/numbers.h */
+#define ORC_PI 3.14159265358979323846
+#define ORC_PI_2 1.57079632679489661923
+#define ORC_PI_6 0.523598775598298873077
 #define N_SQRT3_6 0.288675134594812882255
 
 /* graphene geometry: src/materials/graphene.h:78-95 */
@@ -486,6 +490,69 @@ void orc_pairs_copy(const orc_pairs* p, int which, int32_t* ij, int32_t* sc, dou
 /* ------------------------------------------------------------------ */
 /* assembly                                                             */
 
+void orc_relax_disp(const orc_job* job, double x, double y, int s, double* d)
+{
+    /* sum over Fourier directions k = i b1 + j b2 (j < max(i, 1)), each with its
+     * three 60-degree rotations (one for the constant term), coefficients in
+     * the supercell basis; then turned by the angle of b1 (strain.cpp:390-460)
+     * and given the sheet's sign: +1, or -1 on sheet 2 (strain.cpp:290-311). */
+    const double(*b)[2] = job->relax_b;
+    const int nc = job->relax_max_k * (job->relax_max_k + 1) / 2 + 1;
+    const double* cx = job->relax;
+    const double* cy = job->relax + nc;
+    const double* cz = job->relax + 2 * nc;
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    int idx = 0;
+    for (int i = 0; i < job->relax_max_k + 1; ++i) {
+        for (int j = 0; j < (i > 1 ? i : 1); ++j) {
+            const double ff = i == 0 ? 1.0 : 2.0;
+            const int rot_max = i == 0 ? 1 : 3;
+            const double kb0 = i * b[0][0] + j * b[1][0], kb1 = i * b[0][1] + j * b[1][1];
+            for (int r = 0; r < rot_max; ++r) {
+                const double th = r * ORC_PI / 3.0;
+                const double k0 = cos(th) * kb0 - sin(th) * kb1, k1 = sin(th) * kb0 + cos(th) * kb1;
+                const double c0 = cos(th) * cx[idx] - sin(th) * cy[idx], c1 = sin(th) * cx[idx] + cos(th) * cy[idx];
+                ux = ux + ff * c0 * sin(k0 * x + k1 * y);
+                uy = uy + ff * c1 * sin(k0 * x + k1 * y);
+                uz = uz + ff * cz[idx] * cos(k0 * x + k1 * y);
+            }
+            ++idx;
+        }
+    }
+    const double rot = atan2(b[0][1], b[0][0]);
+    const double rx = cos(rot) * ux - sin(rot) * uy, ry = sin(rot) * ux + cos(rot) * uy;
+    const double sign = s == 1 ? -1.0 : 1.0;
+    d[0] = sign * rx;
+    d[1] = sign * ry;
+    d[2] = sign * uz;
+}
+
+void orc_local_theta(const orc_geom* g, const double* i2pos, double* theta)
+{
+    for (int k1 = 0; k1 < g->n; ++k1) {
+        const int* a = g->i2g + 4 * (size_t)k1;
+        int k2 = -1;
+        for (int dk = -3; dk < 4 && k2 < 0; dk += 2) {
+            const int t = k1 + dk;
+            if (t < 0 || t >= g->n) continue;
+            const int* c = g->i2g + 4 * (size_t)t;
+            /* isNearestNeighbor (locality.cpp:733-767): same sheet, the other
+             * orbital, same cell or the cell at -1 (from A) / +1 (from B) in i or j */
+            if (c[3] != a[3] || c[2] == a[2]) continue;
+            const int di = c[0] - a[0], dj = c[1] - a[1];
+            const int sgn = a[2] == 0 ? -1 : 1;
+            if ((di == 0 && dj == 0) || (di == sgn && dj == 0) || (di == 0 && dj == sgn)) k2 = t;
+        }
+        if (k2 >= 0) {
+            const double dx = i2pos[3 * (size_t)k2] - i2pos[3 * (size_t)k1];
+            const double dy = i2pos[3 * (size_t)k2 + 1] - i2pos[3 * (size_t)k1 + 1];
+            theta[k1] = atan2(dy, dx) - (a[2] == 0 ? ORC_PI_6 : ORC_PI_2);
+        } else {
+            theta[k1] = g->d.sheet[a[3]].angle;
+        }
+    }
+}
+
 void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
 {
     /* Locality::setConfigPositions (locality.cpp:3618-3643, 3849-3857) */
@@ -501,6 +568,14 @@ void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
         i2pos[i * 3 + 0] = g->i2p[i * 3 + 0] + sh[0] * s1_a[0][0] + sh[1] * s1_a[1][0];
         i2pos[i * 3 + 1] = g->i2p[i * 3 + 1] + sh[0] * s1_a[0][1] + sh[1] * s1_a[1][1];
         i2pos[i * 3 + 2] = g->i2p[i * 3 + 2];
+        if (job->strain_type == 1) {
+            /* locality.cpp:3657-3711 */
+            double d[3];
+            orc_relax_disp(job, i2pos[i * 3 + 0], i2pos[i * 3 + 1], s, d);
+            i2pos[i * 3 + 0] = i2pos[i * 3 + 0] + d[0];
+            i2pos[i * 3 + 1] = i2pos[i * 3 + 1] + d[1];
+            i2pos[i * 3 + 2] = i2pos[i * 3 + 2] + d[2];
+        }
         if (job->sheet_strain_on == 1) {
             const double(*e)[2] = job->sheet_strain[s];
             double tx = i2pos[i * 3 + 0], ty = i2pos[i * 3 + 1];
@@ -533,6 +608,13 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     int cpx = job->complex_matrix;
     double* i2pos = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
     orc_job_positions(g, job, i2pos);
+    /* with a relaxation field the interlayer term takes each orbital's local twist
+     * (locality.cpp:4261-4269, 5045-5052) */
+    double* ltheta = NULL;
+    if (job->strain_type != 0) {
+        ltheta = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+        orc_local_theta(g, i2pos, ltheta);
+    }
     double sc[2][2];
     memcpy(sc, g->d.supercell, sizeof(sc));
     if (g->d.boundary_condition == 1 && job->uniform_strain == 1) {
@@ -600,7 +682,8 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                 } else {
                     int o1 = g->i2g[4 * (size_t)k_i + 2], o2 = g->i2g[4 * (size_t)new_k + 2];
                     int s2 = g->i2g[4 * (size_t)new_k + 3];
-                    double th1 = g->d.sheet[s_i].angle, th2 = g->d.sheet[s2].angle;
+                    double th1 = ltheta ? ltheta[k_i] : g->d.sheet[s_i].angle;
+                    double th2 = ltheta ? ltheta[new_k] : g->d.sheet[s2].angle;
                     double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
                     t = orc_interlayer(mat, o1, o2, dx, dy, dz, th1, th2) / resc;
                     support = inter_in_support(mat, dx, dy, dz);
@@ -637,6 +720,7 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
     }
     rp[n] = input_counter;
     free(i2pos);
+    free(ltheta);
     if (job->pruned) {
         int64_t wr = 0;
         for (int r = 0; r < n; ++r) {

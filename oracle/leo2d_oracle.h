@@ -109,7 +109,19 @@ typedef struct {
     int pruned;                         /* 0 reference-exact slots, 1 drop zero slots */
     int sheet_strain_on;                /* per-sheet strain (open boundaries), before u */
     double sheet_strain[ORC_MAX_SHEETS][2][2];
+    /* STRAIN_TYPE = SUPERCELL (1): Fourier relaxation field and local twist angles */
+    int strain_type;
+    int relax_max_k;
+    const double* relax;                /* [3][relax_max_k (relax_max_k + 1) / 2 + 1] u_x, u_y, u_z */
+    double relax_b[2][2];               /* reciprocal vectors of the supercell */
 } orc_job;
+
+/* StrainCalc::fourierStrainDisp_sc (strain.cpp:290-311, 390-460): relaxation
+ * displacement of an orbital of sheet s at in-plane position r. */
+void orc_relax_disp(const orc_job* job, double x, double y, int s, double* d /*[3]*/);
+/* Locality::getLocalTheta (locality.cpp:665-731) on positions i2pos: the
+ * sheet's local twist from an orbital's nearest neighbour, for every orbital. */
+void orc_local_theta(const orc_geom* g, const double* i2pos, double* theta /*[n]*/);
 
 typedef struct {
     int n;

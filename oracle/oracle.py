@@ -7,6 +7,7 @@ Pieces restated here (the numeric core is leo2d_oracle.c):
   * deck parsing          app/src/main.cpp:75-937 (token rules and quirks)
   * Job_params defaults   src/params/job_params.cpp:18-156
   * supercell setup       src/locality.cpp:131-400 (types 1 and 2)
+  * relaxation field      src/geom/strain.cpp:258-311, 390-460 (STRAIN_TYPE = SUPERCELL)
   * job generation        src/locality.cpp:1222-1388 (SQ, LC), 1615-2249 (K_TYPE 0-5)
   * targets               src/geom/hstruct.cpp:1049-1080
   * .cheb writer          src/params/param_tools.cpp:19-285, 467-577
@@ -100,6 +101,10 @@ class Job(C.Structure):
         ("pruned", C.c_int),
         ("sheet_strain_on", C.c_int),
         ("sheet_strain", ((C.c_double * 2) * 2) * MAX_SHEETS),
+        ("strain_type", C.c_int),
+        ("relax_max_k", C.c_int),
+        ("relax", C.POINTER(C.c_double)),
+        ("relax_b", (C.c_double * 2) * 2),
     ]
 
 
@@ -144,6 +149,8 @@ def lib():
         L.orc_pairs_count.argtypes = [vp, C.c_int]
         L.orc_pairs_copy.argtypes = [vp, C.c_int, vp, vp, vp]
         L.orc_job_positions.argtypes = [vp, C.POINTER(Job), vp]
+        L.orc_relax_disp.argtypes = [C.POINTER(Job), C.c_double, C.c_double, C.c_int, vp]
+        L.orc_local_theta.argtypes = [vp, vp, vp]
         L.orc_assemble.argtypes = [vp, vp, C.POINTER(Job), C.POINTER(Csr)]
         L.orc_csr_free.argtypes = [C.POINTER(Csr)]
         L.orc_kpm.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
@@ -350,6 +357,14 @@ def parse_deck(text: str) -> Deck:
                     opts["solver_space"] = 0
                 elif v[:1] == "M":
                     opts["solver_space"] = 1
+            elif tok == "STRAIN_TYPE":
+                # main.cpp:595-614: the value's first letter
+                nxt(); v = nxt()
+                codes = {"N": 0, "S": 1, "C": 2, "R": 3, "F": 4, "P": 5, "D": 6}
+                if v[:1] in codes:
+                    opts["strain_type"] = codes[v[:1]]
+            elif tok == "STRAIN_FILE":
+                nxt(); opts["strain_file"] = nxt()
             elif tok == "UNIFORM_STRAIN":
                 nxt(); opts["uniform_strain"] = atoi(nxt())
             elif tok == "UNIFORM_STRAIN_MAT":
@@ -556,6 +571,11 @@ class Geometry:
     def job_positions(self, job: Job):
         out = np.zeros(3 * self.n, dtype=np.float64)
         lib().orc_job_positions(self.h, C.byref(job), _p(out))
+        return out
+
+    def local_theta(self, i2pos: np.ndarray):
+        out = np.zeros(self.n, dtype=np.float64)
+        lib().orc_local_theta(self.h, _p(np.ascontiguousarray(i2pos, dtype=np.float64)), _p(out))
         return out
 
 
@@ -860,7 +880,46 @@ def make_job(deck: Deck, job: dict, pruned: int = 0) -> Job:
         j.sheet_strain_on = 1
         j.sheet_strain[top][0][0] = eps
         j.sheet_strain[top][1][1] = -nu * eps
+    st = o.get("strain_type", 0)
+    if st == 1:
+        if o["boundary_condition"] != 1:
+            raise DeckError("STRAIN_TYPE = SUPERCELL needs a periodic supercell")
+        max_k, coeffs = read_relaxation(o["strain_file"])
+        set_relaxation(j, max_k, coeffs, reciprocal(o["supercell"]))
+    elif st != 0:
+        raise DeckError(f"STRAIN_TYPE {st} is not on this path (NONE and SUPERCELL are)")
     return j
+
+
+def read_relaxation(path: str):
+    """StrainCalc::loadFourierConfigFile (strain.cpp:258-284): max_k, then the
+    u_x, u_y and u_z coefficient rows of max_k (max_k + 1) / 2 + 1 values each,
+    whitespace separated."""
+    with open(path) as f:
+        toks = f.read().split()
+    max_k = int(toks[0])
+    nc = max_k * (max_k + 1) // 2 + 1
+    if len(toks) < 1 + 3 * nc:
+        raise DeckError(f"relaxation file {path}: {3 * nc} coefficients expected")
+    return max_k, np.array([float(t) for t in toks[1:1 + 3 * nc]]).reshape(3, nc)
+
+
+def set_relaxation(j: Job, max_k: int, coeffs: np.ndarray, b) -> None:
+    """Attach the Fourier relaxation field (STRAIN_TYPE = SUPERCELL) to a job."""
+    c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(3, -1)
+    j._relax_keep = c
+    j.strain_type = 1
+    j.relax_max_k = max_k
+    j.relax = c.ctypes.data_as(C.POINTER(C.c_double))
+    for a in range(2):
+        for d in range(2):
+            j.relax_b[a][d] = b[a][d]
+
+
+def relax_disp(j: Job, x: float, y: float, s: int):
+    d = np.zeros(3)
+    lib().orc_relax_disp(C.byref(j), C.c_double(x), C.c_double(y), s, _p(d))
+    return d
 
 
 # --------------------------------------------------------------------------

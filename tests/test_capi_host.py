@@ -18,7 +18,8 @@ DECKS = os.path.join(ROOT, "tests", "decks")
 def deck_text(name):
     if name.endswith(".in") and os.path.exists(golden_path(name)):
         return open(golden_path(name)).read()
-    return open(os.path.join(DECKS, name)).read()
+    # STRAIN_FILE paths in the decks are relative to the repository root
+    return open(os.path.join(DECKS, name)).read().replace("STRAIN_FILE = tests/", "STRAIN_FILE = %s/tests/" % ROOT)
 
 
 def test_exports_every_declared_symbol():
@@ -33,7 +34,7 @@ def test_exports_every_declared_symbol():
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
              "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
-             "ttlg_grid_k5.in"]
+             "ttlg_grid_k5.in", "tblg_relax_cell.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -73,6 +74,7 @@ def test_jobs_match_oracle(oracle_mod, name):
         if "k_vec" in oj:
             assert (j.k[0], j.k[1]) == tuple(oj["k_vec"])
         assert j.complex_matrix == orc.complex_matrix(od.opts)
+        assert j.strain_type == od.opts["strain_type"]
         assert j.energy_rescale == od.opts["energy_rescale"]
         oj_struct = orc.make_job(od, oj)
         assert j.sheet_strain_on == oj_struct.sheet_strain_on
@@ -136,6 +138,37 @@ def test_k_path_errors(oracle_mod, name, k_type, edits):
     with pytest.raises(L.Leo2dError) as e:
         L.Geometry(L.Deck(text)).jobs()
     assert e.value.code == L.ERR_DECK
+
+
+def test_relaxation_file_matches_oracle(oracle_mod, tmp_path):
+    """leo2d_read_relaxation (StrainCalc::loadFourierConfigFile, strain.cpp:258-284)
+    reads what the oracle reads; a missing or short file is LEO2D_ERR_DECK."""
+    path = os.path.join(DECKS, "tblg_relax_coeffs.txt")
+    mk, c = L.read_relaxation(path)
+    omk, oc = oracle_mod.read_relaxation(path)
+    assert mk == omk == 3 and c.shape == (3, 7) and np.array_equal(c, oc)
+    short = tmp_path / "short.txt"
+    short.write_text("2\n0.1 0.2 0.3\n")
+    for bad in (str(short), str(tmp_path / "missing.txt")):
+        with pytest.raises(L.Leo2dError) as e:
+            L.read_relaxation(bad)
+        assert e.value.code == L.ERR_DECK
+
+
+@pytest.mark.parametrize("edit", [
+    ("STRAIN_TYPE = SUPERCELL", "STRAIN_TYPE = CONFIGURATION"),     # strain sampled from configurations
+    ("STRAIN_TYPE = SUPERCELL", "STRAIN_TYPE = REALSPACE"),
+    ("BOUNDARY_CONDITION = 1", "BOUNDARY_CONDITION = 0"),           # the field needs a supercell
+])
+def test_relaxation_deck_errors(oracle_mod, edit):
+    text = deck_text("tblg_relax_cell.in").replace(*edit).replace("K_SAMPLING = 1", "K_SAMPLING = 0")
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(text))
+    assert e.value.code == L.ERR_DECK
+    od = oracle_mod.parse_deck(text)
+    oracle_mod.setup_supercell(od)
+    with pytest.raises(oracle_mod.DeckError):
+        oracle_mod.make_job(od, {"shifts": [[0.0, 0.0, 0.0]] * 2})
 
 
 def test_commensurate_cell_count():

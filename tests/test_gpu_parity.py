@@ -8,7 +8,9 @@ Bars (stated per test):
   * mixed precision (FP32 recursion): |dmu| <= 2e-4 mu_0, max |dDoS| reported;
   * DoS: |drho| <= 1e-11 (1 + |rho|).
 """
+import io
 import os
+import re
 
 import numpy as np
 import pytest
@@ -22,12 +24,15 @@ pytestmark = pytest.mark.gpu
 
 DECKS = os.path.join(ROOT, "tests", "decks")
 PARITY_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_gshift_strain.in",
-                "tblg_simple_flake.in", "tblg_c0_twin.in", "tblg_strain_flake.in", "tblg_trace.in"]
+                "tblg_simple_flake.in", "tblg_c0_twin.in", "tblg_strain_flake.in", "tblg_trace.in", "tblg_relax_cell.in"]
 
 
 def deck_text(name):
     p = golden_path(name)
-    return open(p).read() if os.path.exists(p) else open(os.path.join(DECKS, name)).read()
+    if os.path.exists(p):
+        return open(p).read()
+    # STRAIN_FILE paths in the decks are relative to the repository root
+    return open(os.path.join(DECKS, name)).read().replace("STRAIN_FILE = tests/", "STRAIN_FILE = %s/tests/" % ROOT)
 
 
 @pytest.fixture(scope="module")
@@ -50,6 +55,9 @@ def _setup(dev, text):
     geom = L.Geometry(deck)
     dev.set_geometry(geom)
     counts = dev.pairs_build()
+    m = re.search(r"STRAIN_FILE = (\S+)", text)
+    if deck.get_int("strain_type") == 1 and m:
+        dev.set_relaxation(*L.read_relaxation(m.group(1)))   # StrainCalc::loadFourierConfigFile
     return deck, geom, counts
 
 
@@ -186,7 +194,8 @@ def _cheb_numbers(text):
 
 @pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in",
                                   "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose",
-                                  "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in", "tblg_sc_7_6_k.in:msave2"])
+                                  "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in", "tblg_sc_7_6_k.in:msave2",
+                                  "tblg_relax_cell.in"])
 def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     name, _, variant = name.partition(":")
     text = deck_text(name)
@@ -205,6 +214,14 @@ def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     wn, ws = _cheb_numbers(want)
     assert gs == ws                                   # identical layout and integers
     assert np.max(np.abs(np.array(gn) - np.array(wn))) <= 1.0000001e-6   # 6-decimal rounding
+    if od.opts["matrix_pos_save"] > 0:
+        # positions of the last job, relaxation included: 12 decimals, the last one may
+        # differ where device and host trigonometry differ by an ulp
+        pos = og.job_positions(oracle_mod.make_job(od, jobs[-1]))
+        got = np.loadtxt(prefix + "_pos.dat").reshape(-1, 5)
+        want = np.loadtxt(io.StringIO(oracle_mod.pos_dat_text(pos, og.index_to_grid))).reshape(-1, 5)
+        assert np.array_equal(got[:, 3:], want[:, 3:])
+        assert np.max(np.abs(got[:, :3] - want[:, :3])) <= 2e-12
     if od.opts["k_sampling"] == 1:
         # Bloch vector of every job, byte for byte (param_tools.cpp:579-602)
         assert open(prefix + ".kpts").read() == oracle_mod.kpts_text(jobs)
