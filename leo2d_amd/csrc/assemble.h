@@ -20,10 +20,12 @@ struct SheetDev {
     int st[2][2];          // supercell stride in unit cells
     double angle;
     double cneg, sneg;     // cos(-angle), sin(-angle) from the host libm
+    int win;               // intralayer search window (cells each way)
 };
 
 struct GeomDev {
     int n, num_sheets, norb, bc, sc_search;
+    int inter_win;         // interlayer search window, from sheet 1's material
     double gsc[2][2];
     double ainv[2][2];
     double lat[2][2];

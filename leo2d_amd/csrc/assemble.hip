@@ -47,6 +47,16 @@ __device__ inline double intralayer(int mat, int orb_row, int orb_col, int v0, i
     if (orb_col == 0 && orb_row == 1) h0 -= 1;
     else if (orb_col == 1 && orb_row == 0) h0 += 1;
     const int r = h0 * (h0 + h1) + h1 * h1;
+    if (mat == MAT_STRAINED_GRAPHENE) {
+        // Coupling::Intralayer::strained_graphene, unstrained (strained_graphene.cpp:16-91)
+        switch (r) {
+        case 0: return 0.7833;
+        case 1: return -2.8219;
+        case 3: return 0.2543;
+        case 4: return -0.1803;
+        default: return 0.0;
+        }
+    }
     if (mat != MAT_GRAPHENE) return r == 1 ? -2.7 : 0.0;
     switch (r) {
     case 0: return 0.3504;
@@ -58,6 +68,32 @@ __device__ inline double intralayer(int mat, int orb_row, int orb_col, int v0, i
     case 12: return -0.0209;
     case 13: return -0.0148;
     case 16: return -0.0211;
+    default: return 0.0;
+    }
+}
+
+// Coupling::Intralayer::strained_graphene with hop direction (hx, hy), strain u and
+// sheet angle cos/sin (strained_graphene.cpp:300-396).  The reference writes its
+// vector-like strain pair's first slot twice, so the pair is (u_xy + u_yx, 0).
+__device__ inline double strained_intra(int orb_row, int orb_col, int v0, int v1, double hx, double hy,
+                                        const double (&u)[2][2], double c, double s)
+{
+    int h0 = 2 * v0 + v1;
+    const int h1 = -1 * v0 + v1;
+    if (orb_col == 0 && orb_row == 1) h0 -= 1;
+    else if (orb_col == 1 && orb_row == 0) h0 += 1;
+    const int r = h0 * (h0 + h1) + h1 * h1;
+    const double onedim = u[0][0] + u[1][1];
+    const double two0 = u[0][1] + u[1][0], two1 = 0.0;
+    const double w0 = c * hx + s * hy;
+    const double w1 = -s * hx + c * hy;
+    // unperturbed hopping t, scalar-strain a and vector-strain b of shells 0, 1, 3, 4
+    // (strained_graphene.cpp:81-88)
+    switch (r) {
+    case 0: return 0.7833 + -4.8782 * onedim;
+    case 1: return -2.8219 + 4.0066 * onedim + -3.0868 * (w0 * two0 + w1 * two1);
+    case 3: return 0.2543 + -0.4633 * onedim + 0.8017 * (-w1 * two0 + w0 * two1);
+    case 4: return -0.1803 + 0.6236 * onedim + 0.4793 * (w0 * two0 + w1 * two1);
     default: return 0.0;
     }
 }
@@ -108,7 +144,7 @@ __device__ double c_to_c(int orb_row, int orb_col, double v0, double v1, double 
 
 __device__ inline double interlayer(int mat, int o1, int o2, double dx, double dy, double dz, double th1, double th2)
 {
-    if (mat != MAT_GRAPHENE) {
+    if (mat != MAT_GRAPHENE && mat != MAT_STRAINED_GRAPHENE) {
         // t_perp(r) = 0.48 exp(-(r - 3.35)/0.45) eV for 3D distance r < 6 A
         const double r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 < 36.0)) return 0.0;
@@ -121,7 +157,7 @@ __device__ inline double interlayer(int mat, int o1, int o2, double dx, double d
 // support of a coupling (pruned pattern): decided by exact comparisons only
 __device__ inline bool inter_support(int mat, double dx, double dy, double dz)
 {
-    if (mat != MAT_GRAPHENE) return dx * dx + dy * dy + dz * dz < 36.0;
+    if (mat != MAT_GRAPHENE && mat != MAT_STRAINED_GRAPHENE) return dx * dx + dy * dy + dz * dz < 36.0;
     if (iabs_trunc(dz) > 5) return false;
     return dx * dx + dy * dy < 64.0;
 }
@@ -174,8 +210,9 @@ __device__ void search_intra(const GeomDev& g, int kh, PairSink& out)
             const double p0 = kp0 + scd0, p1 = kp1 + scd1;
             const int i0 = clamp_cell(g.ainv[0][0] * p0 + g.ainv[0][1] * p1, sd.min0, sd.max0);
             const int j0 = clamp_cell(g.ainv[1][0] * p0 + g.ainv[1][1] * p1, sd.min1, sd.max1);
-            for (int i = i0 - 4; i < i0 + 4; ++i)
-                for (int j = j0 - 4; j < j0 + 4; ++j)
+            const int W = sd.win;
+            for (int i = i0 - W; i < i0 + W; ++i)
+                for (int j = j0 - W; j < j0 + W; ++j)
                     for (int l = 0; l < g.norb; ++l) {
                         const int k2 = grid_at(sd, g.norb, i, j, l);
                         if (k2 == -1) continue;
@@ -221,8 +258,8 @@ __device__ void search_inter(const GeomDev& g, int kh, PairSink& out)
                 const double y_new1 = p1 * t.cneg + p0 * t.sneg + zero * g.lat[0][1] + zero * g.lat[1][1];
                 const int i0 = clamp_cell(g.ainv[0][0] * x_new + g.ainv[0][1] * y_new0, t.min0, t.max0);
                 const int j0 = clamp_cell(g.ainv[1][0] * x_new + g.ainv[1][1] * y_new1, t.min1, t.max1);
-                const int ilo = max(0, i0 - 6), ihi = min(t.H, i0 + 6);
-                const int jlo = max(0, j0 - 6), jhi = min(t.W, j0 + 6);
+                const int ilo = max(0, i0 - g.inter_win), ihi = min(t.H, i0 + g.inter_win);
+                const int jlo = max(0, j0 - g.inter_win), jhi = min(t.W, j0 + g.inter_win);
                 for (int i = ilo; i < ihi; ++i)
                     for (int j = jlo; j < jhi; ++j)
                         for (int l = 0; l < g.norb; ++l) {
@@ -420,6 +457,19 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
                 gd1 = gd1 - sc_i * sd.st[0][1] - sc_j * sd.st[1][1];
             }
             double raw_t = intralayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1);
+            if (mat == MAT_STRAINED_GRAPHENE && (j.strain_type == 1 || j.uniform_strain == 1)) {
+                // strained intralayer term (locality.cpp:4034-4160, 4744-4870): the hop's direction on
+                // the job's positions, the uniform strain (zero for the relaxation field), the sheet angle
+                double hx = x2 - x1, hy = y2 - y1;
+                const double hn = sqrt(hx * hx + hy * hy);
+                if (hn != 0) {
+                    hx = hx / hn;
+                    hy = hy / hn;
+                }
+                const double zero_u[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                raw_t = strained_intra(l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1, hx, hy,
+                                       j.uniform_strain == 1 ? j.u : zero_u, sd.cneg, -sd.sneg);
+            }
             if (mat == MAT_GRAPHENE_SIMPLE_STRAIN && raw_t != 0) {
                 const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
                 raw_t = raw_t * exp(-kStrainBeta * (sqrt(dx * dx + dy * dy + dz * dz) / kAcc - 1.0));

@@ -824,6 +824,7 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
         d.norb = g.norb;
         d.bc = g.bc;
         d.sc_search = g.sc_search;
+        d.inter_win = leo2d::inter_window(g.sheets[0].mat);   // hstruct.cpp:305, sheet 1's material
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) {
                 d.gsc[a][b] = g.gsc[a][b];
@@ -844,6 +845,7 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
             sd.max0 = src.max_shape[0];
             sd.max1 = src.max_shape[1];
             sd.mat = src.mat;
+            sd.win = leo2d::intra_window(src.mat);   // sheet.cpp:39, 626
             sd.bc = src.boundary_condition;
             for (int a = 0; a < 2; ++a)
                 for (int b = 0; b < 2; ++b) {
@@ -959,7 +961,12 @@ int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, d
                     static const double tab[17] = {0.3504, -2.8922, 0, 0.2425, -0.2656, 0, 0, 0.0235, 0, 0.0524,
                                                    0, 0, -0.0209, -0.0148, 0, 0, -0.0211};
                     double v = (rr >= 0 && rr <= 16) ? tab[rr] : 0.0;
-                    if (sd.mat != leo2d::MAT_GRAPHENE) v = rr == 1 ? -2.7 : 0.0;
+                    if (sd.mat == leo2d::MAT_STRAINED_GRAPHENE) {
+                        // strained_graphene.cpp:16-91
+                        v = rr == 0 ? 0.7833 : rr == 1 ? -2.8219 : rr == 3 ? 0.2543 : rr == 4 ? -0.1803 : 0.0;
+                    } else if (sd.mat != leo2d::MAT_GRAPHENE) {
+                        v = rr == 1 ? -2.7 : 0.0;
+                    }
                     t[w] = v;
                 }
             }

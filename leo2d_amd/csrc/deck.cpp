@@ -92,6 +92,7 @@ Tokens split_spaces(const std::string& line)
 int material_of(const std::string& name)
 {
     if (name == "Graphene") return MAT_GRAPHENE;
+    if (name == "StrainedGraphene") return MAT_STRAINED_GRAPHENE;
     if (name == "GrapheneSimple") return MAT_GRAPHENE_SIMPLE;
     if (name == "GrapheneSimpleStrain") return MAT_GRAPHENE_SIMPLE_STRAIN;
     throw DeckError("Failed to find material! Aborting...");
@@ -287,8 +288,9 @@ void setup_supercell(Deck& deck)
     if (o.geti("boundary_condition") != 1) return;
     // Graphene::lattice (graphene.h:78-80); the supercell is built in each
     // sheet's own frame (locality.cpp:240-257)
+    // Materials::lattice(sdata[0].mat): one unit cell for every sheet (locality.cpp:188-196, 318-330)
     const double a = 2.4768;
-    const double uc[2][2] = {{1. * a, 0.}, {.5 * a, 0.866025403784438646764 * a}};
+    const double uc[2][2] = {{1. * a, 0.}, {.5 * a, lattice_s32(deck.sheets[0].mat) * a}};
     if (!o.i.count("supercell_type")) throw DeckError("BOUNDARY_CONDITION = 1 needs SUPERCELL_M_N or SUPERCELL_GRID");
     const long long type = o.geti("supercell_type");
     const int ns = (int)deck.sheets.size();

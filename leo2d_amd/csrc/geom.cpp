@@ -74,11 +74,17 @@ Geom build_geometry(const Deck& d)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) g.gsc[a][b] = it->second[a][b];
     }
+    // Materials::lattice / orbital_pos (graphene.h:78-85, strained_graphene.h:50-56); one lattice
+    // for all sheets: StrainedGraphene on every sheet or on none
+    const int mat0 = d.sheets.empty() ? MAT_GRAPHENE : d.sheets[0].mat;
+    for (const SheetData& sd : d.sheets)
+        if ((sd.mat == MAT_STRAINED_GRAPHENE) != (mat0 == MAT_STRAINED_GRAPHENE))
+            throw DeckError("StrainedGraphene must be the material of every sheet or of none");
     const double A = 2.4768;
     g.lat[0][0] = 1. * A; g.lat[0][1] = 0.;
-    g.lat[1][0] = .5 * A; g.lat[1][1] = 0.866025403784438646764 * A;
+    g.lat[1][0] = .5 * A; g.lat[1][1] = lattice_s32(mat0) * A;
     g.apos[0][0] = 0.; g.apos[0][1] = 0.; g.apos[0][2] = 0.;
-    g.apos[1][0] = 0.5 * A; g.apos[1][1] = 0.288675134594812882255 * A; g.apos[1][2] = 0;
+    g.apos[1][0] = 0.5 * A; g.apos[1][1] = orbital_s36(mat0) * A; g.apos[1][2] = 0;
     {
         const double a11 = g.lat[0][0], a12 = g.lat[0][1], a21 = g.lat[1][0], a22 = g.lat[1][1];
         const double det_a = a11 * a22 - a12 * a21;
@@ -217,7 +223,7 @@ std::vector<std::vector<double>> sheet_reciprocal(const Deck& d, int s)
 {
     const Params& o = d.opts;
     const double A = 2.4768;
-    const double uc[2][2] = {{1. * A, 0.}, {.5 * A, 0.866025403784438646764 * A}};
+    const double uc[2][2] = {{1. * A, 0.}, {.5 * A, lattice_s32(d.sheets[s].mat) * A}};
     const double th = d.angles[s];
     std::vector<std::vector<double>> a(2, std::vector<double>(2));
     for (int i = 0; i < 2; ++i) {
@@ -293,7 +299,7 @@ std::vector<KVec> k_path(const Deck& d)
         const int M = (int)o.geti("m_supercell"), N = (int)o.geti("n_supercell");
         const int A[2][2] = {{M > N ? N : M, M > N ? M : N}, {M > N ? -M : -N, M + N}};
         const double a = 2.4768;
-        const double uc[2][2] = {{1. * a, 0.}, {.5 * a, 0.866025403784438646764 * a}};
+        const double uc[2][2] = {{1. * a, 0.}, {.5 * a, lattice_s32(d.sheets[0].mat) * a}};
         std::vector<std::vector<double>> sc(2, std::vector<double>(2));
         for (int r = 0; r < 2; ++r)
             for (int c = 0; c < 2; ++c) sc[r][c] = A[r][0] * uc[0][c] + A[r][1] * uc[1][c];

@@ -50,14 +50,23 @@ struct orc_geom {
     double* lpos;  /* [3n] sheet-local frame: Sheet::posAtomIndex, sheet.cpp:458-531 */
 };
 
-static void graphene_lattice(double lat[2][2], double apos[2][3])
+static void graphene_lattice(int mat, double lat[2][2], double apos[2][3])
 {
-    /* graphene.h:78-85 */
+    /* graphene.h:78-85; StrainedGraphene (strained_graphene.h:50-56) rounds sqrt(3)/2
+     * and sqrt(3)/6 to 11 digits */
+    const double s32 = mat == ORC_MAT_STRAINED_GRAPHENE ? 0.86602540378 : N_SQRT3_2;
+    const double s36 = mat == ORC_MAT_STRAINED_GRAPHENE ? 0.28867513459 : N_SQRT3_6;
     lat[0][0] = 1. * GR_A; lat[0][1] = 0.;
-    lat[1][0] = .5 * GR_A; lat[1][1] = N_SQRT3_2 * GR_A;
+    lat[1][0] = .5 * GR_A; lat[1][1] = s32 * GR_A;
     apos[0][0] = 0.; apos[0][1] = 0.; apos[0][2] = 0.;
-    apos[1][0] = 0.5 * GR_A; apos[1][1] = N_SQRT3_6 * GR_A; apos[1][2] = 0;
+    apos[1][0] = 0.5 * GR_A; apos[1][1] = s36 * GR_A; apos[1][2] = 0;
 }
+
+/* Materials::intra_search_radius / inter_search_radius truncated to int
+ * (sheet.cpp:39, 626; hstruct.cpp:305): graphene 4 and 6, StrainedGraphene
+ * (5.7 + a/3) and (8 + a/3) */
+static int intra_window(int mat) { return mat == ORC_MAT_STRAINED_GRAPHENE ? (int)(5.7 + GR_A / 3.) : GR_INTRA_SEARCH; }
+static int inter_window(int mat) { return mat == ORC_MAT_STRAINED_GRAPHENE ? (int)(8. + GR_A / 3.) : GR_INTER_SEARCH; }
 
 /* Sheet::posAtomGrid (sheet.cpp:482-531), solver_space = 0 */
 static void pos_atom_grid(const orc_geom* g, int s, int gi, int gj, int l, double p[3])
@@ -102,7 +111,12 @@ orc_geom* orc_geom_build(const orc_geom_desc* d)
     orc_geom* g = (orc_geom*)calloc(1, sizeof(orc_geom));
     g->d = *d;
     g->norb = 2;
-    graphene_lattice(g->lat, g->apos);
+    for (int s = 1; s < d->num_sheets; ++s)
+        if ((d->sheet[s].mat == ORC_MAT_STRAINED_GRAPHENE) != (d->sheet[0].mat == ORC_MAT_STRAINED_GRAPHENE)) {
+            free(g);
+            return NULL;   /* one lattice for all sheets: StrainedGraphene on every sheet or on none */
+        }
+    graphene_lattice(d->sheet[0].mat, g->lat, g->apos);
     {
         double a11 = g->lat[0][0], a12 = g->lat[0][1], a21 = g->lat[1][0], a22 = g->lat[1][1];
         double det_a = a11 * a22 - a12 * a21;
@@ -240,10 +254,57 @@ static double simple_intra(int orb_row, int orb_col, int v0, int v1)
     return r == 1 ? -2.7 : 0.0;
 }
 
+/* honeycomb shell index of a hop (strained_graphene.cpp:16-60, the same homogeneous
+ * coordinates as graphene.cpp:40-61) */
+static int hop_shell(int orb_row, int orb_col, int v0, int v1)
+{
+    int h0 = 2 * v0 + v1;
+    int h1 = -1 * v0 + v1;
+    if (orb_col == 0 && orb_row == 1) h0 -= 1;
+    else if (orb_col == 1 && orb_row == 0) h0 += 1;
+    return h0 * (h0 + h1) + h1 * h1;
+}
+
+static const double SG_T[4] = {0.7833, -2.8219, 0.2543, -0.1803};   /* strained_graphene.cpp:81 */
+static const double SG_A[4] = {-4.8782, 4.0066, -0.4633, 0.6236};
+static const double SG_B[4] = {0.0000, -3.0868, 0.8017, 0.4793};
+
+static double strained_table(int orb_row, int orb_col, int v0, int v1)
+{
+    /* strained_graphene.cpp:16-91: shells 0, 1, 3, 4 */
+    switch (hop_shell(orb_row, orb_col, v0, v1)) {
+    case 0: return SG_T[0];
+    case 1: return SG_T[1];
+    case 3: return SG_T[2];
+    case 4: return SG_T[3];
+    default: return 0.;
+    }
+}
+
+double orc_strained_intra(int orb_row, int orb_col, int v0, int v1, double hx, double hy, const double u[2][2],
+                          double theta)
+{
+    /* strained_graphene.cpp:300-396.  The second component of the vector-like strain
+     * is written twice there (u_xx - u_yy, then u_xy + u_yx into the same slot), so
+     * the pair is (u_xy + u_yx, 0). */
+    const double onedim = u[0][0] + u[1][1];
+    const double two0 = u[0][1] + u[1][0], two1 = 0.0;
+    const double w0 = cos(theta) * hx + sin(theta) * hy;
+    const double w1 = -sin(theta) * hx + cos(theta) * hy;
+    switch (hop_shell(orb_row, orb_col, v0, v1)) {
+    case 0: return SG_T[0] + SG_A[0] * onedim;
+    case 1: return SG_T[1] + SG_A[1] * onedim + SG_B[1] * (w0 * two0 + w1 * two1);
+    case 3: return SG_T[2] + SG_A[2] * onedim + SG_B[2] * (-w1 * two0 + w0 * two1);
+    case 4: return SG_T[3] + SG_A[3] * onedim + SG_B[3] * (w0 * two0 + w1 * two1);
+    default: return 0.;
+    }
+}
+
 double orc_intralayer(int mat, int orb_row, int orb_col, int v0, int v1)
 {
-    return mat != ORC_MAT_GRAPHENE ? simple_intra(orb_row, orb_col, v0, v1)
-                                   : graphene_intra(orb_row, orb_col, v0, v1);
+    if (mat == ORC_MAT_GRAPHENE) return graphene_intra(orb_row, orb_col, v0, v1);
+    if (mat == ORC_MAT_STRAINED_GRAPHENE) return strained_table(orb_row, orb_col, v0, v1);
+    return simple_intra(orb_row, orb_col, v0, v1);
 }
 
 static double c_to_c(int orb_row, int orb_col, double v0, double v1, double v2, double theta_row, double theta_col)
@@ -296,8 +357,10 @@ static double simple_inter(double v0, double v1, double v2)
 double orc_interlayer(int mat, int orb_row, int orb_col, double dx, double dy, double dz,
                       double theta_row, double theta_col)
 {
-    return mat != ORC_MAT_GRAPHENE ? simple_inter(dx, dy, dz)
-                                   : c_to_c(orb_row, orb_col, dx, dy, dz, theta_row, theta_col);
+    /* Materials::interlayer_term (materials.cpp:362-372): C_to_C for Graphene and StrainedGraphene */
+    return (mat == ORC_MAT_GRAPHENE || mat == ORC_MAT_STRAINED_GRAPHENE)
+               ? c_to_c(orb_row, orb_col, dx, dy, dz, theta_row, theta_col)
+               : simple_inter(dx, dy, dz);
 }
 
 /* ------------------------------------------------------------------ */
@@ -384,8 +447,9 @@ static void intra_pairs_sheet(const orc_geom* g, int s, pvec* out)
                 double ph[3] = {kp[0] + scd0, kp[1] + scd1, kp[2]};
                 int i0 = sheet_nearest(g, s, ph, 0);
                 int j0 = sheet_nearest(g, s, ph, 1);
-                for (int i = i0 - GR_INTRA_SEARCH; i < i0 + GR_INTRA_SEARCH; ++i)
-                    for (int j = j0 - GR_INTRA_SEARCH; j < j0 + GR_INTRA_SEARCH; ++j)
+                const int W = intra_window(sd->mat);
+                for (int i = i0 - W; i < i0 + W; ++i)
+                    for (int j = j0 - W; j < j0 + W; ++j)
                         for (int l = 0; l < g->norb; ++l) {
                             int k2 = sheet_grid(g, s, i, j, l);
                             if (k2 == -1) continue;
@@ -433,10 +497,11 @@ static void inter_pairs_all(const orc_geom* g, pvec* out)
                     int i0 = hstruct_nearest(g, ph, s0, 0);
                     int j0 = hstruct_nearest(g, ph, s0, 1);
                     int base = g->base[s0];
-                    int ilo = i0 - GR_INTER_SEARCH > 0 ? i0 - GR_INTER_SEARCH : 0;
-                    int ihi = g->H[s0] < i0 + GR_INTER_SEARCH ? g->H[s0] : i0 + GR_INTER_SEARCH;
-                    int jlo = j0 - GR_INTER_SEARCH > 0 ? j0 - GR_INTER_SEARCH : 0;
-                    int jhi = g->W[s0] < j0 + GR_INTER_SEARCH ? g->W[s0] : j0 + GR_INTER_SEARCH;
+                    const int W = inter_window(g->d.sheet[0].mat);
+                    int ilo = i0 - W > 0 ? i0 - W : 0;
+                    int ihi = g->H[s0] < i0 + W ? g->H[s0] : i0 + W;
+                    int jlo = j0 - W > 0 ? j0 - W : 0;
+                    int jhi = g->W[s0] < j0 + W ? g->W[s0] : j0 + W;
                     for (int i = ilo; i < ihi; ++i)
                         for (int j = jlo; j < jhi; ++j)
                             for (int l = 0; l < g->norb; ++l) {
@@ -595,7 +660,7 @@ void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
  * decided by IEEE-exact operations on both the CPU and the GPU. */
 static int inter_in_support(int mat, double dx, double dy, double dz)
 {
-    if (mat != ORC_MAT_GRAPHENE) return sqrt(dx * dx + dy * dy + dz * dz) < 6.0;
+    if (mat != ORC_MAT_GRAPHENE && mat != ORC_MAT_STRAINED_GRAPHENE) return sqrt(dx * dx + dy * dy + dz * dz) < 6.0;
     if (abs((int)dz) > 5) return 0;
     return sqrt(dx * dx + dy * dy) < GR_INTER_CUTOFF;
 }
@@ -666,6 +731,26 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                 int support;
                 if (blk == 0) {
                     double raw_t = pr->t;
+                    if (mat == ORC_MAT_STRAINED_GRAPHENE && (job->strain_type == 1 || job->uniform_strain == 1)) {
+                        /* strained intralayer term (locality.cpp:4034-4160, 4744-4870): the hop's
+                         * direction on the job's positions, the uniform strain (zero for the
+                         * relaxation field) and the sheet angle */
+                        int lh = g->i2g[4 * (size_t)new_k + 2];
+                        int gd0 = g->i2g[4 * (size_t)new_k + 0] - g->i2g[4 * (size_t)k_i + 0];
+                        int gd1 = g->i2g[4 * (size_t)new_k + 1] - g->i2g[4 * (size_t)k_i + 1];
+                        if (bc == 1) {
+                            const orc_sheet_desc* sd = &g->d.sheet[s_i];
+                            gd0 = gd0 - sc_i * sd->supercell_stride[0][0] - sc_j * sd->supercell_stride[1][0];
+                            gd1 = gd1 - sc_i * sd->supercell_stride[0][1] - sc_j * sd->supercell_stride[1][1];
+                        }
+                        double hx = x2 - x1, hy = y2 - y1;
+                        double hn = sqrt(hx * hx + hy * hy);
+                        if (hn != 0) { hx = hx / hn; hy = hy / hn; }
+                        const double zero_u[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                        raw_t = orc_strained_intra(g->i2g[4 * (size_t)k_i + 2], lh, gd0, gd1, hx, hy,
+                                                   job->uniform_strain == 1 ? job->u : zero_u,
+                                                   g->d.sheet[s_i].angle);
+                    }
                     if (mat == ORC_MAT_GRAPHENE_SIMPLE_STRAIN && raw_t != 0) {
                         /* strained bond: t = -2.7 exp(-3.37 (d / a_cc - 1)) */
                         double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;

@@ -50,6 +50,7 @@ extern "C" {
 #define ORC_MAT_GRAPHENE 0        /* src/materials/graphene.cpp */
 #define ORC_MAT_GRAPHENE_SIMPLE 1 /* NN t=-2.7 eV, t_perp = 0.48 exp(-(r-3.35)/0.45), r<6 A */
 #define ORC_MAT_GRAPHENE_SIMPLE_STRAIN 2 /* as 1, NN t = -2.7 exp(-3.37 (d/a_cc - 1)) at bond length d */
+#define ORC_MAT_STRAINED_GRAPHENE 3 /* src/materials/strained_graphene.{h,cpp} */
 
 typedef struct {
     int mat;
@@ -85,6 +86,10 @@ int orc_target_center(const orc_geom* g, int s);
 
 /* couplings ------------------------------------------------------------ */
 double orc_intralayer(int mat, int orb_row, int orb_col, int v0, int v1);
+/* Coupling::Intralayer::strained_graphene with hop direction, strain and sheet
+ * angle (strained_graphene.cpp:300-396) */
+double orc_strained_intra(int orb_row, int orb_col, int v0, int v1, double hx, double hy, const double u[2][2],
+                          double theta);
 double orc_interlayer(int mat, int orb_row, int orb_col, double dx, double dy, double dz,
                       double theta_row, double theta_col);
 

@@ -33,12 +33,20 @@ MAT_GRAPHENE = 0
 MAT_GRAPHENE_SIMPLE = 1
 # src/materials/materials.cpp:15-21, plus the workload model name
 MAT_GRAPHENE_SIMPLE_STRAIN = 2
-MATERIALS = {"Graphene": MAT_GRAPHENE, "GrapheneSimple": MAT_GRAPHENE_SIMPLE,
+MAT_STRAINED_GRAPHENE = 3
+MATERIALS = {"Graphene": MAT_GRAPHENE, "StrainedGraphene": MAT_STRAINED_GRAPHENE, "GrapheneSimple": MAT_GRAPHENE_SIMPLE,
              "GrapheneSimpleStrain": MAT_GRAPHENE_SIMPLE_STRAIN}
 
 GR_A = 2.4768
 SQRT3_2 = 0.866025403784438646764
 LATTICE = ((1.0 * GR_A, 0.0), (0.5 * GR_A, SQRT3_2 * GR_A))
+# strained_graphene.h:50-52: sqrt(3)/2 rounded to 11 digits
+STRAINED_LATTICE = ((1.0 * GR_A, 0.0), (0.5 * GR_A, 0.86602540378 * GR_A))
+
+
+def lattice(mat: int):
+    """Materials::lattice (materials.cpp:60-75)."""
+    return STRAINED_LATTICE if mat == MAT_STRAINED_GRAPHENE else LATTICE
 
 
 def build(force: bool = False) -> str:
@@ -431,7 +439,8 @@ def setup_supercell(deck: Deck) -> None:
     if o.get("boundary_condition", 0) != 1:
         return
     typ = o.get("supercell_type")
-    uc = LATTICE
+    # Materials::lattice(sdata[0].mat): one unit cell for every sheet (locality.cpp:188-196)
+    uc = lattice(deck.sdata[0].mat)
     if typ == 1:
         M = o["m_supercell"]
         N = o["n_supercell"]
@@ -620,8 +629,9 @@ def sheet_reciprocal(deck: Deck, s: int):
     UNIFORM_STRAIN = 1, then reciprocal(a)."""
     o = deck.opts
     th = deck.angles[s]
-    a = [[LATTICE[i][0] * math.cos(th) - LATTICE[i][1] * math.sin(th),
-          LATTICE[i][0] * math.sin(th) + LATTICE[i][1] * math.cos(th)] for i in range(2)]
+    lat = lattice(deck.sdata[s].mat)
+    a = [[lat[i][0] * math.cos(th) - lat[i][1] * math.sin(th),
+          lat[i][0] * math.sin(th) + lat[i][1] * math.cos(th)] for i in range(2)]
     if o["boundary_condition"] == 1 and o["uniform_strain"] == 1:
         u = o["u_ij"]
         a = [[(1.0 + u[0][0]) * t[0] + u[0][1] * t[1], u[1][0] * t[0] + (1.0 + u[1][1]) * t[1]] for t in a]
@@ -698,7 +708,7 @@ def k_path(deck: Deck) -> list:
         else:
             t = M * M + M * N + N * N
             A = ((0, t), (-t, t))
-        uc = LATTICE
+        uc = lattice(deck.sdata[0].mat)
         sc = [[A[0][0] * uc[0][0] + A[0][1] * uc[1][0], A[0][0] * uc[0][1] + A[0][1] * uc[1][1]],
               [A[1][0] * uc[0][0] + A[1][1] * uc[1][0], A[1][0] * uc[0][1] + A[1][1] * uc[1][1]]]
         bs = reciprocal(sc)

@@ -11,10 +11,40 @@
 #include "geom/sdata.h"
 #include "geom/sheet.h"
 #include "materials/graphene.h"
+#include "materials/materials.h"
+#include "materials/strained_graphene.h"
 #include "matrix/spmatrix.h"
 #include "params/job_params.h"
 
+// material of the sheets that ref_sheet_intra_pairs / ref_sheet_positions build
+static Materials::Mat g_sheet_mat = Materials::Mat::Graphene;
+
 extern "C" {
+
+// 0: Graphene, 1: StrainedGraphene for the sheets built below
+void ref_set_sheet_material(int strained)
+{
+    g_sheet_mat = strained ? Materials::Mat::StrainedGraphene : Materials::Mat::Graphene;
+}
+
+// Coupling::Intralayer::strained_graphene (strained_graphene.cpp:16), the unstrained table
+double ref_strained_intra(int orb_row, int orb_col, int v0, int v1)
+{
+    std::array<int, 2> v = {{v0, v1}};
+    return Coupling::Intralayer::strained_graphene(Graphene::orbital(orb_row), Graphene::orbital(orb_col), v);
+}
+
+// Coupling::Intralayer::strained_graphene (strained_graphene.cpp:300) with hop direction,
+// strain u[4] = {u_xx, u_xy, u_yx, u_yy} and sheet angle
+double ref_strained_intra_hop(int orb_row, int orb_col, int v0, int v1, double hx, double hy, const double* u,
+                              double theta)
+{
+    std::array<int, 2> v = {{v0, v1}};
+    std::vector<double> hop = {hx, hy};
+    std::vector<std::vector<double> > strain = {{u[0], u[1]}, {u[2], u[3]}};
+    return Coupling::Intralayer::strained_graphene(Graphene::orbital(orb_row), Graphene::orbital(orb_col), v, hop,
+                                                   strain, theta);
+}
 
 // Coupling::Intralayer::graphene (graphene.cpp:13)
 double ref_graphene_intra(int orb_row, int orb_col, int v0, int v1)
@@ -48,7 +78,7 @@ long ref_sheet_intra_pairs(int min0, int min1, int max0, int max1, double max_R,
     gs[0][0] = gshift ? gshift[0] : 0.0;
     gs[0][1] = gshift ? gshift[1] : 0.0;
     opts.setParam("global_shifts", gs);
-    Sdata sd(Materials::Mat::Graphene, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
+    Sdata sd(g_sheet_mat, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
     sd.sheet_index = 0;
     sd.opts = opts;
     if (bc == 1) {
@@ -82,7 +112,7 @@ int ref_sheet_positions(int min0, int min1, int max0, int max1, double max_R, in
     std::vector<int> mx = {max0, max1, max0};
     Job_params opts;
     opts.setParam("boundary_condition", bc);
-    Sdata sd(Materials::Mat::Graphene, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
+    Sdata sd(g_sheet_mat, mn, mx, max_R, bc, 0, 0, std::string(""), 0);
     sd.sheet_index = 0;
     sd.opts = opts;
     if (bc == 1) {

@@ -29,7 +29,7 @@ def oracle_mod():
 
 @pytest.fixture(scope="session")
 def ref_lib():
-    """The reference's own graphene.cpp / sheet.cpp / spmatrix.cpp (oracle/_ref)."""
+    """The reference's own graphene.cpp / strained_graphene.cpp / sheet.cpp / spmatrix.cpp (oracle/_ref)."""
     import ctypes as C
 
     from oracle import oracle as orc
@@ -51,6 +51,12 @@ def ref_lib():
     L.ref_sheet_positions.restype = C.c_int
     L.ref_sheet_positions.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int]
+    L.ref_set_sheet_material.restype = None
+    L.ref_set_sheet_material.argtypes = [C.c_int]
+    L.ref_strained_intra.restype = C.c_double
+    L.ref_strained_intra.argtypes = [C.c_int] * 4
+    L.ref_strained_intra_hop.restype = C.c_double
+    L.ref_strained_intra_hop.argtypes = [C.c_int] * 4 + [C.c_double, C.c_double, C.c_void_p, C.c_double]
     L.ref_spmv.restype = None
     L.ref_spmv.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                            C.c_double, C.c_double]

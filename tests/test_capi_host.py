@@ -34,7 +34,8 @@ def test_exports_every_declared_symbol():
 ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_c0_twin.in",
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
              "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
-             "ttlg_grid_k5.in", "tblg_relax_cell.in"]
+             "ttlg_grid_k5.in", "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
+             "tblg_strained_relax.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -138,6 +139,19 @@ def test_k_path_errors(oracle_mod, name, k_type, edits):
     with pytest.raises(L.Leo2dError) as e:
         L.Geometry(L.Deck(text)).jobs()
     assert e.value.code == L.ERR_DECK
+
+
+def test_strained_graphene_needs_one_lattice(oracle_mod):
+    """StrainedGraphene has its own lattice constants; a deck that mixes it with
+    another material on the same structure is LEO2D_ERR_DECK (the oracle refuses
+    the geometry too)."""
+    text = deck_text("tblg_strained_flake.in").replace("MATERIAL = StrainedGraphene", "MATERIAL = Graphene", 1)
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(text))
+    assert e.value.code == L.ERR_DECK
+    od = oracle_mod.parse_deck(text)
+    with pytest.raises(oracle_mod.DeckError):
+        oracle_mod.Geometry(oracle_mod.geom_desc(od))
 
 
 def test_relaxation_file_matches_oracle(oracle_mod, tmp_path):

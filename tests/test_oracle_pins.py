@@ -93,6 +93,31 @@ def test_couplings_match_reference_build(oracle_mod, ref_lib):
         assert a == b, (x, y, z, th1, th2, o1, o2, a, b)
 
 
+def test_strained_graphene_couplings_match_reference_build(oracle_mod, ref_lib):
+    """StrainedGraphene intralayer terms (strained_graphene.cpp:16, 300) from the
+    reference build, bit for bit: the unstrained table over a window of hops, and
+    the strained form over random hop directions, strains and sheet angles."""
+    L = oracle_mod.lib()
+    L.orc_strained_intra.restype = C.c_double
+    L.orc_strained_intra.argtypes = [C.c_int] * 4 + [C.c_double, C.c_double, C.c_void_p, C.c_double]
+    for orow in range(2):
+        for ocol in range(2):
+            for v0 in range(-7, 8):
+                for v1 in range(-7, 8):
+                    assert L.orc_intralayer(3, orow, ocol, v0, v1) == ref_lib.ref_strained_intra(orow, ocol, v0, v1)
+    rng = np.random.default_rng(11)
+    for _ in range(3000):
+        orow, ocol = (int(x) for x in rng.integers(0, 2, 2))
+        v0, v1 = (int(x) for x in rng.integers(-3, 4, 2))
+        t = rng.uniform(-math.pi, math.pi)
+        h = np.array([math.cos(t), math.sin(t)])
+        u = rng.normal(0, 0.05, 4)
+        th = rng.uniform(-0.3, 0.3)
+        a = L.orc_strained_intra(orow, ocol, v0, v1, h[0], h[1], _p(u), th)
+        b = ref_lib.ref_strained_intra_hop(orow, ocol, v0, v1, h[0], h[1], _p(u), th)
+        assert a == b, (orow, ocol, v0, v1, a, b)
+
+
 # ---------------------------------------------------------------- sheet pairs
 def _ref_sheet_pairs(ref_lib, min0, max0, max_R, bc, sc=None, st=None, S=1):
     sca = np.asarray(sc if sc is not None else [[0, 0], [0, 0]], dtype=np.float64).reshape(4)
@@ -111,13 +136,13 @@ def _ref_sheet_pairs(ref_lib, min0, max0, max_R, bc, sc=None, st=None, S=1):
     return nor.value, ij.reshape(-1, 2), scv.reshape(-1, 2), t, pos
 
 
-def _oracle_single_sheet(orc, min0, max0, max_R, bc, sc=None, st=None, S=1):
+def _oracle_single_sheet(orc, min0, max0, max_R, bc, sc=None, st=None, S=1, mat=0):
     d = orc.GeomDesc()
     d.num_sheets = 1
     d.boundary_condition = bc
     d.sc_search_size = S
     sh = d.sheet[0]
-    sh.mat = 0
+    sh.mat = mat
     sh.min_shape[0] = sh.min_shape[1] = min0
     sh.max_shape[0] = sh.max_shape[1] = max0
     sh.max_R = max_R
@@ -131,10 +156,17 @@ def _oracle_single_sheet(orc, min0, max0, max_R, bc, sc=None, st=None, S=1):
     return orc.Geometry(d)
 
 
-@pytest.mark.parametrize("case", ["flake10", "flake14", "periodic_7_6_sheet0", "periodic_7_6_sheet1", "grid_3x3"])
+@pytest.mark.parametrize("case", ["flake10", "flake14", "periodic_7_6_sheet0", "periodic_7_6_sheet1", "grid_3x3",
+                                  "strained:flake10", "strained:periodic_7_6_sheet1"])
 def test_sheet_pairs_match_reference_build(oracle_mod, ref_lib, case):
+    """Enumeration, positions and intralayer pairs of one sheet against the
+    reference's Sheet; 'strained:' cases build StrainedGraphene sheets (its own
+    lattice and orbital constants, a 6-cell search window and its own table)."""
     orc = oracle_mod
-    uc = orc.LATTICE
+    strained = case.startswith("strained:")
+    case = case.partition(":")[2] if strained else case
+    ref_lib.ref_set_sheet_material(1 if strained else 0)
+    uc = orc.lattice(orc.MAT_STRAINED_GRAPHENE if strained else orc.MAT_GRAPHENE)
     if case.startswith("flake"):
         R = int(case[5:])
         args = dict(min0=-R, max0=R, max_R=float(R), bc=0)
@@ -147,7 +179,8 @@ def test_sheet_pairs_match_reference_build(oracle_mod, ref_lib, case):
         sc = [[A[r][0] * uc[0][0] + A[r][1] * uc[1][0], A[r][0] * uc[0][1] + A[r][1] * uc[1][1]] for r in range(2)]
         args = dict(min0=-8, max0=21, max_R=21.0, bc=1, sc=sc, st=[list(A[0]), list(A[1])])
     nref, ij, scv, t, pos = _ref_sheet_pairs(ref_lib, **args)
-    g = _oracle_single_sheet(orc, **args)
+    ref_lib.ref_set_sheet_material(0)
+    g = _oracle_single_sheet(orc, mat=orc.MAT_STRAINED_GRAPHENE if strained else 0, **args)
     assert g.n == nref
     i2g, i2p = g.index_to_grid, g.index_to_pos
     # single unrotated sheet at height 0: global positions are the sheet positions
@@ -206,6 +239,22 @@ def test_inter_pairs_against_all_pairs_scan(oracle_mod):
                 assert (a, b) in have, (a, b)
     # flake: no images
     assert np.all(sc == 0)
+
+
+def test_strained_graphene_inter_window(oracle_mod):
+    """StrainedGraphene searches 8 cells each way for interlayer pairs (int(8 + a/3),
+    hstruct.cpp:305): on a flake that window holds every pair within the 12.5 A
+    cut, so the stored list is exactly the all-pairs scan at 12.5 A (graphene's
+    6-cell window misses some between 10 and 12.5 A)."""
+    orc = oracle_mod
+    deck, geom, jobs, _ = orc.run_deck(open(os.path.join(DECKS, "tblg_strained_flake.in")).read())
+    assert all(sd.mat == orc.MAT_STRAINED_GRAPHENE for sd in deck.sdata)
+    ij, _, _ = geom.pair_arrays(1)
+    p = geom.index_to_pos.reshape(-1, 3)
+    s = geom.index_to_grid.reshape(-1, 4)[:, 3]
+    d2 = (p[:, None, 0] - p[None, :, 0]) ** 2 + (p[:, None, 1] - p[None, :, 1]) ** 2
+    scan = np.argwhere((np.abs(s[:, None] - s[None, :]) == 1) & (d2 < 12.5 ** 2))
+    assert sorted(map(tuple, ij.tolist())) == sorted(map(tuple, scan.tolist()))
 
 
 # ---------------------------------------------------------------- KPM
