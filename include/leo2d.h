@@ -98,11 +98,12 @@ typedef struct {
      * [[eps, 0], [0, -nu eps]] on the last sheet. */
     int32_t sheet_strain_on;
     double sheet_strain[LEO2D_MAX_SHEETS][2][2];
-    /* STRAIN_TYPE (main.cpp:595-614): 0 none, 1 SUPERCELL -- the Fourier
+    /* STRAIN_TYPE (main.cpp:595-614): 0 none; 1 SUPERCELL -- the Fourier
      * relaxation field given to leo2d_set_relaxation moves every orbital after
-     * the shifts (locality.cpp:3657-3711), and the interlayer terms take each
-     * orbital's local twist from its nearest neighbour (locality.cpp:665-731,
-     * 4261-4269, 5045-5052). */
+     * the shifts (locality.cpp:3657-3711); 6 DIRECT -- the same with the
+     * reference's built-in coefficients (strain.cpp:756-876).  With either, the
+     * interlayer terms take each orbital's local twist from its nearest
+     * neighbour (locality.cpp:665-731, 4261-4269, 5045-5052). */
     int32_t strain_type;
 } leo2d_job;
 
@@ -307,7 +308,7 @@ int leo2d_mlp_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, in
  * coefficients in the supercell basis, in the file's order.  Jobs with
  * strain_type = 1 assembled on this handle use them; max_k < 0 clears them. */
 int leo2d_set_relaxation(leo2d_handle* h, int32_t max_k, const double* coeffs);
-/* Read such a file (max_k, then the three rows, whitespace separated) into
+/* max_k is at most 64.  Read such a file (max_k, then the three rows, whitespace separated) into
  * coeffs (NULL to ask for the size): *max_k receives max_k and *count the
  * number of doubles, 3 (max_k (max_k + 1) / 2 + 1). */
 int leo2d_read_relaxation(const char* path, int32_t* max_k, double* coeffs, int64_t* count);

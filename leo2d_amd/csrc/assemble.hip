@@ -338,19 +338,33 @@ __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j
     double x = g.i2p[3 * (size_t)i + 0] + sh0 * a[0] + sh1 * a[2];
     double y = g.i2p[3 * (size_t)i + 1] + sh0 * a[1] + sh1 * a[3];
     double z = g.i2p[3 * (size_t)i + 2];
-    if (j.strain_type == 1) {
+    if (j.strain_type == 1 || j.strain_type == 6) {
         // relaxation field (locality.cpp:3657-3711; StrainCalc::fourierStrainDisp_sc and
         // fourierStrainDisp, strain.cpp:290-311, 390-460): directions k = p b1 + q b2,
-        // q < max(p, 1), each with its three 60-degree turns (one for the constant term)
-        const int nc = j.relax_max_k * (j.relax_max_k + 1) / 2 + 1;
+        // q < max(p, 1), each with its three 60-degree turns (one for the constant term).
+        // STRAIN_TYPE = DIRECT (6): the same sum over the built-in max_k = 2 coefficients of
+        // StrainCalc::supercellDisp (strain.cpp:756-876)
+        const bool direct = j.strain_type == 6;
+        const int max_k = direct ? 2 : j.relax_max_k;
+        const int nc = max_k * (max_k + 1) / 2 + 1;
         double ux = 0.0, uy = 0.0, uz = 0.0;
         int idx = 0;
-        for (int p = 0; p < j.relax_max_k + 1; ++p) {
+        for (int p = 0; p < max_k + 1; ++p) {
             for (int q = 0; q < (p > 1 ? p : 1); ++q) {
                 const double ff = p == 0 ? 1.0 : 2.0;
                 const int rot_max = p == 0 ? 1 : 3;
                 const double kb0 = p * j.rb[0][0] + q * j.rb[1][0], kb1 = p * j.rb[0][1] + q * j.rb[1][1];
-                const double cx = j.relax[idx], cy = j.relax[nc + idx], cz = j.relax[2 * nc + idx];
+                double cx, cy, cz;
+                if (direct) {
+                    cx = 0.0;
+                    cy = idx == 1 ? -0.019951155162081 : idx == 2 ? -0.000806807799645 : 0.0;
+                    cz = idx == 0 ? -0.061901691526695 : idx == 1 ? -0.0220634404118861
+                       : idx == 2 ? -0.004100201789620 : -0.002676762110702;
+                } else {
+                    cx = j.relax[idx];
+                    cy = j.relax[nc + idx];
+                    cz = j.relax[2 * nc + idx];
+                }
                 for (int r = 0; r < rot_max; ++r) {
                     const double c = j.rot_cs[r][0], sn = j.rot_cs[r][1];
                     const double k0 = c * kb0 - sn * kb1, k1 = sn * kb0 + c * kb1;
@@ -364,10 +378,14 @@ __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j
             }
         }
         const double rx = j.basis_cs[0] * ux - j.basis_cs[1] * uy, ry = j.basis_cs[1] * ux + j.basis_cs[0] * uy;
-        const double sign = s == 1 ? -1.0 : 1.0;   // the second sheet moves the other way
-        x = x + sign * rx;
-        y = y + sign * ry;
-        z = z + sign * uz;
+        // per-sheet factors: SUPERCELL moves the second sheet the other way; DIRECT scales the
+        // in-plane field by -2 on sheet 2 and the out-of-plane one by 0 on sheet 2 and -1 on
+        // sheet 3 (factors of 0 and powers of two, so scaling the sum equals scaling the terms)
+        const double fin = direct ? (s == 1 ? -2.0 : 1.0) : (s == 1 ? -1.0 : 1.0);
+        const double fout = direct ? (s == 2 ? -1.0 : (s == 1 ? 0.0 : 1.0)) : (s == 1 ? -1.0 : 1.0);
+        x = x + fin * rx;
+        y = y + fin * ry;
+        z = z + fout * uz;
     }
     if (j.sheet_strain_on == 1) {
         // per-sheet strain (open boundaries only), before the uniform strain
@@ -457,7 +475,7 @@ __global__ __launch_bounds__(kThreads) void fill_kernel(GeomDev g, JobDev j, con
                 gd1 = gd1 - sc_i * sd.st[0][1] - sc_j * sd.st[1][1];
             }
             double raw_t = intralayer(mat, l0, g.i2g[4 * (size_t)new_k + 2], gd0, gd1);
-            if (mat == MAT_STRAINED_GRAPHENE && (j.strain_type == 1 || j.uniform_strain == 1)) {
+            if (mat == MAT_STRAINED_GRAPHENE && (j.strain_type == 1 || j.strain_type == 6 || j.uniform_strain == 1)) {
                 // strained intralayer term (locality.cpp:4034-4160, 4744-4870): the hop's direction on
                 // the job's positions, the uniform strain (zero for the relaxation field), the sheet angle
                 double hx = x2 - x1, hy = y2 - y1;

@@ -284,7 +284,7 @@ leo2d::JobDev job_dev(const leo2d::Geom& g, const leo2d_job& j)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) d.ss[s][a][b] = j.sheet_strain[s][a][b];
     d.strain_type = j.strain_type;
-    if (j.strain_type == 1) {
+    if (j.strain_type == 1 || j.strain_type == 6) {
         // the relaxation field is periodic on the supercell (strain.cpp:390-460): its
         // directions are the supercell's reciprocal vectors turned by r pi / 3
         leo2d::reciprocal_vectors(g.gsc, d.rb);
@@ -984,8 +984,9 @@ int assemble_impl(leo2d_handle* h, const leo2d_job* job, int32_t pattern, int64_
         if (!(job->energy_rescale != 0.0) || !std::isfinite(job->energy_rescale))
             throw ArgError("energy_rescale must be finite and non-zero");
         if (job->sheet_strain_on && h->geom.bc == 1) throw ArgError("per-sheet strain needs open boundaries");
-        if (job->strain_type != 0 && job->strain_type != 1) throw ArgError("strain_type must be 0 or 1");
-        if (job->strain_type == 1 && h->geom.bc != 1) throw ArgError("the relaxation field needs a periodic supercell");
+        if (job->strain_type != 0 && job->strain_type != 1 && job->strain_type != 6)
+            throw ArgError("strain_type must be 0, 1 or 6");
+        if (job->strain_type != 0 && h->geom.bc != 1) throw ArgError("the relaxation field needs a periodic supercell");
         if (job->strain_type == 1 && h->relax_max_k < 0) throw StateError("strain_type 1 before leo2d_set_relaxation");
         if (!h->has_pairs) {
             int rc = leo2d_pairs_build(h, nullptr, nullptr, nullptr);
@@ -1105,7 +1106,7 @@ int leo2d_set_relaxation(leo2d_handle* h, int32_t max_k, const double* coeffs)
             return LEO2D_OK;
         }
         if (!coeffs) throw ArgError("relaxation coefficients missing");
-        if (max_k > 1024) throw ArgError("relaxation max_k above 1024");
+        if (max_k > 64) throw ArgError("relaxation max_k above 64");
         const size_t count = 3 * ((size_t)max_k * (max_k + 1) / 2 + 1);
         for (size_t q = 0; q < count; ++q)
             if (!std::isfinite(coeffs[q])) throw NanError("relaxation coefficient is not finite");
@@ -1125,7 +1126,7 @@ int leo2d_read_relaxation(const char* path, int32_t* max_k, double* coeffs, int6
     std::ifstream f(path);
     if (!f) return LEO2D_ERR_DECK;
     long long mk = -1;
-    if (!(f >> mk) || mk < 0 || mk > 1024) return LEO2D_ERR_DECK;
+    if (!(f >> mk) || mk < 0 || mk > 64) return LEO2D_ERR_DECK;
     const int64_t n = 3 * (mk * (mk + 1) / 2 + 1);
     *max_k = (int32_t)mk;
     *count = n;

@@ -59,11 +59,11 @@ Geom build_geometry(const Deck& d)
     Geom g;
     const Params& o = d.opts;
     if (o.geti("solver_space") != 0) throw DeckError("momentum-space solver is not on this path");
-    if (o.geti("strain_type") != 0 && o.geti("strain_type") != 1)
-        throw DeckError("STRAIN_TYPE " + std::to_string(o.geti("strain_type")) +
-                        " is not on this path (NONE and SUPERCELL are)");
-    if (o.geti("strain_type") == 1 && o.geti("boundary_condition") != 1)
-        throw DeckError("STRAIN_TYPE = SUPERCELL needs a periodic supercell");
+    const long long st = o.geti("strain_type");
+    if (st != 0 && st != 1 && st != 6)
+        throw DeckError("STRAIN_TYPE " + std::to_string(st) + " is not on this path (NONE, SUPERCELL and DIRECT are)");
+    if (st != 0 && o.geti("boundary_condition") != 1)
+        throw DeckError("STRAIN_TYPE SUPERCELL and DIRECT need a periodic supercell");
     if (o.geti("mat_from_file") != 0) throw DeckError("MAT_FROM_FILE is not on this path");
     g.num_sheets = (int)d.sheets.size();
     g.bc = (int)o.geti("boundary_condition");

@@ -560,15 +560,39 @@ void orc_relax_disp(const orc_job* job, double x, double y, int s, double* d)
     /* sum over Fourier directions k = i b1 + j b2 (j < max(i, 1)), each with its
      * three 60-degree rotations (one for the constant term), coefficients in
      * the supercell basis; then turned by the angle of b1 (strain.cpp:390-460)
-     * and given the sheet's sign: +1, or -1 on sheet 2 (strain.cpp:290-311). */
+     * and given the sheet's sign: +1, or -1 on sheet 2 (strain.cpp:290-311).
+     * STRAIN_TYPE = DIRECT (6, StrainCalc::supercellDisp, strain.cpp:756-876) uses
+     * the same sum over fixed max_k = 2 coefficients of a relaxed graphene
+     * sandwich, scaled per sheet before the sum (in plane x -2 on sheet 2, out of
+     * plane x 0 on sheet 2 and x -1 on sheet 3) and no sign. */
     const double(*b)[2] = job->relax_b;
-    const int nc = job->relax_max_k * (job->relax_max_k + 1) / 2 + 1;
-    const double* cx = job->relax;
-    const double* cy = job->relax + nc;
-    const double* cz = job->relax + 2 * nc;
+    int max_k = job->relax_max_k;
+    const double* cx = NULL;
+    const double* cy = NULL;
+    const double* cz = NULL;
+    double c6[3][4];
+    if (job->strain_type == 6) {
+        static const double sx[4] = {0.0, 0.0, 0.0, 0.0};
+        static const double sy[4] = {0.0, -0.019951155162081, -0.000806807799645, 0.0};
+        static const double sz[4] = {-0.061901691526695, -0.0220634404118861, -0.004100201789620, -0.002676762110702};
+        const double fin = s == 1 ? -2.0 : 1.0;
+        const double fout = s == 2 ? -1.0 : (s == 1 ? 0.0 : 1.0);
+        for (int q = 0; q < 4; ++q) {
+            c6[0][q] = fin * sx[q];
+            c6[1][q] = fin * sy[q];
+            c6[2][q] = fout * sz[q];
+        }
+        max_k = 2;
+        cx = c6[0]; cy = c6[1]; cz = c6[2];
+    } else {
+        const int nc = max_k * (max_k + 1) / 2 + 1;
+        cx = job->relax;
+        cy = job->relax + nc;
+        cz = job->relax + 2 * nc;
+    }
     double ux = 0.0, uy = 0.0, uz = 0.0;
     int idx = 0;
-    for (int i = 0; i < job->relax_max_k + 1; ++i) {
+    for (int i = 0; i < max_k + 1; ++i) {
         for (int j = 0; j < (i > 1 ? i : 1); ++j) {
             const double ff = i == 0 ? 1.0 : 2.0;
             const int rot_max = i == 0 ? 1 : 3;
@@ -586,6 +610,12 @@ void orc_relax_disp(const orc_job* job, double x, double y, int s, double* d)
     }
     const double rot = atan2(b[0][1], b[0][0]);
     const double rx = cos(rot) * ux - sin(rot) * uy, ry = sin(rot) * ux + cos(rot) * uy;
+    if (job->strain_type == 6) {
+        d[0] = rx;
+        d[1] = ry;
+        d[2] = uz;
+        return;
+    }
     const double sign = s == 1 ? -1.0 : 1.0;
     d[0] = sign * rx;
     d[1] = sign * ry;
@@ -633,7 +663,7 @@ void orc_job_positions(const orc_geom* g, const orc_job* job, double* i2pos)
         i2pos[i * 3 + 0] = g->i2p[i * 3 + 0] + sh[0] * s1_a[0][0] + sh[1] * s1_a[1][0];
         i2pos[i * 3 + 1] = g->i2p[i * 3 + 1] + sh[0] * s1_a[0][1] + sh[1] * s1_a[1][1];
         i2pos[i * 3 + 2] = g->i2p[i * 3 + 2];
-        if (job->strain_type == 1) {
+        if (job->strain_type == 1 || job->strain_type == 6) {
             /* locality.cpp:3657-3711 */
             double d[3];
             orc_relax_disp(job, i2pos[i * 3 + 0], i2pos[i * 3 + 1], s, d);
@@ -731,7 +761,8 @@ int orc_assemble(const orc_geom* g, const orc_pairs* p, const orc_job* job, orc_
                 int support;
                 if (blk == 0) {
                     double raw_t = pr->t;
-                    if (mat == ORC_MAT_STRAINED_GRAPHENE && (job->strain_type == 1 || job->uniform_strain == 1)) {
+                    if (mat == ORC_MAT_STRAINED_GRAPHENE &&
+                        (job->strain_type == 1 || job->strain_type == 6 || job->uniform_strain == 1)) {
                         /* strained intralayer term (locality.cpp:4034-4160, 4744-4870): the hop's
                          * direction on the job's positions, the uniform strain (zero for the
                          * relaxation field) and the sheet angle */

@@ -114,7 +114,8 @@ typedef struct {
     int pruned;                         /* 0 reference-exact slots, 1 drop zero slots */
     int sheet_strain_on;                /* per-sheet strain (open boundaries), before u */
     double sheet_strain[ORC_MAX_SHEETS][2][2];
-    /* STRAIN_TYPE = SUPERCELL (1): Fourier relaxation field and local twist angles */
+    /* STRAIN_TYPE = SUPERCELL (1) or DIRECT (6): Fourier relaxation field and local
+     * twist angles (6 has its coefficients built in) */
     int strain_type;
     int relax_max_k;
     const double* relax;                /* [3][relax_max_k (relax_max_k + 1) / 2 + 1] u_x, u_y, u_z */

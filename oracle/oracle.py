@@ -896,8 +896,17 @@ def make_job(deck: Deck, job: dict, pruned: int = 0) -> Job:
             raise DeckError("STRAIN_TYPE = SUPERCELL needs a periodic supercell")
         max_k, coeffs = read_relaxation(o["strain_file"])
         set_relaxation(j, max_k, coeffs, reciprocal(o["supercell"]))
+    elif st == 6:
+        # DIRECT plane waves: StrainCalc::supercellDisp's built-in coefficients (strain.cpp:756-876)
+        if o["boundary_condition"] != 1:
+            raise DeckError("STRAIN_TYPE = DIRECT needs a periodic supercell")
+        j.strain_type = 6
+        b = reciprocal(o["supercell"])
+        for a in range(2):
+            for d in range(2):
+                j.relax_b[a][d] = b[a][d]
     elif st != 0:
-        raise DeckError(f"STRAIN_TYPE {st} is not on this path (NONE and SUPERCELL are)")
+        raise DeckError(f"STRAIN_TYPE {st} is not on this path (NONE, SUPERCELL and DIRECT are)")
     return j
 
 

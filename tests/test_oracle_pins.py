@@ -900,3 +900,29 @@ def test_local_twist_of_an_unrelaxed_sheet(oracle_mod):
         dev = np.remainder(th - ang + third / 2, third) - third / 2
         assert np.max(np.abs(dev)) < 1e-12
         assert np.mean(th != ang) > 0.5                 # most orbitals found a bond
+
+
+def test_direct_plane_wave_field(oracle_mod):
+    """STRAIN_TYPE = DIRECT (StrainCalc::supercellDisp): the same Fourier sum over
+    the reference's built-in max_k = 2 coefficients, scaled per sheet.  At the
+    origin u_z = c_z0 + 6 (c_z1 + c_z2 + c_z3) on sheet 1; sheet 2 moves in plane
+    by -2 times sheet 1 and not at all out of plane; sheet 3 mirrors sheet 1 out of
+    plane; the field repeats with the supercell."""
+    orc = oracle_mod
+    A, b = _cell_b(orc)
+    j = orc.Job()
+    j.strain_type = 6
+    for a in range(2):
+        for d in range(2):
+            j.relax_b[a][d] = b[a][d]
+    cz = [-0.061901691526695, -0.0220634404118861, -0.004100201789620, -0.002676762110702]
+    d0 = orc.relax_disp(j, 0.0, 0.0, 0)
+    assert np.allclose(d0, [0.0, 0.0, cz[0] + 6 * sum(cz[1:])], rtol=0, atol=1e-15)
+    rng = np.random.default_rng(3)
+    for x, y in rng.uniform(-30, 30, (20, 2)):
+        u0, u1, u2 = (orc.relax_disp(j, x, y, s) for s in range(3))
+        assert np.hypot(*u0[:2]) > 0
+        assert np.array_equal(u1[:2], -2 * u0[:2]) and u1[2] == 0.0
+        assert np.array_equal(u2[:2], u0[:2]) and u2[2] == -u0[2]
+        for a in A:
+            assert np.allclose(orc.relax_disp(j, x + a[0], y + a[1], 0), u0, rtol=0, atol=1e-12)
