@@ -619,10 +619,52 @@ def cpu_baseline(dev, nnz, n, mu_gpu, log):
     err = float(np.max(np.abs(mu - mu_gpu[:R_s, :M_s])) / n)
     log(f"CPU baseline: {threads} threads, {R_s} vectors x {steps} steps in {dt:.1f}s -> {rate:.3e} nnz.vec/s; "
         f"max|dmu|/mu0 vs GPU = {err:.2e}")
-    return {"value": rate, "unit": "nnz·vec updates/s", "cores": threads, "kind": "port",
-            "sample": f"{R_s} random-phase vectors x {steps} Chebyshev steps (M={M_s}) on the same "
-                      f"{n}-orbital complex H, one vector per thread",
-            "seconds": dt, "max_rel_dmu_vs_gpu": err}
+    out = {"value": rate, "unit": "nnz·vec updates/s", "cores": threads, "kind": "port",
+           "sample": f"{R_s} random-phase vectors x {steps} Chebyshev steps (M={M_s}) on the same "
+                     f"{n}-orbital complex H, one vector per thread",
+           "seconds": dt, "max_rel_dmu_vs_gpu": err}
+    ref = reference_spmv_rate(rp, ci, v, n, log)
+    if ref is not None:
+        out["reference_spmv"] = ref
+    return out
+
+
+def reference_spmv_rate(rp, ci, v, n, log):
+    """The reference's own SpMatrix::vectorMultiply (spmatrix.cpp:270-411, its
+    non-MKL CSR loop, compiled from its sources into oracle/_ref) on the same
+    matrix: one vector, one core, the matrix copied in once outside the timed
+    calls.  Skipped when that library was not built."""
+    import ctypes as C
+
+    from oracle import oracle as orc
+
+    if not os.path.exists(orc.REF_PATH):
+        return None
+    L = C.CDLL(orc.REF_PATH, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    L.ref_spmv_time.restype = C.c_double
+    L.ref_spmv_time.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                C.c_void_p, C.c_int]
+    nnz = len(ci)
+    cpx = np.iscomplexobj(v)
+    rp32 = np.ascontiguousarray(rp, dtype=np.int32)
+    ci32 = np.ascontiguousarray(ci, dtype=np.int32)
+    vv = np.ascontiguousarray(v.view(np.float64) if cpx else v)
+    rng = np.random.default_rng(0)
+    x = np.exp(2j * np.pi * rng.random(n)) if cpx else rng.standard_normal(n)
+    y = np.zeros_like(x)
+
+    def run(reps):
+        return L.ref_spmv_time(n, nnz, rp32.ctypes.data, ci32.ctypes.data, vv.ctypes.data, int(cpx),
+                               x.ctypes.data, y.ctypes.data, reps)
+
+    t1 = run(1)
+    reps = max(1, min(200, int(5.0 / max(t1, 1e-6))))
+    dt = run(reps)
+    rate = nnz * reps / dt
+    log(f"reference SpMatrix::vectorMultiply: {reps} SpMVs in {dt:.1f}s on 1 core -> {rate:.3e} nnz.vec/s")
+    return {"value": rate, "unit": "nnz·vec updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{reps} calls of the reference's SpMatrix::vectorMultiply (y = H x, one vector) on the "
+                      f"same {n}-orbital complex H", "seconds": dt}
 
 
 if __name__ == "__main__":

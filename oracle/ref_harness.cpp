@@ -4,6 +4,7 @@
 // oracle's restatement against the reference's code on this host.  No
 // reference source is copied here; this file only calls the reference's
 // public functions.
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -149,6 +150,39 @@ void ref_spmv(int n, int nnz, const int* row_ptr, const int* col, const double* 
                          reinterpret_cast<std::complex<double>*>(y), std::complex<double>(alpha),
                          std::complex<double>(beta));
     }
+}
+
+// Seconds spent in `reps` calls of SpMatrix::vectorMultiply (y <- H x, the
+// reference's own CSR loop) on a matrix built once: the CPU timing of the
+// reference's SpMV, with the matrix copy outside the timed calls.
+double ref_spmv_time(int n, int nnz, const int* row_ptr, const int* col, const double* val, int is_complex,
+                     const double* x, double* y, int reps)
+{
+    SpMatrix H;
+    H.setup(nnz, n, n);
+    int* ci = H.allocColIndx();
+    int* rp = H.allocRowPtr();
+    std::memcpy(rp, row_ptr, sizeof(int) * (n + 1));
+    std::memcpy(ci, col, sizeof(int) * nnz);
+    double* rv = nullptr;
+    std::complex<double>* cv = nullptr;
+    if (!is_complex) {
+        rv = H.allocRealVal();
+        std::memcpy(rv, val, sizeof(double) * nnz);
+    } else {
+        cv = H.allocCpxVal();
+        std::memcpy(cv, val, sizeof(double) * 2 * nnz);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) {
+        if (!is_complex)
+            H.vectorMultiply(const_cast<double*>(x), y, 1.0, 0.0);
+        else
+            H.vectorMultiply(reinterpret_cast<std::complex<double>*>(const_cast<double*>(x)),
+                             reinterpret_cast<std::complex<double>*>(y), std::complex<double>(1.0),
+                             std::complex<double>(0.0));
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // extern "C"
