@@ -1228,6 +1228,13 @@ def run_deck(text: str, pruned: int = 0):
             vecs = np.array([[dist(gen) for _ in range(geom.n)] for _ in range(S)])
             mu = kpm(rp, ci, v, o["poly_order"], vecs=vecs)
             rho = dos_trace(mu, o["energy_rescale"], o["energy_shift"], 1.0 / geom.n)
+            if o["verbose_save"] == 1:
+                # the verbose writer prints cheb_coeffs, which for a trace job holds the
+                # samples' raw moments, one row per entry of the target list
+                # (param_tools.cpp:39, 45-78; locality.cpp:3112-3130)
+                if S < len(job["target_list"]):
+                    raise DeckError("VERBOSE_SAVE = 1 with fewer trace samples than targets")
+                rho = mu
         else:
             mu = kpm(rp, ci, v, o["poly_order"], targets=job["target_list"])
             rho = dos(mu, o["energy_rescale"], o["energy_shift"]) if o["dos_transform"] == 1 else mu

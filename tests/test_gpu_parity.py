@@ -196,6 +196,7 @@ def _cheb_numbers(text):
 
 @pytest.mark.parametrize("name", ["tblg_test1.in", "tblg_test2.in", "tblg_gshift_strain.in", "tblg_sc_7_6_k.in",
                                   "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_cond_raw.in:verbose",
+                                  "tblg_trace.in:verbose",
                                   "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in", "tblg_sc_7_6_k.in:msave2",
                                   "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
                                   "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in"])
