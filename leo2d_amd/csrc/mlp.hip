@@ -283,13 +283,38 @@ __global__ __launch_bounds__(kT) void mse_grad_kernel(const float* Y, const floa
 }
 
 // db[n] = sum_m dZ[m][n] (bf16 operand values, FP32 sum in row order)
+// db[c] = sum_r dZ[r][c].  A workgroup owns kColsumCols columns; its kColsumGroups
+// row groups each sum a contiguous slice of rows (8 loads in flight), and the
+// slices are added in group order, so the result does not depend on timing.
+constexpr int kColsumCols = 16;
+constexpr int kColsumGroups = kT / kColsumCols;
+
 __global__ __launch_bounds__(kT) void colsum_kernel(const uint16_t* dZ, int rows, int cols, float* db)
 {
-    const int c = blockIdx.x * kT + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[kColsumGroups][kColsumCols];
+    const int cx = threadIdx.x % kColsumCols, g = threadIdx.x / kColsumCols;
+    const int c = blockIdx.x * kColsumCols + cx;
+    const int per = (rows + kColsumGroups - 1) / kColsumGroups;
+    const int r0 = min(rows, g * per), r1 = min(rows, r0 + per);
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += bf2f(dZ[(int64_t)r * cols + c]);
-    db[c] = s;
+    if (c < cols) {
+        int r = r0;
+        for (; r + 8 <= r1; r += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = bf2f(dZ[(int64_t)(r + u) * cols + c]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; r < r1; ++r) s += bf2f(dZ[(int64_t)r * cols + c]);
+    }
+    part[g][cx] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        float t = part[0][cx];
+        for (int q = 1; q < kColsumGroups; ++q) t += part[q][cx];
+        db[c] = t;
+    }
 }
 
 // rows idx[b] of X (FP32) -> bf16 batch; rows idx[b] of T -> FP32 batch
@@ -422,7 +447,7 @@ hipError_t launch_mse_grad(const float* Y, const float* T, int rows, int cols, f
 
 hipError_t launch_colsum(const uint16_t* dZ, int rows, int cols, float* db, hipStream_t s)
 {
-    hipLaunchKernelGGL(colsum_kernel, dim3((cols + kT - 1) / kT), dim3(kT), 0, s, dZ, rows, cols, db);
+    hipLaunchKernelGGL(colsum_kernel, dim3((cols + kColsumCols - 1) / kColsumCols), dim3(kT), 0, s, dZ, rows, cols, db);
     return hipGetLastError();
 }
 
