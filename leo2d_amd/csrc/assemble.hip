@@ -177,11 +177,19 @@ __device__ inline int clamp_cell(double v, int lo, int hi)
     return f - lo;
 }
 
+// Each supercell image of the search yields its candidates in increasing column
+// order (the windows walk i, j, l ascending, and the sheet below precedes the
+// sheet above), so a list is a run of sorted runs, one per image; run_end
+// records where each ends so the write pass can merge instead of sorting.
+constexpr int kMaxRuns = 49;   // (2 S + 1)^2 images for SUPERCELL_SEARCH_SIZE S <= 3
+
 struct PairSink {
     int32_t* col;
     int32_t* sc;   // (dx & 0xffff) | (dy << 16)
     int64_t pos;
     int count;
+    int nrun;
+    int run_end[kMaxRuns];
     __device__ inline void push(int c, int dx, int dy)
     {
         if (col) {
@@ -190,6 +198,11 @@ struct PairSink {
             ++pos;
         }
         ++count;
+    }
+    __device__ inline void end_image()
+    {
+        if (nrun < kMaxRuns) run_end[nrun] = count;
+        ++nrun;
     }
 };
 
@@ -224,6 +237,7 @@ __device__ void search_intra(const GeomDev& g, int kh, PairSink& out)
                         const double t = intralayer(sd.mat, l0, l, gd0, gd1);
                         if (t != 0 || (dx == 0 && dy == 0)) out.push(k2 + sd.base, dx, dy);
                     }
+            out.end_image();
         }
 }
 
@@ -270,6 +284,7 @@ __device__ void search_inter(const GeomDev& g, int kh, PairSink& out)
                             if ((x2 - p0) * (x2 - p0) + (y2 - p1) * (y2 - p1) < cut2) out.push((int)K, dx, dy);
                         }
             }
+            out.end_image();
         }
 }
 
@@ -308,21 +323,67 @@ __device__ inline int64_t distinct_runs(const int32_t* col, int64_t n)
     return r;
 }
 
+// Stable merge of a list's per-image sorted runs (tcol/tsc) into col/sc: at equal
+// columns the earlier image wins, which is the order the reference's stable
+// sort by column leaves (Sheet::orderPairs / Hstruct::orderPairs).
+__device__ void merge_runs(const int32_t* tcol, const int32_t* tsc, const PairSink& k, int32_t* col, int32_t* sc)
+{
+    int head[kMaxRuns];
+    int start = 0;
+    for (int r = 0; r < k.nrun; ++r) {
+        head[r] = start;
+        start = k.run_end[r];
+    }
+    for (int o = 0; o < k.count; ++o) {
+        int best = -1;
+        int32_t bc = 0;
+        for (int r = 0; r < k.nrun; ++r)
+            if (head[r] < k.run_end[r]) {
+                const int32_t c = tcol[head[r]];
+                if (best < 0 || c < bc) {
+                    bc = c;
+                    best = r;
+                }
+            }
+        col[o] = bc;
+        sc[o] = tsc[head[best]];
+        ++head[best];
+    }
+}
+
+// The pairs of one list, in the reference's order: merged from the image runs in
+// the scratch copy when the list is sorted by column (periodic sheets), copied
+// as found otherwise; more images than kMaxRuns fall back to the insertion sort.
+__device__ void place_pairs(const int32_t* tcol, const int32_t* tsc, const PairSink& k, bool sorted, int32_t* col,
+                            int32_t* sc)
+{
+    if (sorted && k.nrun <= kMaxRuns) {
+        merge_runs(tcol, tsc, k, col, sc);
+        return;
+    }
+    for (int o = 0; o < k.count; ++o) {
+        col[o] = tcol[o];
+        sc[o] = tsc[o];
+    }
+    if (sorted) sort_by_col(col, sc, k.count);
+}
+
 __global__ __launch_bounds__(kThreads) void pair_write_kernel(GeomDev g, const int64_t* pair_rp, const int32_t* cnt_intra,
-                                                              int32_t* pcol, int32_t* psc, int64_t* nslots)
+                                                              int32_t* tcol, int32_t* tsc, int32_t* pcol, int32_t* psc,
+                                                              int64_t* nslots)
 {
     const int kh = blockIdx.x * kThreads + threadIdx.x;
     if (kh >= g.n) return;
     const int64_t base = pair_rp[kh];
-    PairSink a = {pcol, psc, base, 0};
+    PairSink a = {tcol, tsc, base, 0};
     search_intra(g, kh, a);
-    PairSink b = {pcol, psc, base + a.count, 0};
+    PairSink b = {tcol, tsc, base + a.count, 0};
     search_inter(g, kh, b);
     const int ni = cnt_intra[kh];
     const int64_t nt = pair_rp[kh + 1] - base;
     const int sh = g.i2g[4 * (size_t)kh + 3];
-    if (g.s[sh].bc == 1) sort_by_col(pcol + base, psc + base, ni);
-    if (g.bc == 1) sort_by_col(pcol + base + ni, psc + base + ni, nt - ni);
+    place_pairs(tcol + base, tsc + base, a, g.s[sh].bc == 1, pcol + base, psc + base);
+    place_pairs(tcol + base + ni, tsc + base + ni, b, g.bc == 1, pcol + base + ni, psc + base + ni);
     nslots[kh] = distinct_runs(pcol + base, ni) + distinct_runs(pcol + base + ni, nt - ni);
 }
 
@@ -619,10 +680,11 @@ hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total,
     return scan_counts(cnt_total, pair_rp, g.n, s);
 }
 
-hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* pcol, int32_t* psc,
-                       int64_t* nslots, int64_t* slot_rp, hipStream_t s)
+hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* tcol, int32_t* tsc,
+                       int32_t* pcol, int32_t* psc, int64_t* nslots, int64_t* slot_rp, hipStream_t s)
 {
-    hipLaunchKernelGGL(pair_write_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, pair_rp, cnt_intra, pcol, psc, nslots);
+    hipLaunchKernelGGL(pair_write_kernel, rows_grid(g.n), dim3(kThreads), 0, s, g, pair_rp, cnt_intra, tcol, tsc, pcol,
+                       psc, nslots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return scan_counts(nslots, slot_rp, g.n, s);

@@ -898,8 +898,12 @@ int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64
         dalloc(h->psc, sizeof(int32_t) * (size_t)total, "pair images");
         dalloc(h->nslots, sizeof(int64_t) * n, "slot counts");
         dalloc(h->slot_rp, sizeof(int64_t) * (n + 1), "slot offsets");
+        DevBuf tcol, tsc;   // the pairs as found, before the per-image runs are merged
+        dalloc(tcol, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
+        dalloc(tsc, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
         check(leo2d::pairs_write(h->gdev, (const int64_t*)h->pair_rp.p, (const int32_t*)h->cnt_intra.p,
-                                 (int32_t*)h->pcol.p, (int32_t*)h->psc.p, (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
+                                 (int32_t*)tcol.p, (int32_t*)tsc.p, (int32_t*)h->pcol.p, (int32_t*)h->psc.p,
+                                 (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
               "pair write");
         int64_t slots = 0;
         check(hipMemcpyAsync(&slots, (int64_t*)h->slot_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "slot total");
