@@ -36,6 +36,20 @@
 namespace leo2d {
 namespace {
 
+// a += the value held by lane (lane ^ off), per component
+__device__ inline void xor_add(double2& a, int off)
+{
+    a.x += __shfl_xor(a.x, off);
+    a.y += __shfl_xor(a.y, off);
+}
+__device__ inline void xor_add(float2& a, int off)
+{
+    a.x += __shfl_xor(a.x, off);
+    a.y += __shfl_xor(a.y, off);
+}
+__device__ inline void xor_add(double& a, int off) { a += __shfl_xor(a, off); }
+__device__ inline void xor_add(float& a, int off) { a += __shfl_xor(a, off); }
+
 template <typename HT, typename XT, int R, int MODE>
 __global__ __launch_bounds__(kThreads) void batch_step_kernel(BatchStepArgs a)
 {
@@ -110,10 +124,7 @@ __global__ __launch_bounds__(kThreads) void batch_step_kernel(BatchStepArgs a)
                 if (q < q1) mac(acc, val[q], cur[(size_t)col[q] * R + r]);
                 // sum the entry slots: every lane ends with the row total
 #pragma unroll
-                for (int off = R; off < 64; off <<= 1) {
-                    acc.x += __shfl_xor(acc.x, off);
-                    acc.y += __shfl_xor(acc.y, off);
-                }
+                for (int off = R; off < 64; off <<= 1) xor_add(acc, off);
                 if (grp == 0) {
                     const size_t idx = (size_t)row * R + r;
                     const XT cv = cur[idx];
@@ -319,7 +330,9 @@ __global__ void batch_init_kernel(void* x, int vec_type, const int64_t* job_row0
             sincos(phi, &sn, &cs);
         }
         if (vec_type == VEC_CPX_F64) static_cast<double2*>(x)[t] = make_double2(cs, sn);
-        else static_cast<float2*>(x)[t] = make_float2((float)cs, (float)sn);
+        else if (vec_type == VEC_CPX_F32) static_cast<float2*>(x)[t] = make_float2((float)cs, (float)sn);
+        else if (vec_type == VEC_REAL_F64) static_cast<double*>(x)[t] = cs;
+        else static_cast<float*>(x)[t] = (float)cs;
     }
 }
 
@@ -374,6 +387,9 @@ hipError_t launch_batch_step(int mat_type, int vec_type, const BatchStepArgs& a,
     if (mat_type == MAT_REAL_F64 && vec_type == VEC_CPX_F64) return launch_r<double, double2>(a, s);
     if (mat_type == MAT_CPX_F32 && vec_type == VEC_CPX_F32) return launch_r<float2, float2>(a, s);
     if (mat_type == MAT_REAL_F32 && vec_type == VEC_CPX_F32) return launch_r<float, float2>(a, s);
+    // real H with real start vectors (target starts): real vector blocks, half the gathered bytes
+    if (mat_type == MAT_REAL_F64 && vec_type == VEC_REAL_F64) return launch_r<double, double>(a, s);
+    if (mat_type == MAT_REAL_F32 && vec_type == VEC_REAL_F32) return launch_r<float, float>(a, s);
     return hipErrorInvalidValue;
 }
 
@@ -388,7 +404,10 @@ hipError_t launch_batch_reduce(const double* part, const int32_t* job_tile0, int
 hipError_t launch_batch_init(void* x, int vec_type, const int64_t* job_row0, int P, int64_t nrows, int R, uint64_t seed,
                              const int32_t* targets, hipStream_t s)
 {
-    if (vec_type != VEC_CPX_F64 && vec_type != VEC_CPX_F32) return hipErrorInvalidValue;
+    // real vectors hold unit-vector starts only (random phases are complex)
+    const bool real = vec_type == VEC_REAL_F64 || vec_type == VEC_REAL_F32;
+    if (!real && vec_type != VEC_CPX_F64 && vec_type != VEC_CPX_F32) return hipErrorInvalidValue;
+    if (real && !targets) return hipErrorInvalidValue;
     const size_t total = (size_t)nrows * R;
     const int blocks = (int)std::min<size_t>(8192, (total + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(batch_init_kernel, dim3(std::max(blocks, 1)), dim3(kThreads), 0, s, x, vec_type, job_row0, P,

@@ -1861,7 +1861,10 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         }
         const int mat_type = B.is_complex ? (mixed ? leo2d::MAT_CPX_F32 : leo2d::MAT_CPX_F64)
                                           : (mixed ? leo2d::MAT_REAL_F32 : leo2d::MAT_REAL_F64);
-        const int vec_type = mixed ? leo2d::VEC_CPX_F32 : leo2d::VEC_CPX_F64;
+        // a real H from unit-vector starts keeps real vectors (as the single-job path does)
+        const bool real_vec = !B.is_complex && p->start == LEO2D_START_TARGETS;
+        const int vec_type = real_vec ? (mixed ? leo2d::VEC_REAL_F32 : leo2d::VEC_REAL_F64)
+                                      : (mixed ? leo2d::VEC_CPX_F32 : leo2d::VEC_CPX_F64);
         const void* dval = B.val.p;
         if (mixed) {
             if (!B.val32_valid) {
