@@ -149,6 +149,23 @@ Deck parse_deck(const std::string& text)
                 deck.angles.assign(num_sheets, 0.0);
                 deck.has_material.assign(num_sheets, 0);
                 o.i["num_sheets"] = num_sheets;
+            } else if (key == "SUPERCELL_GROUPS") {
+                // main.cpp:131-155: the sheets (1-based) of group 1, "/", then those of group 2,
+                // until NUM_SHEETS sheets are placed
+                tk.next();
+                std::vector<int> groups((size_t)std::max(num_sheets, 0), 0);
+                int placed = 0;
+                for (int grp = 0; grp < 2; ++grp) {
+                    while (placed < num_sheets) {
+                        const std::string v = tk.next();
+                        if (v == "/") break;
+                        const int sh = std::atoi(v.c_str());
+                        if (v.empty() || sh < 1 || sh > num_sheets) throw DeckError("SUPERCELL_GROUPS names no sheet '" + v + "'");
+                        groups[(size_t)sh - 1] = grp;
+                        ++placed;
+                    }
+                }
+                o.iv["sc_groups"] = groups;
             } else if (key == "SUPERCELL_M_N") {
                 tk.next();
                 o.i["m_supercell"] = std::atoi(tk.next().c_str());
@@ -298,9 +315,16 @@ void setup_supercell(Deck& deck)
         const int M = (int)o.geti("m_supercell"), N = (int)o.geti("n_supercell");
         double theta = std::acos((N * N + 4 * N * M + M * M) / (2.0 * (N * N + N * M + M * M)));
         if (M < N) theta = -theta;
-        if (ns >= 3) throw DeckError("SUPERCELL_M_N with more than two sheets needs SUPERCELL_GROUPS (not on this path)");
+        // sc_groups = {0, 1} for one or two sheets, else the deck's SUPERCELL_GROUPS (locality.cpp:150-170)
+        std::vector<int> groups = {0, 1};
+        if (ns >= 3) {
+            auto it = o.iv.find("sc_groups");
+            if (it == o.iv.end() || (int)it->second.size() < ns)
+                throw DeckError("SUPERCELL_M_N with more than two sheets needs SUPERCELL_GROUPS");
+            groups = it->second;
+        }
         for (int s = 0; s < ns; ++s) {
-            const int group = s;  // sc_groups = {0, 1} (locality.cpp:156-160)
+            const int group = groups[s];
             int A[2][2];
             if (group == 0) {
                 deck.angles[s] = 0;

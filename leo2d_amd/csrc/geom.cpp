@@ -297,7 +297,12 @@ std::vector<KVec> k_path(const Deck& d)
         // (M, N) cell zone: Gamma -> K -> M -> Gamma, nk points per line
         if (o.geti("supercell_type") != 1) throw DeckError("K_TYPE = 4 needs SUPERCELL_M_N");
         const int M = (int)o.geti("m_supercell"), N = (int)o.geti("n_supercell");
-        const int A[2][2] = {{M > N ? N : M, M > N ? M : N}, {M > N ? -M : -N, M + N}};
+        // two supercell groups (one or two sheets): the bilayer cell; three sheets: the
+        // trilayer cell A = ((0, t), (-t, t)), t = M^2 + M N + N^2 (locality.cpp:2018-2040)
+        if (ns > 3) throw DeckError("K_TYPE = 4 takes at most three sheets");
+        const int tt = M * M + M * N + N * N;
+        const int A[2][2] = {{ns < 3 ? (M > N ? N : M) : 0, ns < 3 ? (M > N ? M : N) : tt},
+                             {ns < 3 ? (M > N ? -M : -N) : -tt, ns < 3 ? M + N : tt}};
         const double a = 2.4768;
         const double uc[2][2] = {{1. * a, 0.}, {.5 * a, lattice_s32(d.sheets[0].mat) * a}};
         std::vector<std::vector<double>> sc(2, std::vector<double>(2));

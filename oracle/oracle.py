@@ -300,6 +300,22 @@ def parse_deck(text: str) -> Deck:
                 heights = [0.0] * num_sheets
                 angles = [0.0] * num_sheets
                 opts["num_sheets"] = num_sheets
+            elif tok == "SUPERCELL_GROUPS":
+                # main.cpp:131-155
+                nxt()
+                groups = [0] * num_sheets
+                placed = 0
+                for grp in range(2):
+                    while placed < num_sheets:
+                        v = nxt()
+                        if v == "/":
+                            break
+                        sh = atoi(v)
+                        if v == "" or not 1 <= sh <= num_sheets:
+                            raise DeckError(f"SUPERCELL_GROUPS names no sheet {v!r}")
+                        groups[sh - 1] = grp
+                        placed += 1
+                opts["sc_groups"] = groups
             elif tok == "SUPERCELL_M_N":
                 nxt(); opts["m_supercell"] = atoi(nxt()); opts["n_supercell"] = atoi(nxt())
                 opts["supercell_type"] = 1
@@ -447,6 +463,8 @@ def setup_supercell(deck: Deck) -> None:
         theta = math.acos((N * N + 4 * N * M + M * M) / (2.0 * (N * N + N * M + M * M)))
         if M < N:
             theta = -theta
+        if len(deck.sdata) >= 3 and "sc_groups" not in o:
+            raise DeckError("SUPERCELL_M_N with more than two sheets needs SUPERCELL_GROUPS")
         groups = [0, 1] if len(deck.sdata) < 3 else o["sc_groups"]
         for i, sd in enumerate(deck.sdata):
             g = groups[i]
@@ -702,6 +720,8 @@ def k_path(deck: Deck) -> list:
         if o.get("supercell_type") != 1:
             raise DeckError("K_TYPE = 4 needs SUPERCELL_M_N")
         M, N = o["m_supercell"], o["n_supercell"]
+        if ns > 3:
+            raise DeckError("K_TYPE = 4 takes at most three sheets")
         groups = [0, 1] if ns < 3 else o["sc_groups"]
         if len(groups) == 2:
             A = ((N, M), (-M, M + N)) if M > N else ((M, N), (-N, M + N))

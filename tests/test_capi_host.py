@@ -35,7 +35,7 @@ ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
              "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
              "ttlg_grid_k5.in", "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
-             "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in"]
+             "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in", "ttg_sc_7_6.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -105,6 +105,8 @@ def _k_variant(name, k_type, nk, *edits):
     ("ttlg_grid_k5.in", 5, 1, ()),
     ("ttlg_grid_k5.in", 5, 6, ()),
     ("ttlg_grid_k5.in", 2, 4, ()),
+    ("ttg_sc_7_6.in", 4, 5, ()),          # the trilayer (M, N) cell of K_TYPE 4
+    ("ttg_sc_7_6.in", 5, 3, ()),
 ])
 def test_k_paths_match_oracle(oracle_mod, name, k_type, nk, edits):
     """K_TYPE 1-5 (locality.cpp:1655-2249): the library's job list has the oracle's
@@ -119,7 +121,9 @@ def test_k_paths_match_oracle(oracle_mod, name, k_type, nk, edits):
     assert len(ojobs) % len(oracle_mod.k_path(od)) == 0
     for j, oj in zip(jobs, ojobs):
         assert j.job_id == oj["jobID"]
-        assert (j.k[0], j.k[1]) == tuple(oj["k_vec"])
+        # bit for bit, NaN included: the reference's slope form of K_TYPE 4 divides by zero when K
+        # lies on the y axis (the trilayer cell), and both give the same NaN components
+        assert np.array([j.k[0], j.k[1]]).tobytes() == np.array(oj["k_vec"], dtype=float).tobytes()
         assert tuple(j.shift[len(od.sdata) - 1][:2]) == tuple(oj["shifts"][-1][:2])
 
 
@@ -191,6 +195,14 @@ def test_commensurate_cell_count():
     assert g.n == 4 * (6 * 6 + 6 * 7 + 7 * 7)
     g = L.Geometry(L.Deck(deck_text("tblg_c0_twin.in")))
     assert g.n == 4 * (24 * 24 + 24 * 28 + 28 * 28) == 8128
+    # three sheets in supercell groups {1, 3} and {2}: 2 (N^2 + N M + M^2) orbitals per sheet,
+    # the middle sheet turned by the commensurate angle
+    g = L.Geometry(L.Deck(deck_text("ttg_sc_7_6.in")))
+    assert g.n == 3 * 2 * (6 * 6 + 6 * 7 + 7 * 7)
+    assert g.info.angle[0] == g.info.angle[2] == 0.0 and g.info.angle[1] > 0
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(deck_text("ttg_sc_7_6.in").replace("SUPERCELL_GROUPS = 1 3 / 2\n", "")))
+    assert e.value.code == L.ERR_DECK
 
 
 @pytest.mark.parametrize("i,tiles", [(2, 1), (3, 1), (9, 1), (6, 2)])
