@@ -33,6 +33,8 @@ Params::Params()
     i["global_shifts_on"] = 0;
     i["strain_type"] = 0;
     i["uniform_strain"] = 0;
+    i["h_supercell_modify"] = 0;     // job_params.cpp:55-56
+    i["hex_supercell_modify"] = 0;
     i["matrix_save"] = 0;
     i["matrix_pos_save"] = 0;
     i["verbose_save"] = 1;
@@ -166,7 +168,9 @@ Deck parse_deck(const std::string& text)
                     }
                 }
                 o.iv["sc_groups"] = groups;
-            } else if (key == "SUPERCELL_M_N") {
+            } else if (key == "H_SUPERCELL_MODIFY") o.i["h_supercell_modify"] = std::atoi(val().c_str());
+            else if (key == "HEX_SUPERCELL_MODIFY") o.i["hex_supercell_modify"] = std::atoi(val().c_str());
+            else if (key == "SUPERCELL_M_N") {
                 tk.next();
                 o.i["m_supercell"] = std::atoi(tk.next().c_str());
                 o.i["n_supercell"] = std::atoi(tk.next().c_str());
@@ -315,6 +319,13 @@ void setup_supercell(Deck& deck)
         const int M = (int)o.geti("m_supercell"), N = (int)o.geti("n_supercell");
         double theta = std::acos((N * N + 4 * N * M + M * M) / (2.0 * (N * N + N * M + M * M)));
         if (M < N) theta = -theta;
+        // H_SUPERCELL_MODIFY: the second group turned by a further 180 degrees, its basis with it
+        // (locality.cpp:178-185)
+        int h_fac = 1;
+        if (o.geti("h_supercell_modify") == 1) {
+            theta = theta + M_PI;
+            h_fac = -1;
+        }
         // sc_groups = {0, 1} for one or two sheets, else the deck's SUPERCELL_GROUPS (locality.cpp:150-170)
         std::vector<int> groups = {0, 1};
         if (ns >= 3) {
@@ -331,7 +342,13 @@ void setup_supercell(Deck& deck)
                 A[0][0] = N; A[0][1] = M; A[1][0] = -M; A[1][1] = M + N;
             } else {
                 deck.angles[s] = theta;
-                A[0][0] = M; A[0][1] = N; A[1][0] = -N; A[1][1] = M + N;
+                A[0][0] = h_fac * M; A[0][1] = h_fac * N; A[1][0] = -h_fac * N; A[1][1] = h_fac * (M + N);
+            }
+            // HEX_SUPERCELL_MODIFY: first components of both supercell vectors take the second's too
+            // (locality.cpp:222-233)
+            if (o.geti("hex_supercell_modify") == 1) {
+                A[0][0] = A[0][0] + A[0][1];
+                A[1][0] = A[1][0] + A[1][1];
             }
             SheetData& sd = deck.sheets[s];
             for (int r = 0; r < 2; ++r) {

@@ -300,6 +300,10 @@ def parse_deck(text: str) -> Deck:
                 heights = [0.0] * num_sheets
                 angles = [0.0] * num_sheets
                 opts["num_sheets"] = num_sheets
+            elif tok == "H_SUPERCELL_MODIFY":
+                nxt(); opts["h_supercell_modify"] = atoi(nxt())
+            elif tok == "HEX_SUPERCELL_MODIFY":
+                nxt(); opts["hex_supercell_modify"] = atoi(nxt())
             elif tok == "SUPERCELL_GROUPS":
                 # main.cpp:131-155
                 nxt()
@@ -463,6 +467,10 @@ def setup_supercell(deck: Deck) -> None:
         theta = math.acos((N * N + 4 * N * M + M * M) / (2.0 * (N * N + N * M + M * M)))
         if M < N:
             theta = -theta
+        h = 1                          # locality.cpp:178-185
+        if o.get("h_supercell_modify", 0) == 1:
+            theta = theta + math.pi
+            h = -1
         if len(deck.sdata) >= 3 and "sc_groups" not in o:
             raise DeckError("SUPERCELL_M_N with more than two sheets needs SUPERCELL_GROUPS")
         groups = [0, 1] if len(deck.sdata) < 3 else o["sc_groups"]
@@ -473,7 +481,9 @@ def setup_supercell(deck: Deck) -> None:
                 A = ((N, M), (-M, M + N))
             else:
                 deck.angles[i] = theta
-                A = ((M, N), (-N, M + N))
+                A = ((h * M, h * N), (-h * N, h * (M + N)))
+            if o.get("hex_supercell_modify", 0) == 1:      # locality.cpp:222-233
+                A = ((A[0][0] + A[0][1], A[0][1]), (A[1][0] + A[1][1], A[1][1]))
             sc = [[A[0][0] * uc[0][0] + A[0][1] * uc[1][0], A[0][0] * uc[0][1] + A[0][1] * uc[1][1]],
                   [A[1][0] * uc[0][0] + A[1][1] * uc[1][0], A[1][0] * uc[0][1] + A[1][1] * uc[1][1]]]
             sd.supercell = sc

@@ -2,6 +2,7 @@
 symbol include/leo2d.h declares; the product's deck parser, geometry and job
 list reproduce the oracle's bit for bit; errors surface as codes."""
 import ctypes as C
+import math
 import os
 import re
 
@@ -35,7 +36,8 @@ ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x
              "tblg_gshift_strain.in", "tblg_simple_flake.in", "tblg_strain_flake.in", "tblg_trace.in",
              "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
              "ttlg_grid_k5.in", "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
-             "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in", "ttg_sc_7_6.in"]
+             "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in", "ttg_sc_7_6.in",
+             "tblg_sc_hex.in", "tblg_sc_h.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)
@@ -203,6 +205,13 @@ def test_commensurate_cell_count():
     with pytest.raises(L.Leo2dError) as e:
         L.Geometry(L.Deck(deck_text("ttg_sc_7_6.in").replace("SUPERCELL_GROUPS = 1 3 / 2\n", "")))
     assert e.value.code == L.ERR_DECK
+    # HEX_SUPERCELL_MODIFY keeps the cell's area (A11 += A12, A21 += A22 leaves det A unchanged) and
+    # H_SUPERCELL_MODIFY negates the second group's basis and turns it by a further pi
+    base = L.Geometry(L.Deck(deck_text("tblg_sc_7_6_k.in")))
+    for name, dtheta in (("tblg_sc_hex.in", 0.0), ("tblg_sc_h.in", math.pi)):
+        g = L.Geometry(L.Deck(deck_text(name)))
+        assert g.n == base.n == 4 * (6 * 6 + 6 * 7 + 7 * 7)
+        assert g.info.angle[1] == base.info.angle[1] + dtheta
 
 
 @pytest.mark.parametrize("i,tiles", [(2, 1), (3, 1), (9, 1), (6, 2)])

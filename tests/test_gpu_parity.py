@@ -26,7 +26,7 @@ DECKS = os.path.join(ROOT, "tests", "decks")
 PARITY_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x3.in", "tblg_gshift_strain.in",
                 "tblg_simple_flake.in", "tblg_c0_twin.in", "tblg_strain_flake.in", "tblg_trace.in", "tblg_relax_cell.in",
                 "tblg_strained_flake.in", "tblg_strained_uniform.in", "tblg_strained_relax.in", "tblg_direct_cell.in",
-                "tblg_strained_direct.in", "ttg_sc_7_6.in"]
+                "tblg_strained_direct.in", "ttg_sc_7_6.in", "tblg_sc_hex.in", "tblg_sc_h.in"]
 
 
 def deck_text(name):
