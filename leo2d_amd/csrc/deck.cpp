@@ -168,7 +168,14 @@ Deck parse_deck(const std::string& text)
                     }
                 }
                 o.iv["sc_groups"] = groups;
-            } else if (key == "H_SUPERCELL_MODIFY") o.i["h_supercell_modify"] = std::atoi(val().c_str());
+            } else if (key == "CHIRAL_ON") o.i["chiral_on"] = std::atoi(val().c_str());
+            else if (key == "BALLISTIC_TRANSPORT") o.i["ballistic_transport"] = std::atoi(val().c_str());
+            else if (key == "MAT_SOC_ON") o.i["mat_soc_on"] = std::atoi(val().c_str());
+            else if (key == "MAT_FROM_FILE") o.i["mat_from_file"] = std::atoi(val().c_str());
+            else if (key == "MLMC") o.i["mlmc"] = std::atoi(val().c_str());
+            else if (key == "GSFE_Z_ON") o.i["gsfe_z_on"] = std::atoi(val().c_str());
+            else if (key == "TRILAYER_SUPERCELL") o.i["supercell_type"] = 3;   // main.cpp:294-302
+            else if (key == "H_SUPERCELL_MODIFY") o.i["h_supercell_modify"] = std::atoi(val().c_str());
             else if (key == "HEX_SUPERCELL_MODIFY") o.i["hex_supercell_modify"] = std::atoi(val().c_str());
             else if (key == "SUPERCELL_M_N") {
                 tk.next();

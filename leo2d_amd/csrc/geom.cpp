@@ -65,6 +65,9 @@ Geom build_geometry(const Deck& d)
     if (st != 0 && o.geti("boundary_condition") != 1)
         throw DeckError("STRAIN_TYPE SUPERCELL and DIRECT need a periodic supercell");
     if (o.geti("mat_from_file") != 0) throw DeckError("MAT_FROM_FILE is not on this path");
+    // keys whose observables or couplings lie outside this path: refused rather than ignored
+    for (const char* k : {"chiral_on", "ballistic_transport", "mat_soc_on", "mlmc", "gsfe_z_on"})
+        if (o.i.count(k) && o.geti(k) != 0) throw DeckError(std::string(k) + " = 1 is not on this path");
     g.num_sheets = (int)d.sheets.size();
     g.bc = (int)o.geti("boundary_condition");
     g.sc_search = (int)o.geti("sc_search_size");

@@ -300,6 +300,10 @@ def parse_deck(text: str) -> Deck:
                 heights = [0.0] * num_sheets
                 angles = [0.0] * num_sheets
                 opts["num_sheets"] = num_sheets
+            elif tok in ("CHIRAL_ON", "BALLISTIC_TRANSPORT", "MAT_SOC_ON", "MAT_FROM_FILE", "MLMC", "GSFE_Z_ON"):
+                nxt(); opts[tok.lower()] = atoi(nxt())
+            elif tok == "TRILAYER_SUPERCELL":
+                opts["supercell_type"] = 3
             elif tok == "H_SUPERCELL_MODIFY":
                 nxt(); opts["h_supercell_modify"] = atoi(nxt())
             elif tok == "HEX_SUPERCELL_MODIFY":
@@ -511,6 +515,9 @@ def geom_desc(deck: Deck) -> GeomDesc:
     if d.num_sheets > MAX_SHEETS:
         raise DeckError("too many sheets")
     o = deck.opts
+    for k in ("chiral_on", "ballistic_transport", "mat_soc_on", "mlmc", "gsfe_z_on", "mat_from_file"):
+        if o.get(k, 0) != 0:
+            raise DeckError(f"{k} = 1 is outside this path")
     d.boundary_condition = o["boundary_condition"]
     d.sc_search_size = o["sc_search_size"]
     if d.boundary_condition == 1:

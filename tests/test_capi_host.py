@@ -237,6 +237,19 @@ def test_deck_errors_are_codes(edit, needle):
     assert needle.lower() in str(e.value).lower()
 
 
+@pytest.mark.parametrize("line", ["CHIRAL_ON = 1", "BALLISTIC_TRANSPORT = 1", "MAT_SOC_ON = 1", "MAT_FROM_FILE = 1",
+                                  "MLMC = 1", "GSFE_Z_ON = 1"])
+def test_keys_outside_the_path_are_refused(oracle_mod, line):
+    """Keys that switch on observables or couplings this path does not compute are
+    refused (LEO2D_ERR_DECK) rather than silently ignored; the oracle refuses them too."""
+    text = deck_text("tblg_test1.in") + "\n" + line + "\n"
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(text))
+    assert e.value.code == L.ERR_DECK
+    with pytest.raises(oracle_mod.DeckError):
+        oracle_mod.geom_desc(oracle_mod.parse_deck(text))
+
+
 def test_truncated_deck_is_an_error():
     text = deck_text("tblg_test1.in")
     with pytest.raises(L.Leo2dError):
