@@ -445,6 +445,18 @@ std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
                 }
         };
         rec(nss);
+    } else if (st == 6) {
+        // FIXED_SHIFT: one job, sheet 1 unshifted and every other sheet at (1/3, 1/3)
+        // (locality.cpp:1580-1613)
+        leo2d_job j = proto;
+        j.job_id = 1;
+        j.max_jobs = 1;
+        for (int s = 0; s < ns; ++s) {
+            j.shift[s][0] = s == 0 ? 0.0 : 1.0 / 3.0;
+            j.shift[s][1] = s == 0 ? 0.0 : 1.0 / 3.0;
+            j.shift[s][2] = 0.0;
+        }
+        jobs.push_back(j);
     } else if (st == 2) {
         // linecut jobs (locality.cpp:1330-1388)
         const int max_jobs = (int)o.geti("nShifts");
@@ -465,7 +477,7 @@ std::vector<leo2d_job> build_jobs(const Deck& d, const Geom& g)
             jobs.push_back(j);
         }
     } else {
-        throw DeckError("SOLVER_TYPE " + std::to_string(st) + " is not on this path (SQ and LC are)");
+        throw DeckError("SOLVER_TYPE " + std::to_string(st) + " is not on this path (SQ, LC and FIXED_SHIFT are)");
     }
     if (o.geti("k_sampling") == 1 && o.geti("k_type") != 0) {
         if (g.bc != 1) throw DeckError("K_SAMPLING needs BOUNDARY_CONDITION = 1");

@@ -845,6 +845,10 @@ def build_jobs(deck: Deck, targets: list) -> list:
                     rec(shifts, level - 1)
 
         rec([[0.0, 0.0, 0.0] for _ in range(ns)], nss)
+    elif st == 6:
+        # FIXED_SHIFT (locality.cpp:1580-1613)
+        shifts = [[0.0, 0.0, 0.0] if s == 0 else [1.0 / 3.0, 1.0 / 3.0, 0] for s in range(ns)]
+        jobs.append({"shifts": shifts, "jobID": 1, "max_jobs": 1, "target_list": list(targets)})
     elif st == 2:
         max_jobs = o["nShifts"]
         lc = o["lc_points"]

@@ -37,7 +37,7 @@ ALL_DECKS = ["tblg_test1.in", "tblg_test2.in", "tblg_sc_7_6_k.in", "tblg_grid_3x
              "tblg_cond_flake.in", "tblg_cond_raw.in", "tblg_pos_save.in", "tblg_sc_k1.in", "mlg_strain_k2.in",
              "ttlg_grid_k5.in", "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
              "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in", "ttg_sc_7_6.in",
-             "tblg_sc_hex.in", "tblg_sc_h.in"]
+             "tblg_sc_hex.in", "tblg_sc_h.in", "tblg_fixed_shift.in"]
 
 
 @pytest.mark.parametrize("name", ALL_DECKS)

@@ -200,7 +200,7 @@ def _cheb_numbers(text):
                                   "tblg_sc_k1.in", "mlg_strain_k2.in", "ttlg_grid_k5.in", "tblg_sc_7_6_k.in:msave2",
                                   "tblg_relax_cell.in", "tblg_strained_flake.in", "tblg_strained_uniform.in",
                                   "tblg_strained_relax.in", "tblg_direct_cell.in", "tblg_strained_direct.in",
-                                  "ttg_sc_7_6.in"])
+                                  "ttg_sc_7_6.in", "tblg_fixed_shift.in"])
 def test_run_deck_cheb_matches_oracle(dev, oracle_mod, tmp_path, name):
     name, _, variant = name.partition(":")
     text = deck_text(name)
