@@ -597,6 +597,8 @@ def kernel_name(variant, t):
         return f"kpm_step_pair_kernel<{t},{8 if variant == 15 else 16}>"
     if variant == 14:
         return f"kpm_step_xchunk_kernel<{t},{t},4>"
+    if variant == 20:
+        return f"kpm_step_sell_kernel<{t},{t},{1 if t == 'double2' else 2},{4 if t == 'double2' else 2}>"
     return f"kpm step variant {variant}"
 
 

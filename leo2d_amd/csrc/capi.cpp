@@ -6,6 +6,7 @@
 // the KPM recursion.  No exception crosses the ABI: every entry point catches
 // and converts to a LEO2D_ERR_* code with a message in leo2d_last_error().
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -48,6 +49,9 @@ struct NanError : std::runtime_error {
     explicit NanError(const std::string& m) : std::runtime_error(m) {}
 };
 
+// the SELL-4-sigma step with LDS-staged row tiles (sell.hip)
+constexpr int kVariantSell = 20;
+
 // step kernel for a block of B vectors (LEO2D_KPM_VARIANT overrides the
 // default).  B >= 64: the wavefront-per-row kernel with readlane broadcasts (1)
 // for FP64; for the FP32 recursion, two rows per wavefront with 16-byte gathers
@@ -59,7 +63,14 @@ int kpm_variant(bool mixed, bool spatial, int B)
 {
     if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) return std::atoi(e);
     if (spatial && (B == 16 || B == 32)) return 14;
+    if (spatial && B >= 64) return kVariantSell;
     return mixed && spatial ? 15 : 1;
+}
+
+int env_int(const char* k, int d)
+{
+    const char* e = std::getenv(k);
+    return e ? std::atoi(e) : d;
 }
 
 // row schedule of the wavefront-per-row kernels: 1 = one contiguous slice of
@@ -123,6 +134,16 @@ struct leo2d_handle {
         int ucap = 0, ntiles = 0;
         DevBuf row0, u0, ucols, lcol;
     } tiles;
+    // SELL-4-sigma row tiles of the recursion matrix for the LDS-staged step
+    // (sell.hip): pattern built on the host on first use, values gathered on the device
+    struct Sell {
+        bool valid = false, failed = false, spatial = false;
+        int ucap = 0, tmax = 0, ntiles = 0, ngroups = 0, tile_rows = 0, max_len = 0;
+        int64_t entries = 0;
+        double build_ms = 0.0;
+        DevBuf row0, u0, ucols, g0, grow, glen, goff, eu, map, val, val32;
+        bool val_valid = false, val32_valid = false;
+    } sell;
     // row groups: kGroupRows consecutive rows of the recursion matrix with their
     // merged column list and a value per (column, row) (zero where absent)
     struct Groups {
@@ -310,6 +331,10 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->pcsr_valid = false;
     h->pval32_valid = false;
     h->tiles.valid = false;
+    h->sell.valid = false;
+    h->sell.failed = false;
+    h->sell.val_valid = false;
+    h->sell.val32_valid = false;
     h->groups.valid = false;
     h->tiles.failed = false;
     h->chunks.valid = false;
@@ -427,6 +452,159 @@ bool build_tiles(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int u
     T.valid = true;
     T.failed = false;
     return true;
+}
+
+// SELL-4-sigma row tiles for the LDS-staged step (sell.hip).  Greedy tiles of
+// consecutive rows of the recursion matrix (at most kSellMaxRows) while the
+// tile's distinct columns -- its own rows first, then the other columns in
+// increasing order -- fit `ucap` LDS rows.  Within a tile the rows are sorted by
+// length (longest first, ties by row) and cut into groups of four; a group of
+// longest row L stores round_up(L, 4) entries per slot, entry e of slot s at
+// goff + 4 e + s, padded with (union index 0, value 0).  `map` gives each
+// stored entry's position in the CSR (-1 for padding) so the values can be
+// gathered on the device for every matrix with this pattern.  Returns false
+// when a row has more than kSellMaxLen entries or does not fit `ucap` alone.
+bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, int tmax, bool spatial)
+{
+    auto& S = h->sell;
+    if (S.valid && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return true;
+    if (S.failed && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return false;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = h->n;
+    hipStream_t s = h->stream;
+    std::vector<int64_t> rp(n + 1);
+    std::vector<int32_t> col((size_t)h->nnz);
+    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "sell");
+    if (h->nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "sell");
+    check(hipStreamSynchronize(s), "sell");
+    S.ucap = ucap;
+    S.tmax = tmax;
+    S.spatial = spatial;
+    auto fail = [&]() {
+        S.failed = true;
+        S.valid = false;
+        return false;
+    };
+    std::vector<int32_t> row0, ucols, g0, grow, glen;
+    std::vector<int64_t> u0, goff, map;
+    std::vector<uint16_t> eu;
+    std::vector<int32_t> stamp(n, -1), loc(n, 0), added, halo, order;
+    int r = 0, tile = 0, tile_rows = 0, max_len = 0;
+    while (r < n) {
+        const int start = r;
+        int size = 0;   // distinct columns of the tile so far (own rows included)
+        while (r < n && r - start < tmax) {
+            if (rp[r + 1] - rp[r] > leo2d::kSellMaxLen) return fail();
+            added.clear();
+            auto take = [&](int c) {
+                if (stamp[c] != tile) {
+                    stamp[c] = tile;
+                    added.push_back(c);
+                }
+            };
+            take(r);
+            for (int64_t q = rp[r]; q < rp[r + 1]; ++q) take(col[q]);
+            if (size + (int)added.size() > ucap) {
+                for (int c : added) stamp[c] = -1;
+                break;
+            }
+            size += (int)added.size();
+            ++r;
+        }
+        if (r == start) return fail();
+        const int T = r - start;
+        tile_rows = std::max(tile_rows, T);
+        // union: own rows, then the other columns in increasing order
+        halo.clear();
+        for (int rr = start; rr < r; ++rr)
+            for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) {
+                const int c = col[q];
+                if ((c < start || c >= r) && stamp[c] == tile) {
+                    stamp[c] = -2 - tile;   // listed once
+                    halo.push_back(c);
+                }
+            }
+        std::sort(halo.begin(), halo.end());
+        row0.push_back(start);
+        u0.push_back((int64_t)ucols.size());
+        for (int rr = start; rr < r; ++rr) {
+            loc[rr] = rr - start;
+            ucols.push_back(rr);
+        }
+        for (size_t k = 0; k < halo.size(); ++k) {
+            loc[halo[k]] = T + (int)k;
+            ucols.push_back(halo[k]);
+        }
+        // SELL-4 groups, rows by decreasing length (sigma = the tile)
+        order.resize(T);
+        for (int k = 0; k < T; ++k) order[k] = start + k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+        g0.push_back((int32_t)glen.size());
+        for (int k = 0; k < T; k += 4) {
+            int L = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int row = k + j < T ? order[k + j] : -1;
+                grow.push_back(row);
+                if (row >= 0) L = std::max(L, (int)(rp[row + 1] - rp[row]));
+            }
+            L = (L + 3) / 4 * 4;
+            max_len = std::max(max_len, L);
+            glen.push_back(L);
+            goff.push_back((int64_t)eu.size());
+            for (int e = 0; e < L; ++e)
+                for (int j = 0; j < 4; ++j) {
+                    const int row = k + j < T ? order[k + j] : -1;
+                    if (row >= 0 && e < rp[row + 1] - rp[row]) {
+                        const int64_t q = rp[row] + e;
+                        eu.push_back((uint16_t)loc[col[q]]);
+                        map.push_back(q);
+                    } else {
+                        eu.push_back(0);
+                        map.push_back(-1);
+                    }
+                }
+        }
+        ++tile;
+    }
+    row0.push_back(n);
+    u0.push_back((int64_t)ucols.size());
+    g0.push_back((int32_t)glen.size());
+    to_device(S.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "sell tiles");
+    to_device(S.u0, u0.data(), sizeof(int64_t) * u0.size(), s, "sell tiles");
+    to_device(S.ucols, ucols.data(), sizeof(int32_t) * ucols.size(), s, "sell tiles");
+    to_device(S.g0, g0.data(), sizeof(int32_t) * g0.size(), s, "sell tiles");
+    to_device(S.grow, grow.data(), sizeof(int32_t) * grow.size(), s, "sell groups");
+    to_device(S.glen, glen.data(), sizeof(int32_t) * glen.size(), s, "sell groups");
+    to_device(S.goff, goff.data(), sizeof(int64_t) * goff.size(), s, "sell groups");
+    to_device(S.eu, eu.data(), sizeof(uint16_t) * eu.size(), s, "sell entries");
+    to_device(S.map, map.data(), sizeof(int64_t) * map.size(), s, "sell entries");
+    check(hipStreamSynchronize(s), "sell");
+    S.ntiles = tile;
+    S.tile_rows = tile_rows;
+    S.max_len = max_len;
+    S.ngroups = (int)glen.size();
+    S.entries = (int64_t)eu.size();
+    S.val_valid = S.val32_valid = false;
+    S.valid = true;
+    S.failed = false;
+    S.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+// the SELL values of the current matrix (FP64, or FP32 for the mixed recursion)
+const void* sell_values(leo2d_handle* h, const void* csr_val, int elem_bytes, bool fp32)
+{
+    auto& S = h->sell;
+    DevBuf& buf = fp32 ? S.val32 : S.val;
+    bool& ok = fp32 ? S.val32_valid : S.val_valid;
+    if (!ok) {
+        dalloc(buf, (size_t)elem_bytes * (size_t)S.entries, "sell values");
+        check(leo2d::launch_sell_fill((const int64_t*)S.map.p, csr_val, buf.p, elem_bytes, S.entries, h->stream),
+              "sell values");
+        ok = true;
+    }
+    return buf.p;
 }
 
 // Row groups of the recursion matrix: rows g*K .. g*K+K-1 share one sorted
@@ -1455,6 +1633,43 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     sa.chunk_threads = nt;
                 }
             }
+            leo2d::SellStepArgs ss{};
+            if (sa.variant == kVariantSell) {
+                // one workgroup per CU with two LDS buffers of kSellUcap x 256 B; tiles of at most
+                // tmax rows whose union fits ucap staged rows
+                const int ucap = std::max(32, std::min(leo2d::kSellUcap, env_int("LEO2D_SELL_UCAP", leo2d::kSellUcap)));
+                const int tmax = std::max(4, std::min(leo2d::kSellMaxRows, env_int("LEO2D_SELL_TMAX", leo2d::kSellMaxRows)));
+                if (!spatial || !leo2d::sell_supported(mat_type, vec_type, B) ||
+                    !build_sell(h, drp, dcol, ucap, tmax, spatial)) {
+                    sa.variant = B == 16 || B == 32 ? 14 : (mixed ? 15 : 1);
+                } else {
+                    auto& S = h->sell;
+                    const int hb = mat_type == leo2d::MAT_CPX_F64 ? 16
+                                   : (mat_type == leo2d::MAT_REAL_F32 ? 4 : 8);
+                    ss.ev = sell_values(h, dval, hb, mixed);
+                    int dev_id = 0, cus = 256;
+                    if (hipGetDevice(&dev_id) == hipSuccess)
+                        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+                    ss.grid = std::max(8, std::min(G, cus) / 8 * 8);
+                    sa.grid = ss.grid;   // the reduce reads this many partials
+                    ss.n = n;
+                    ss.R = B;
+                    ss.part = (double*)part.p;
+                    ss.v = keep_v ? V.p : nullptr;
+                    ss.ntiles = S.ntiles;
+                    ss.tile_row0 = (const int32_t*)S.row0.p;
+                    ss.tile_u0 = (const int64_t*)S.u0.p;
+                    ss.ucols = (const int32_t*)S.ucols.p;
+                    ss.tile_g0 = (const int32_t*)S.g0.p;
+                    ss.grow = (const int32_t*)S.grow.p;
+                    ss.glen = (const int32_t*)S.glen.p;
+                    ss.goff = (const int64_t*)S.goff.p;
+                    ss.eu = (const uint16_t*)S.eu.p;
+                    ss.tile_rows = S.tile_rows;
+                    ss.max_len = S.max_len;
+                    ss.sliced = env_int("LEO2D_SELL_SLICED", 1);
+                }
+            }
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
             if ((sa.variant == 15 || sa.variant == 16) && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32))
                 sa.variant = mixed && spatial && B % 64 == 0 ? 8 : 1;
@@ -1467,7 +1682,8 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     1024 % per_row != 0)
                     sa.variant = B >= 64 ? 1 : 0;
             }
-            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14 && sa.variant != 17)
+            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14 && sa.variant != 17 &&
+                sa.variant != kVariantSell)
                 sa.variant = 0;
             kernel_used = sa.variant;
             if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
@@ -1500,7 +1716,16 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 sa.alpha = j == 0 ? 1.0 : 2.0;
                 sa.beta = j == 0 ? 0 : 1;
                 if (p->time_steps) check(hipEventRecord(event_at(h, 2 + 2 * (size_t)j), s), "event");
-                check(leo2d::launch_kpm_step(mat_type, vec_type, sa, s), "kpm step");
+                if (sa.variant == kVariantSell) {
+                    ss.cur = sa.cur;
+                    ss.prev = sa.prev;
+                    ss.out = sa.out;
+                    ss.alpha = sa.alpha;
+                    ss.beta = sa.beta;
+                    check(leo2d::launch_kpm_step_sell(mat_type, vec_type, ss, s), "kpm step");
+                } else {
+                    check(leo2d::launch_kpm_step(mat_type, vec_type, sa, s), "kpm step");
+                }
                 if (p->time_steps) check(hipEventRecord(event_at(h, 3 + 2 * (size_t)j), s), "event");
                 ra.j = j;
                 ra.cur = cur;

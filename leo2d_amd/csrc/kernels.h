@@ -74,6 +74,44 @@ constexpr int kTileWaveLdsBytes = 144 * 1024;
 inline int tile_wave_ucap(int vec_type, int R) { return kTileWaveLdsBytes / (R * (int)vec_bytes(vec_type)); }
 hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStream_t s);
 
+// ---- SELL-4-sigma step with LDS-staged row tiles (sell.hip) -----------------
+// Tiles of at most kSellMaxRows consecutive rows whose distinct columns (the
+// union, own rows first) fit `ucap` LDS rows of 256 B; each tile's rows sorted
+// by length into groups of 4 (SELL-C-sigma with C = 4, sigma = the tile).
+constexpr int kSellMaxRows = 64;    // rows per tile (8 wavefronts x 2 groups x 4 rows)
+constexpr int kSellMaxLen = 64;     // longest row a group can hold
+constexpr int kSellUcap = 288;      // staged rows per LDS buffer (two buffers of 288 x 256 B)
+struct SellStepArgs {
+    int n;
+    int R;
+    const void* cur;
+    const void* prev;
+    void* out;
+    const void* v;
+    double alpha;
+    int beta;
+    double* part;               // [grid][R][3]
+    int grid;
+    int ntiles;
+    const int32_t* tile_row0;   // [ntiles + 1]
+    const int64_t* tile_u0;     // [ntiles + 1] offsets into ucols
+    const int32_t* ucols;       // union rows of every tile (own rows first)
+    const int32_t* tile_g0;     // [ntiles + 1] first group of each tile
+    const int32_t* grow;        // [groups][4] row of each group slot (-1: empty)
+    const int32_t* glen;        // [groups] steps of the group (multiple of 4, <= kSellMaxLen)
+    const int64_t* goff;        // [groups] first entry of the group; entry e of slot s at goff + 4 e + s
+    const uint16_t* eu;         // [entries] union index of the entry's column
+    const void* ev;             // [entries] value (matrix element type)
+    int tile_rows;              // most rows in a tile (<= kSellMaxRows)
+    int max_len;                // most steps in a group (<= kSellMaxLen)
+    int sliced;                 // 1: XCD x walks its own contiguous eighth of the tiles
+};
+bool sell_supported(int mat_type, int vec_type, int R);
+hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s);
+// dst[i] = src[map[i]] (zero where map[i] < 0), elements of elem_bytes (4, 8 or 16)
+hipError_t launch_sell_fill(const int64_t* map, const void* src, void* dst, int elem_bytes, int64_t count,
+                            hipStream_t s);
+
 // Deterministic reduction of the partials of one step into moments.
 //   scheme 0 (reference, target starts): mu_j = Re cur[target_r][r] for
 //            2 <= j < M/2, dots used for j >= M/4 (locality.cpp:6002-6034)

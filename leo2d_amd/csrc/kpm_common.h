@@ -104,18 +104,18 @@ __device__ inline double re(float a) { return a; }
 // bc16<K>(v): every lane receives v from lane K of its 16-lane row (DPP
 // row_newbcast).  A row of 16 lanes loads 16 CSR entries in one access and
 // passes them to each other this way.
+// Every lane of the row is enabled and row_newbcast always has a valid source,
+// so the plain DPP move (bound_ctrl set, no "old" value to materialise) is used.
 template <int K>
 __device__ inline int bc16(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 template <int K>
 __device__ inline double bc16(double v)
 {
-    const long long b = __double_as_longlong(v);
-    const int lo = bc16<K>((int)(b & 0xffffffffLL));
-    const int hi = bc16<K>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    // one 64-bit DPP move (gfx950 allows row_newbcast on 64-bit operands)
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 template <int K>
 __device__ inline float bc16(float v) { return __int_as_float(bc16<K>(__float_as_int(v))); }

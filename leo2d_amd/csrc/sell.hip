@@ -1,0 +1,477 @@
+// sell.hip -- Chebyshev step on SELL-4-sigma row groups with the neighbour rows
+// of each row tile staged in LDS.
+//
+// One launch = one step T_{j+1} = alpha H T_j - beta T_{j-1} for a block of R
+// vectors (row-major [n][R]) fused with the per-vector partial sums of the
+// moment reductions, like the CSR step kernels in kpm.hip.  It replaces the
+// same loop of the reference (locality.cpp:6002-6041 / 6089-6127 calling
+// SpMatrix::vectorMultiply, spmatrix.cpp:346-360 / 395-410).
+//
+// Why: a CSR step gathers one whole row of T_j (R x 16 B = 1 KiB at R = 64
+// complex FP64) per stored entry, about 32 times per output row on bilayer
+// graphene, all through the vector-memory path (L2 -> L1 -> lanes).  Here a
+// workgroup owns a tile of up to 64 spatially consecutive rows; for each chunk
+// of C vectors (C x element = 256 B) it copies the tile's distinct neighbour
+// row chunks (the tile's "union", about 4 per output row) once into LDS with
+// direct global->LDS loads, and every entry then reads its operand from LDS.
+//
+// Layout of the matrix (built on the host once per sparsity pattern):
+//  * tiles: consecutive rows [tile_row0[t], tile_row0[t+1]) of the spatially
+//    ordered matrix; the tile's union is ucols[tile_u0[t] ...], own rows
+//    first (union index u = row - tile_row0[t]) so T_j of an own row is in LDS;
+//  * SELL-4-sigma groups: the tile's rows sorted by length (sigma = the tile)
+//    and cut into groups of 4; group g holds the rows grow[4g .. 4g+3] (-1 for
+//    an empty slot) and glen[g] steps (the longest row, rounded up to 4);
+//    entry e of slot s is at goff[g] + 4 e + s (padding: value 0, index 0);
+//  * each entry is (eu = local union index (uint16), ev = value).
+//
+// Lanes: a wavefront runs one group at a time; 16-lane DPP row s is matrix row
+// grow[4g+s] and lane k of it is vector k (or 2k, 2k+1 for FP32 vectors) of the
+// chunk.  Lane (s, k) keeps entries k, k+16, k+32, k+48 of its row in
+// registers for all chunks of the tile; step e broadcasts entry e inside each
+// DPP row (row_newbcast), so the four rows advance together and every row's
+// products are added in CSR order.  One LDS read of 16 B per lane per step:
+// the four rows' operand chunks are four 256-byte runs.
+//
+// Reductions are deterministic (fixed tile order per workgroup, fixed lane
+// and wavefront folds, no floating-point atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+#include "kpm_common.h"
+
+namespace leo2d {
+namespace {
+
+constexpr int kSellThreads = 512;
+constexpr int kSellWaves = kSellThreads / 64;
+constexpr int kSellStagePerWave = kSellUcap / 4 / kSellWaves;   // staging loads per wavefront and chunk (9)
+static_assert(kSellUcap % (4 * kSellWaves) == 0, "LDS rows must split evenly over the wavefronts");
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// one 16-byte LDS read of the VPL vector elements of a lane
+template <typename XT, int VPL>
+__device__ inline void lds_ld(XT (&x)[VPL], const XT* p)
+{
+    if constexpr (VPL == 1) {
+        x[0] = *p;
+    } else {
+        const f4v q = *reinterpret_cast<const f4v*>(p);
+        x[0] = make_float2(q.x, q.y);
+        x[1] = make_float2(q.z, q.w);
+    }
+}
+
+// entries k, k+16, ..., k+16(QS-1) of the lane's row (union index, value)
+template <typename HT, int QS>
+struct GroupRegs {
+    int eu[QS];
+    HT ev[QS];
+};
+
+// operands of steps 4B .. 4B+3 (entry e is held by lane e % 16 of register e / 16)
+template <int B, typename HT, typename XT, int VPL, int QS>
+__device__ inline void ld_block(XT (&x)[4][VPL], const GroupRegs<HT, QS>& g, const XT* xs, int koff)
+{
+    {
+        const int o = bc16<(4 * B + 0) & 15>(g.eu[(4 * B + 0) >> 4]);
+        lds_ld<XT, VPL>(x[0], xs + o * (16 * VPL) + koff);
+    }
+    {
+        const int o = bc16<(4 * B + 1) & 15>(g.eu[(4 * B + 1) >> 4]);
+        lds_ld<XT, VPL>(x[1], xs + o * (16 * VPL) + koff);
+    }
+    {
+        const int o = bc16<(4 * B + 2) & 15>(g.eu[(4 * B + 2) >> 4]);
+        lds_ld<XT, VPL>(x[2], xs + o * (16 * VPL) + koff);
+    }
+    {
+        const int o = bc16<(4 * B + 3) & 15>(g.eu[(4 * B + 3) >> 4]);
+        lds_ld<XT, VPL>(x[3], xs + o * (16 * VPL) + koff);
+    }
+}
+
+template <int E, typename HT, typename XT, typename AT, int VPL, int QS>
+__device__ inline void mac_step(AT (&acc)[VPL], const XT (&x)[VPL], const GroupRegs<HT, QS>& g)
+{
+    const HT h = bc16<E & 15>(g.ev[E >> 4]);
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) mac(acc[v], h, x[v]);
+}
+
+// steps 4B .. L-1 of the group (L a multiple of 4, at most 16 QS); xa holds the
+// operands of block B, the next block's are read before this block's products
+// (one block past L is read and not used)
+template <int B, typename HT, typename XT, typename AT, int VPL, int QS>
+__device__ inline void run_blocks(AT (&acc)[VPL], const GroupRegs<HT, QS>& g, const XT* xs, int koff, int L,
+                                  const XT (&xa)[4][VPL])
+{
+    if constexpr (B < 4 * QS) {
+        if (4 * B < L) {
+            XT xb[4][VPL];
+            // unconditional (entries past L are index 0, a valid row): a branch here would
+            // merge two LDS counts and make the wait below cover the next block too
+            if constexpr (B + 1 < 4 * QS) ld_block<B + 1, HT, XT, VPL, QS>(xb, g, xs, koff);
+            // keep those reads ahead of this block's products (the scheduler would sink them)
+            __builtin_amdgcn_sched_barrier(0);
+            mac_step<4 * B + 0, HT, XT, AT, VPL, QS>(acc, xa[0], g);
+            mac_step<4 * B + 1, HT, XT, AT, VPL, QS>(acc, xa[1], g);
+            mac_step<4 * B + 2, HT, XT, AT, VPL, QS>(acc, xa[2], g);
+            mac_step<4 * B + 3, HT, XT, AT, VPL, QS>(acc, xa[3], g);
+            run_blocks<B + 1, HT, XT, AT, VPL, QS>(acc, g, xs, koff, L, xb);
+        }
+    }
+}
+
+__device__ inline void wait_vm0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Pipeline over (tile, chunk) pairs p of one workgroup, two LDS buffers:
+//   top of pair p: wait for everything issued during pair p-1, barrier;
+//     then issue T_{j-1} (and v) of this pair's rows, then the staging copies of
+//     pair p+1 into the other buffer (always SPW per wavefront, a valid dummy
+//     row when there is no pair p+1, so the wait counts stay static);
+//   compute every group of the pair from LDS, outputs, then the stores;
+//   then the loads pair p+1 needs that are not staged: the next tile's union
+//     columns (for the copies of pair p+2) and entries (when the tile changes).
+// Nothing issued in a pair is consumed in the same pair except T_{j-1} and v,
+// which are issued before the copies, so their wait leaves the copies in flight.
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, bool DIRECT>
+__global__ __launch_bounds__(kSellThreads, 1) void kpm_step_sell_kernel(SellStepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int C = 16 * VPL;
+    constexpr int R = C * NCH;
+    constexpr int SPW = kSellStagePerWave;
+    __shared__ __align__(16) XT xs0[kSellUcap * C];
+    __shared__ __align__(16) XT xs1[kSellUcap * C];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: group loops branch on scalars
+    const int s = lane >> 4;
+    const int k = lane & 15;
+    const int koff = k * VPL;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    int t_first, t_end, t_stride;
+    if (a.sliced) {
+        const int x = b & 7;
+        t_first = (int)((int64_t)a.ntiles * x / 8) + (b >> 3);
+        t_end = (int)((int64_t)a.ntiles * (x + 1) / 8);
+        t_stride = G >> 3;
+    } else {
+        t_first = lb;
+        t_end = a.ntiles;
+        t_stride = G;
+    }
+
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+    const XT* prev = static_cast<const XT*>(a.prev);
+    XT* out = static_cast<XT*>(a.out);
+    const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+    const HT* __restrict__ ev = static_cast<const HT*>(a.ev);
+    const uint16_t* __restrict__ eu = a.eu;
+    const double alpha = a.alpha;
+    const bool beta = a.beta != 0;
+
+    double d1[NCH][VPL], d2[NCH][VPL], d3[NCH][VPL];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            d1[c][v] = 0.0;
+            d2[c][v] = 0.0;
+            d3[c][v] = 0.0;
+        }
+
+    // entries, rows and lengths of this wavefront's groups of tile t
+    auto load_groups = [&](int t, GroupRegs<HT, QS> (&gr)[GPW], int (&grow)[GPW], int (&glen)[GPW]) {
+        const int g0 = a.tile_g0[t], g1 = a.tile_g0[t + 1];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int g = g0 + wave + kSellWaves * j;
+            const bool live = g < g1;
+            const int L = live ? a.glen[g] : 0;
+            const int64_t go = live ? a.goff[g] : 0;
+            glen[j] = L;
+            grow[j] = live ? a.grow[4 * g + s] : -1;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const int e = 16 * q + k;
+                const bool ok = e < L;
+                gr[j].eu[q] = ok ? (int)eu[go + 4 * e + s] : 0;
+                if (ok) gr[j].ev[q] = ev[go + 4 * e + s];
+                else zero(gr[j].ev[q]);
+            }
+        }
+    };
+    // union rows this lane copies for tile t (a valid dummy row past the union)
+    auto load_ucols = [&](int t, int (&uc)[SPW]) {
+        const int64_t u0 = a.tile_u0[t];
+        const int U = (int)(a.tile_u0[t + 1] - u0);
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) {
+            const int u = 4 * (wave + kSellWaves * m) + s;
+            uc[m] = a.ucols[u0 + (u < U ? u : 0)];
+        }
+    };
+    auto stage = [&](XT* buf, int c, const int (&uc)[SPW]) {
+#pragma unroll
+        for (int m = 0; m < SPW; ++m) {
+            const int i = wave + kSellWaves * m;
+            const XT* src = cur + (size_t)uc[m] * R + c * C + koff;
+            __builtin_amdgcn_global_load_lds(
+                (const void*)src, (__attribute__((address_space(3))) void*)(buf + (size_t)4 * i * C), 16, 0, 0);
+        }
+    };
+
+    GroupRegs<HT, QS> ec[GPW], en[GPW];
+    int gc_row[GPW], gn_row[GPW], gc_len[GPW], gn_len[GPW];
+    int ucs[SPW];
+    int t = t_first, c = 0, p = 0;
+    int r0 = 0, r0n = 0;
+    if (t < t_end) {
+        load_groups(t, ec, gc_row, gc_len);
+        r0 = a.tile_row0[t];
+        load_ucols(t, ucs);
+        stage(xs0, 0, ucs);
+        if (NCH == 1) load_ucols(t + t_stride < t_end ? t + t_stride : t, ucs);
+    }
+
+    auto pair = [&](XT* buf, XT* nbuf) {
+        wait_vm0();
+        __syncthreads();
+        if (c == 0 && p > 0) {
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                ec[j] = en[j];
+                gc_row[j] = gn_row[j];
+                gc_len[j] = gn_len[j];
+            }
+            r0 = r0n;
+        }
+        // T_{j-1} and v of this pair's rows
+        XT pv[GPW][VPL], vv[GPW][VPL];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int row = gc_row[j];
+            const size_t idx = (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                if (beta && row >= 0) pv[j][v] = prev[idx + v];
+                else zero(pv[j][v]);
+                if (DIRECT) {
+                    if (row >= 0) vv[j][v] = vin[idx + v];
+                    else zero(vv[j][v]);
+                }
+            }
+        }
+        // copies of pair p+1 into the other buffer
+        int t1 = t, c1 = c + 1;
+        if (c1 == NCH) {
+            c1 = 0;
+            t1 = t + t_stride;
+        }
+        const bool has1 = t1 < t_end;
+        stage(nbuf, has1 ? c1 : 0, ucs);
+        // products of every group of the pair
+        AT acc[GPW][VPL];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) zero(acc[j][v]);
+            int L = __builtin_amdgcn_readfirstlane(gc_len[j]);
+            // opaque per pair: keeps the block conditions from being hoisted out of the loop
+            asm volatile("" : "+s"(L));
+            if (L > 0) {
+                XT xa[4][VPL];
+                ld_block<0, HT, XT, VPL, QS>(xa, ec[j], buf, koff);
+                run_blocks<0, HT, XT, AT, VPL, QS>(acc[j], ec[j], buf, koff, L, xa);
+            }
+        }
+        // outputs and moment terms, then the stores
+        XT o[GPW][VPL];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int row = gc_row[j];
+            XT cv[VPL];
+            lds_ld<XT, VPL>(cv, buf + (row < 0 ? 0 : row - r0) * C + koff);
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                o[j][v] = comb(alpha, acc[j][v], pv[j][v]);
+                const double t1v = row >= 0 ? nrm2(cv[v]) : 0.0;
+                const double t2v = row >= 0 ? redot(o[j][v], cv[v]) : 0.0;
+                const double t3v = DIRECT && row >= 0 ? redot(vv[j][v], cv[v]) : 0.0;
+                // the chunk index is uniform: select this chunk's sums without indexing registers
+#pragma unroll
+                for (int cc = 0; cc < NCH; ++cc) {
+                    if (cc == c) {
+                        d1[cc][v] += t1v;
+                        d2[cc][v] += t2v;
+                        if (DIRECT) d3[cc][v] += t3v;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int row = gc_row[j];
+            if (row >= 0) {
+                const size_t idx = (size_t)row * R + c * C + koff;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) out[idx + v] = o[j][v];
+            }
+        }
+        // loads for the next pairs: union columns of pair p+2's tile, the next tile's groups
+        int t2 = t1, c2 = c1 + 1;
+        if (c2 == NCH) {
+            c2 = 0;
+            t2 = t1 + t_stride;
+        }
+        if (c2 == 0) load_ucols(t2 < t_end ? t2 : t, ucs);
+        if (c1 == 0 && has1) {
+            load_groups(t1, en, gn_row, gn_len);
+            r0n = a.tile_row0[t1];
+        }
+        t = t1;
+        c = c1;
+        ++p;
+    };
+    while (t < t_end) {
+        pair(xs0, xs1);
+        if (t >= t_end) break;
+        pair(xs1, xs0);
+    }
+    wait_vm0();
+    __syncthreads();
+
+    // fold the four DPP rows (same vectors), then the wavefronts
+#pragma unroll
+    for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            d1[cc][v] += __shfl_xor(d1[cc][v], 16);
+            d2[cc][v] += __shfl_xor(d2[cc][v], 16);
+            d1[cc][v] += __shfl_xor(d1[cc][v], 32);
+            d2[cc][v] += __shfl_xor(d2[cc][v], 32);
+            if (DIRECT) {
+                d3[cc][v] += __shfl_xor(d3[cc][v], 16);
+                d3[cc][v] += __shfl_xor(d3[cc][v], 32);
+            }
+        }
+    double* red = reinterpret_cast<double*>(xs0);   // [waves][R][3]
+    if (s == 0) {
+#pragma unroll
+        for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                const int r = cc * C + koff + v;
+                red[(wave * R + r) * 3 + 0] = d1[cc][v];
+                red[(wave * R + r) * 3 + 1] = d2[cc][v];
+                red[(wave * R + r) * 3 + 2] = DIRECT ? d3[cc][v] : 0.0;
+            }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < R * 3; i += kSellThreads) {
+        double sum = red[i];
+#pragma unroll
+        for (int w = 1; w < kSellWaves; ++w) sum += red[w * R * 3 + i];
+        a.part[(size_t)lb * R * 3 + i] = sum;
+    }
+}
+
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS>
+hipError_t launch_sell(const SellStepArgs& a, hipStream_t s)
+{
+    if (a.v) hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, true>), dim3(a.grid),
+                                dim3(kSellThreads), 0, s, a);
+    else hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false>), dim3(a.grid),
+                            dim3(kSellThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename HT, typename XT, int VPL, int GPW, int QS>
+hipError_t launch_sell_r(const SellStepArgs& a, hipStream_t s)
+{
+    switch (a.R / (16 * VPL)) {
+    case 1: return launch_sell<HT, XT, VPL, 1, GPW, QS>(a, s);
+    case 2: return launch_sell<HT, XT, VPL, 2, GPW, QS>(a, s);
+    case 4:
+        if constexpr (VPL == 1) return launch_sell<HT, XT, VPL, 4, GPW, QS>(a, s);
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// two groups of four rows per wavefront (tiles of at most 64 rows): with the
+// next tile's entries double-buffered in registers, four groups per wavefront
+// exceed the 256 registers a lane has at two wavefronts per SIMD
+template <typename HT, typename XT, int VPL>
+hipError_t launch_sell_shape(const SellStepArgs& a, hipStream_t s)
+{
+    if (a.max_len > 48) return launch_sell_r<HT, XT, VPL, 2, 4>(a, s);
+    return launch_sell_r<HT, XT, VPL, 2, 3>(a, s);
+}
+
+// dst[i] = src[map[i]] (0 where map[i] < 0): the SELL values from the CSR values
+template <typename E>
+__global__ void sell_fill_kernel(const int64_t* __restrict__ map, const E* __restrict__ src, E* __restrict__ dst,
+                                 int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = map[i];
+        E e;
+        if (m >= 0) e = src[m];
+        else zero(e);
+        dst[i] = e;
+    }
+}
+
+}  // namespace
+
+bool sell_supported(int mat_type, int vec_type, int R)
+{
+    const bool cpx64 = vec_type == VEC_CPX_F64 && (mat_type == MAT_CPX_F64 || mat_type == MAT_REAL_F64);
+    const bool cpx32 = vec_type == VEC_CPX_F32 && (mat_type == MAT_CPX_F32 || mat_type == MAT_REAL_F32);
+    if (cpx64) return R == 16 || R == 32 || R == 64;
+    if (cpx32) return R == 32 || R == 64;
+    return false;
+}
+
+hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s)
+{
+    if (a.grid % 8 != 0 || a.grid <= 0 || !sell_supported(mat_type, vec_type, a.R) || a.tile_rows > kSellMaxRows ||
+        a.max_len > kSellMaxLen)
+        return hipErrorInvalidValue;
+    if (mat_type == MAT_CPX_F64) return launch_sell_shape<double2, double2, 1>(a, s);
+    if (mat_type == MAT_REAL_F64) return launch_sell_shape<double, double2, 1>(a, s);
+    if (mat_type == MAT_CPX_F32) return launch_sell_shape<float2, float2, 2>(a, s);
+    if (mat_type == MAT_REAL_F32) return launch_sell_shape<float, float2, 2>(a, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sell_fill(const int64_t* map, const void* src, void* dst, int elem_bytes, int64_t count, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const int blocks = (int)std::min<int64_t>(4096, (count + 255) / 256);
+    switch (elem_bytes) {
+    case 4:
+        hipLaunchKernelGGL(sell_fill_kernel<float>, dim3(blocks), dim3(256), 0, s, map, (const float*)src, (float*)dst,
+                           count);
+        break;
+    case 8:
+        hipLaunchKernelGGL(sell_fill_kernel<double>, dim3(blocks), dim3(256), 0, s, map, (const double*)src,
+                           (double*)dst, count);
+        break;
+    case 16:
+        hipLaunchKernelGGL(sell_fill_kernel<double2>, dim3(blocks), dim3(256), 0, s, map, (const double2*)src,
+                           (double2*)dst, count);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace leo2d

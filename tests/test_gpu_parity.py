@@ -321,7 +321,7 @@ def test_spatial_order_matches_natural_order(dev, name):
     assert np.max(np.abs(a - b)) <= 1e-12 * geom.n
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15", "16", "17"])
+@pytest.mark.parametrize("variant", ["0", "1", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13", "14", "15", "16", "17", "20"])
 @pytest.mark.parametrize("R", [4, 16, 64, 128])
 def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     """Every step-kernel variant (lane-per-vector, wavefront-per-row with 8/16
@@ -348,7 +348,7 @@ def test_step_kernel_variants(dev, oracle_mod, monkeypatch, variant, R):
     assert np.max(np.abs(mur - omur)) <= 2e-5 * geom.n, (variant, R)
 
 
-@pytest.mark.parametrize("variant", ["1", "8", "14", "15", "17"])
+@pytest.mark.parametrize("variant", ["1", "8", "14", "15", "17", "20"])
 @pytest.mark.parametrize("R", [32, 64])
 def test_variants_ragged_row_count(dev, oracle_mod, monkeypatch, variant, R):
     """A flake whose orbital count (1022) is not a multiple of the rows a
@@ -520,3 +520,36 @@ def test_xcd_slice_schedule(dev, oracle_mod, monkeypatch, variant, R):
     for grid in (0, 8, 24):
         mu = dev.kpm(M, random_phase=(R, 9), scheme=L.SCHEME_DOUBLING, grid=grid)
         assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n, (variant, R, grid)
+
+
+@pytest.mark.parametrize("cfg", [("288", "1", "64"), ("64", "0", "64"), ("128", "1", "13"), ("288", "0", "5")])
+@pytest.mark.parametrize("R", [16, 32, 64, 128])
+def test_sell_variant_shapes(dev, oracle_mod, monkeypatch, cfg, R):
+    """SELL-4-sigma step with LDS-staged tiles (variant 20) under LDS budgets
+    of 64 to 288 staged rows (tiles cut short by the budget), tiles of at most
+    5 to 64 rows (partial groups, idle wavefronts) and both tile schedules (XCD
+    slices, interleaved): FP64 moments within 1e-12 mu_0 of the oracle in both
+    schemes, FP32 recursion within 2e-5 mu_0, complex H(k) and a real H.  R =
+    128 falls back to the CSR step."""
+    ucap, sliced, tmax = cfg
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", "20")
+    monkeypatch.setenv("LEO2D_SELL_UCAP", ucap)
+    monkeypatch.setenv("LEO2D_SELL_SLICED", sliced)
+    monkeypatch.setenv("LEO2D_SELL_TMAX", tmax)
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    for kx in (0.0, 0.011):
+        job.complex_matrix = 1 if kx else 0
+        job.k[0], job.k[1] = kx, -0.004 if kx else 0.0
+        dev.assemble(job, L.PATTERN_PRUNED)
+        rp, ci, v = dev.csr_copy()
+        M = 40
+        vecs = oracle_mod.random_phase(geom.n, R, 9)
+        omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+        omur = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, nthreads=8)
+        for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+            mu = dev.kpm(M, random_phase=(R, 9), scheme=L.SCHEME_DOUBLING, precision=prec)
+            assert np.max(np.abs(mu - omu)) <= tol * geom.n, (cfg, R, kx, prec)
+            mu = dev.kpm(M, random_phase=(R, 9), scheme=L.SCHEME_REFERENCE, precision=prec)
+            assert np.max(np.abs(mu - omur)) <= tol * geom.n, (cfg, R, kx, prec)
