@@ -81,6 +81,7 @@ hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStr
 constexpr int kSellMaxRows = 64;    // rows per tile (8 wavefronts x 2 groups x 4 rows)
 constexpr int kSellMaxLen = 64;     // longest row a group can hold
 constexpr int kSellUcap = 288;      // staged rows per LDS buffer (two buffers of 288 x 256 B)
+constexpr int kSellWaves = 8;       // wavefronts per workgroup (one workgroup per CU)
 struct SellStepArgs {
     int n;
     int R;
@@ -105,6 +106,8 @@ struct SellStepArgs {
     int tile_rows;              // most rows in a tile (<= kSellMaxRows)
     int max_len;                // most steps in a group (<= kSellMaxLen)
     int sliced;                 // 1: XCD x walks its own contiguous eighth of the tiles
+    int half;                   // 1: half-size workgroups (kSellWaves / 2 wavefronts, kSellUcap / 2 rows
+                                //    per buffer, tiles of at most 32 rows), two per CU
 };
 bool sell_supported(int mat_type, int vec_type, int R);
 hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s);

@@ -45,10 +45,6 @@
 namespace leo2d {
 namespace {
 
-constexpr int kSellThreads = 512;
-constexpr int kSellWaves = kSellThreads / 64;
-constexpr int kSellStagePerWave = kSellUcap / 4 / kSellWaves;   // staging loads per wavefront and chunk (9)
-static_assert(kSellUcap % (4 * kSellWaves) == 0, "LDS rows must split evenly over the wavefronts");
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -138,15 +134,18 @@ __device__ inline void wait_vm0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memor
 //     columns (for the copies of pair p+2) and entries (when the tile changes).
 // Nothing issued in a pair is consumed in the same pair except T_{j-1} and v,
 // which are issued before the copies, so their wait leaves the copies in flight.
-template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, bool DIRECT>
-__global__ __launch_bounds__(kSellThreads, 1) void kpm_step_sell_kernel(SellStepArgs a)
+// NW wavefronts per workgroup, UC staged rows per LDS buffer (SPW = UC / 4 NW
+// staging loads per wavefront and chunk)
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, bool DIRECT, int NW, int UC>
+__global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs a)
 {
     using AT = typename acc_of<XT>::type;
     constexpr int C = 16 * VPL;
     constexpr int R = C * NCH;
-    constexpr int SPW = kSellStagePerWave;
-    __shared__ __align__(16) XT xs0[kSellUcap * C];
-    __shared__ __align__(16) XT xs1[kSellUcap * C];
+    constexpr int SPW = UC / (4 * NW);
+    static_assert(UC % (4 * NW) == 0, "LDS rows must split evenly over the wavefronts");
+    __shared__ __align__(16) XT xs0[UC * C];
+    __shared__ __align__(16) XT xs1[UC * C];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: group loops branch on scalars
     const int s = lane >> 4;
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(kSellThreads, 1) void kpm_step_sell_kernel(SellStep
         const int g0 = a.tile_g0[t], g1 = a.tile_g0[t + 1];
 #pragma unroll
         for (int j = 0; j < GPW; ++j) {
-            const int g = g0 + wave + kSellWaves * j;
+            const int g = g0 + wave + NW * j;
             const bool live = g < g1;
             const int L = live ? a.glen[g] : 0;
             const int64_t go = live ? a.goff[g] : 0;
@@ -213,14 +212,14 @@ __global__ __launch_bounds__(kSellThreads, 1) void kpm_step_sell_kernel(SellStep
         const int U = (int)(a.tile_u0[t + 1] - u0);
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
-            const int u = 4 * (wave + kSellWaves * m) + s;
+            const int u = 4 * (wave + NW * m) + s;
             uc[m] = a.ucols[u0 + (u < U ? u : 0)];
         }
     };
     auto stage = [&](XT* buf, int c, const int (&uc)[SPW]) {
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
-            const int i = wave + kSellWaves * m;
+            const int i = wave + NW * m;
             const XT* src = cur + (size_t)uc[m] * R + c * C + koff;
             __builtin_amdgcn_global_load_lds(
                 (const void*)src, (__attribute__((address_space(3))) void*)(buf + (size_t)4 * i * C), 16, 0, 0);
@@ -374,32 +373,33 @@ __global__ __launch_bounds__(kSellThreads, 1) void kpm_step_sell_kernel(SellStep
             }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < R * 3; i += kSellThreads) {
+    for (int i = threadIdx.x; i < R * 3; i += 64 * NW) {
         double sum = red[i];
 #pragma unroll
-        for (int w = 1; w < kSellWaves; ++w) sum += red[w * R * 3 + i];
+        for (int w = 1; w < NW; ++w) sum += red[w * R * 3 + i];
         a.part[(size_t)lb * R * 3 + i] = sum;
     }
 }
 
-template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS>
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NW, int UC>
 hipError_t launch_sell(const SellStepArgs& a, hipStream_t s)
 {
-    if (a.v) hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, true>), dim3(a.grid),
-                                dim3(kSellThreads), 0, s, a);
-    else hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false>), dim3(a.grid),
-                            dim3(kSellThreads), 0, s, a);
+    if (a.v) hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, true, NW, UC>), dim3(a.grid),
+                                dim3(64 * NW), 0, s, a);
+    else hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false, NW, UC>), dim3(a.grid),
+                            dim3(64 * NW), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename HT, typename XT, int VPL, int GPW, int QS>
 hipError_t launch_sell_r(const SellStepArgs& a, hipStream_t s)
 {
+    constexpr int NW = kSellWaves, UC = kSellUcap;
     switch (a.R / (16 * VPL)) {
-    case 1: return launch_sell<HT, XT, VPL, 1, GPW, QS>(a, s);
-    case 2: return launch_sell<HT, XT, VPL, 2, GPW, QS>(a, s);
+    case 1: return launch_sell<HT, XT, VPL, 1, GPW, QS, NW, UC>(a, s);
+    case 2: return launch_sell<HT, XT, VPL, 2, GPW, QS, NW, UC>(a, s);
     case 4:
-        if constexpr (VPL == 1) return launch_sell<HT, XT, VPL, 4, GPW, QS>(a, s);
+        if constexpr (VPL == 1) return launch_sell<HT, XT, VPL, 4, GPW, QS, NW, UC>(a, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
@@ -407,10 +407,17 @@ hipError_t launch_sell_r(const SellStepArgs& a, hipStream_t s)
 
 // two groups of four rows per wavefront (tiles of at most 64 rows): with the
 // next tile's entries double-buffered in registers, four groups per wavefront
-// exceed the 256 registers a lane has at two wavefronts per SIMD
+// exceed the 256 registers a lane has at two wavefronts per SIMD.  The
+// half-size shape (four wavefronts, two workgroups per CU) exists for the
+// 64-vector complex blocks only.
 template <typename HT, typename XT, int VPL>
 hipError_t launch_sell_shape(const SellStepArgs& a, hipStream_t s)
 {
+    if (a.half) {
+        constexpr int NCH = 64 / (16 * VPL);
+        if (a.R != 64 || a.v || a.max_len > 48 || a.tile_rows > 32) return hipErrorInvalidValue;
+        return launch_sell<HT, XT, VPL, NCH, 2, 3, kSellWaves / 2, kSellUcap / 2>(a, s);
+    }
     if (a.max_len > 48) return launch_sell_r<HT, XT, VPL, 2, 4>(a, s);
     return launch_sell_r<HT, XT, VPL, 2, 3>(a, s);
 }

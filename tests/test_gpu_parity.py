@@ -5,7 +5,8 @@ Bars (stated per test):
   * Hamiltonian values: |dv| <= 1e-12 |v| + 1e-14 max|H| (the GPU's atan2/cos/exp
     differ from glibc by a few ulp; everything else is IEEE-exact);
   * FP64 moments: |dmu| <= 1e-12 mu_0 (different summation order only);
-  * mixed precision (FP32 recursion): |dmu| <= 2e-4 mu_0, max |dDoS| reported;
+  * mixed precision (FP32 recursion): |dmu| <= 2e-5 mu_0, |dDoS| <= 1e-6 states/eV/atom inside
+    the spectrum (the full c1 cell meets it at every energy, tests/test_gpu_c1.py);
   * DoS: |drho| <= 1e-11 (1 + |rho|).
 """
 import io
@@ -169,12 +170,20 @@ def test_mixed_precision_against_fp64(dev, oracle_mod):
     mu64 = dev.kpm(M, random_phase=(16, 0), scheme=L.SCHEME_DOUBLING)
     mu32 = dev.kpm(M, random_phase=(16, 0), scheme=L.SCHEME_DOUBLING, precision=L.PREC_MIXED)
     n = geom.n
-    assert np.max(np.abs(mu32 - mu64)) <= 2e-4 * n
+    print("mixed max|dmu|/mu0:", float(np.max(np.abs(mu32 - mu64))) / n, flush=True)
+    assert np.max(np.abs(mu32 - mu64)) <= 2e-5 * n
     a, b = job.energy_rescale, job.energy_shift
-    _, r64 = dev.dos(mu64.mean(axis=0) / n, a, b)
+    E, r64 = dev.dos(mu64.mean(axis=0) / n, a, b)
     _, r32 = dev.dos(mu32.mean(axis=0) / n, a, b)
-    print("mixed max|dDoS| states/eV/atom:", float(np.max(np.abs(r64 - r32))))
-    assert np.max(np.abs(r64 - r32)) < 1e-4
+    dd = np.abs(r64[0] - r32[0])
+    inner = np.abs((E - b) / a) <= 0.99
+    print("mixed max|dDoS| states/eV/atom: %.3e (all bins), %.3e (|x| <= 0.99)" % (dd.max(), dd[inner].max()),
+          flush=True)
+    # inside the spectrum the bar holds; the two outermost bins (DoS zero there,
+    # reconstruction divided by sqrt(1 - x^2) ~ pi/2M) carry the FP32 storage
+    # rounding of 8,128 orbitals amplified ~2M/pi times (measured 5.9e-6)
+    assert dd[inner].max() <= 1e-6
+    assert dd.max() <= 3e-5
 
 
 def test_dos_parity(dev, oracle_mod):
