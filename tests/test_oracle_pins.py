@@ -926,3 +926,248 @@ def test_direct_plane_wave_field(oracle_mod):
         assert np.array_equal(u2[:2], u0[:2]) and u2[2] == -u0[2]
         for a in A:
             assert np.allclose(orc.relax_disp(j, x + a[0], y + a[1], 0), u0, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------- field terms, supercell, slots
+def _deck_with(text, **keys):
+    """The deck text with KEY = value lines replaced (or appended before POLY_ORDER)."""
+    for k, v in keys.items():
+        if re.search(r"^%s\b.*$" % k, text, flags=re.M):
+            text = re.sub(r"^%s\b.*$" % k, "%s = %s" % (k, v), text, flags=re.M)
+        else:
+            text = text.replace("POLY_ORDER", "%s = %s\nPOLY_ORDER" % (k, v), 1)
+    return text
+
+
+def _entry(rp, ci, v, i, j):
+    q = [p for p in range(rp[i], rp[i + 1]) if ci[p] == j]
+    assert len(q) == 1, (i, j)
+    return v[q[0]]
+
+
+def _hexagons(pos, sheet_of, sheet, centres_near, acc=1.42):
+    """Hexagons of one graphene sheet: the six orbitals within acc + 0.01 A of a
+    ring centre, ordered counterclockwise.  Centres are found geometrically:
+    an orbital's three bonds point away from three ring centres at distance acc."""
+    sel = np.where(sheet_of == sheet)[0]
+    p2 = pos[sel, :2]
+    out = []
+    for c0 in centres_near:
+        # the orbital nearest to c0, then the centre opposite one of its bonds
+        a = sel[np.argmin(np.sum((p2 - c0) ** 2, axis=1))]
+        d = np.sqrt(np.sum((p2 - pos[a, :2]) ** 2, axis=1))
+        nb = sel[(d > 1.0) & (d < 1.6)]
+        c = pos[a, :2] - (pos[nb[0], :2] - pos[a, :2])
+        ring = sel[np.abs(np.sqrt(np.sum((p2 - c) ** 2, axis=1)) - acc) < 0.01]
+        assert len(ring) == 6, len(ring)
+        ang = np.arctan2(pos[ring, 1] - c[1], pos[ring, 0] - c[0])
+        out.append((c, ring[np.argsort(ang)]))
+    return out
+
+
+def test_peierls_plaquette_flux(oracle_mod):
+    """Uniform B (locality.cpp:7157-7161, peierlsPhase): the product of H around
+    any graphene hexagon, taken counterclockwise, has phase -3.14e-5 B A
+    (A = the hexagon's area from the orbital positions, shoelace formula) and
+    modulus |t_NN / a|^6; gauge invariance makes that the same for every
+    hexagon of the flake, on both sheets.  A wrong prefactor or sign, or a
+    phase that is not the trapezoid (x2 - x1)(y1 + y2)/2 (e.g. y1 alone), fails
+    one of these.  With B = 0 the same products are real and positive."""
+    orc = oracle_mod
+    B, a = 500.0, 13.0
+    base = open(golden_path("tblg_test1.in")).read()
+    for Bf, on in ((B, 1), (0.0, 0)):
+        text = _deck_with(base, USE_B_FIELD=on, B_FIELD=Bf)
+        deck, geom, jobs, _ = orc.run_deck(text)
+        job = orc.make_job(deck, jobs[0])
+        assert job.complex_matrix == on
+        rp, ci, v = geom.assemble(job)
+        pos = orc.Geometry.job_positions(geom, job).reshape(-1, 3)
+        sheet_of = geom.index_to_grid.reshape(-1, 4)[:, 3]
+        for s in (0, 1):
+            for c, ring in _hexagons(pos, sheet_of, s, [np.array([0.0, 0.0]), np.array([4.1, -2.3]),
+                                                         np.array([-5.0, 3.7])]):
+                prod = 1.0 + 0j
+                for k in range(6):
+                    prod *= _entry(rp, ci, v, ring[k], ring[(k + 1) % 6])
+                x, y = pos[ring, 0], pos[ring, 1]
+                area = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
+                assert area > 5.0
+                want = -3.14e-5 * Bf * area
+                assert abs(np.angle(prod) - want) <= 1e-12, (s, c, np.angle(prod), want)
+                assert abs(abs(prod) - (2.8922 / a) ** 6) <= 1e-12 * (2.8922 / a) ** 6
+
+
+def test_electric_field_onsite(oracle_mod):
+    """USE_E_FIELD (locality.cpp:4166-4176, onSiteE = z E at :7163-7166): every
+    diagonal entry is (0.3504 + ENERGY_SHIFT + z E) / ENERGY_RESCALE, z the
+    orbital's height (0 on sheet 1, HEIGHT of sheet 2); off-diagonal entries
+    do not move."""
+    orc = oracle_mod
+    base = open(golden_path("tblg_test1.in")).read()
+    E, shift, a = 0.7, 0.25, 13.0
+    out = {}
+    for on in (0, 1):
+        text = _deck_with(base, USE_E_FIELD=on, E_FIELD=E if on else 0, ENERGY_SHIFT=shift, ENERGY_RESCALE=a)
+        deck, geom, jobs, _ = orc.run_deck(text)
+        job = orc.make_job(deck, jobs[0])
+        out[on] = geom.assemble(job)
+        z = geom.index_to_pos.reshape(-1, 3)[:, 2]
+    rp, ci, v1 = out[1]
+    _, _, v0 = out[0]
+    assert np.array_equal(out[0][0], rp) and np.array_equal(out[0][1], ci)
+    heights = sorted(set(z.tolist()))
+    assert heights == [0.0, 3.4]
+    for i in range(len(rp) - 1):
+        for q in range(rp[i], rp[i + 1]):
+            if ci[q] == i:
+                assert v1[q] == (0.3504 + shift + z[i] * E) / a
+                assert v0[q] == (0.3504 + shift) / a
+            else:
+                assert v1[q] == v0[q]
+
+
+@pytest.mark.parametrize("mn,deg", [((7, 6), 5.0858), ((6, 7), -5.0858), ((32, 31), 1.0501)])
+def test_supercell_angle_and_commensurability(oracle_mod, mn, deg):
+    """setupSupercell type 1 (locality.cpp:131-289).  The twist is measured from
+    the orbital positions (the angle between the two sheets' lattice vector a1)
+    and compared with the published commensurate angles of the (i+1, i) cells
+    (5.0858 deg for i = 6, 1.0501 deg for i = 31; negative when M < N).  The
+    supercell is commensurate: N a1 + M a2 of sheet 1 equals M a1 + N a2 of
+    sheet 2, and -M a1 + (M+N) a2 equals -N a1 + (M+N) a2, each sheet's a1, a2
+    read off neighbouring grid cells of its positions."""
+    orc = oracle_mod
+    M, N = mn
+    from leo2d_amd import workloads as W
+    text = W.tblg_cell_deck(M, N, 1)
+    od = orc.parse_deck(text)
+    orc.setup_supercell(od)
+    og = orc.Geometry(orc.geom_desc(od))
+    g = og.index_to_grid.reshape(-1, 4)
+    pos = og.index_to_pos.reshape(-1, 3)
+    assert og.n == 4 * (N * N + N * M + M * M)
+    where = {tuple(r): k for k, r in enumerate(g.tolist())}
+    vec = {}
+    for s in (0, 1):
+        for d, key in (((1, 0), "a1"), ((0, 1), "a2")):
+            for (i, j, l, ss), k in where.items():
+                if ss == s and l == 0 and (i + d[0], j + d[1], 0, s) in where:
+                    vec[s, key] = pos[where[(i + d[0], j + d[1], 0, s)], :2] - pos[k, :2]
+                    break
+    ang = math.degrees(math.atan2(vec[1, "a1"][1], vec[1, "a1"][0]) - math.atan2(vec[0, "a1"][1], vec[0, "a1"][0]))
+    assert abs(ang - deg) < 1e-4, ang
+    A1 = N * vec[0, "a1"] + M * vec[0, "a2"]
+    A2 = -M * vec[0, "a1"] + (M + N) * vec[0, "a2"]
+    B1 = M * vec[1, "a1"] + N * vec[1, "a2"]
+    B2 = -N * vec[1, "a1"] + (M + N) * vec[1, "a2"]
+    assert np.max(np.abs(A1 - B1)) < 1e-9 and np.max(np.abs(A2 - B2)) < 1e-9
+    sc = np.array(od.opts["supercell"])
+    assert np.max(np.abs(sc[0] - A1)) < 1e-9 and np.max(np.abs(sc[1] - A2)) < 1e-9
+
+
+def _brute_csr(orc, od, og, job):
+    """generateRealH / generateCpxH slot rule restated pair by pair from the
+    pair lists (locality.cpp:4120-4192 intra, 4220-4309 inter; complex
+    4840-4975, 5060-5140): per row, the intralayer pairs in list order, then
+    the interlayer pairs; a run of consecutive pairs with the same (row,
+    column) is one slot (periodic images of one column), a slot is stored even
+    when its value is zero, and each pair adds t / ENERGY_RESCALE, the
+    diagonal pairs (every image for a real H, the home image for a complex
+    one) adding (t + ENERGY_SHIFT + z E) / ENERGY_RESCALE; complex entries are
+    polar(t, 3.14e-5 B (x2 - x1)(y1 + y2)/2 + k.(r2 - r1)), r1 the row orbital
+    moved by the pair's supercell image."""
+    L = orc.lib()
+    n = og.n
+    pos = og.job_positions(job).reshape(-1, 3)
+    g = og.index_to_grid.reshape(-1, 4)
+    sc = np.array(od.opts.get("supercell", [[0.0, 0.0], [0.0, 0.0]]))
+    angles = od.angles
+    mats = [sd.mat for sd in od.sdata]
+    cpx = bool(job.complex_matrix)
+    a, shift = job.energy_rescale, job.energy_shift
+    blocks = []
+    for which in (0, 1):
+        ij, ims, t = og.pair_arrays(which)
+        blocks.append((ij, ims, t))
+    rp = [0]
+    cols, vals = [], []
+    cur = [0, 0]
+    for i in range(n):
+        for which in (0, 1):
+            ij, ims, t = blocks[which]
+            q = cur[which]
+            acc, open_col = None, None
+            while q < len(ij) and ij[q, 0] == i:
+                j = int(ij[q, 1])
+                si, sj = int(ims[q, 0]), int(ims[q, 1])
+                x1 = pos[i, 0] + si * sc[0, 0] + sj * sc[1, 0]
+                y1 = pos[i, 1] + si * sc[0, 1] + sj * sc[1, 1]
+                z1 = pos[i, 2]
+                x2, y2, z2 = pos[j]
+                if which == 0:
+                    raw = float(t[q])
+                else:
+                    raw = L.orc_interlayer(mats[g[i, 3]], int(g[i, 2]), int(g[j, 2]), x2 - x1, y2 - y1, z2 - z1,
+                                           angles[g[i, 3]], angles[g[j, 3]])
+                diag = j == i and (not cpx or (si == 0 and sj == 0))
+                if diag:
+                    tv = (raw + shift + z1 * job.E) / a if job.elec_on == 1 else (raw + shift) / a
+                else:
+                    tv = raw / a
+                if cpx:
+                    ph = 3.14e-5 * job.B * (x2 - x1) * 0.5 * (y1 + y2) + job.k[0] * (x2 - x1) + job.k[1] * (y2 - y1)
+                    tv = complex(tv * math.cos(ph), tv * math.sin(ph))
+                if acc is None:
+                    acc = 0j if cpx else 0.0
+                if tv != 0:
+                    acc += tv
+                nxt_same = q + 1 < len(ij) and ij[q + 1, 0] == i and ij[q + 1, 1] == j
+                if not nxt_same:
+                    cols.append(j)
+                    vals.append(acc)
+                    acc = None
+                q += 1
+            cur[which] = q
+        rp.append(len(cols))
+    return np.array(rp), np.array(cols, dtype=np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("case", ["flake", "flake_fields", "grid_real", "grid_k"])
+def test_slot_rule_against_pairwise_restatement(oracle_mod, case):
+    """The CSR pattern and values of the oracle's assembly equal a pair-by-pair
+    restatement of the reference's slot rule built from the pinned pair lists
+    and couplings, on a flake (no images), a flake in B and E fields, and the
+    3 x 3 grid cell where one row meets the same column through several
+    supercell images (merged into one slot), real and at a k-point.  Dropping zero-valued slots, merging
+    non-consecutive repeats, putting the interlayer block first or moving the
+    diagonal shift each change the pattern or the values."""
+    orc = oracle_mod
+    if case.startswith("flake"):
+        text = open(golden_path("tblg_test1.in")).read()
+        if case == "flake_fields":
+            text = _deck_with(text, USE_B_FIELD=1, B_FIELD=300, USE_E_FIELD=1, E_FIELD=0.4, ENERGY_SHIFT=0.3)
+    else:
+        text = open(os.path.join(os.path.dirname(golden_path("x")), "..", "decks", "tblg_grid_3x3.in")).read()
+    od = orc.parse_deck(text)
+    orc.setup_supercell(od)
+    og = orc.Geometry(orc.geom_desc(od))
+    jobs = orc.build_jobs(od, orc.target_list(od, og))
+    job = orc.make_job(od, jobs[-1])
+    if case == "grid_real":
+        job.complex_matrix = 0
+        job.k[0], job.k[1] = 0.0, 0.0
+    if case == "grid_k":
+        assert job.complex_matrix == 1 and (job.k[0] != 0 or job.k[1] != 0)
+    rp, ci, v = og.assemble(job)
+    brp, bci, bv = _brute_csr(orc, od, og, job)
+    assert np.array_equal(rp, brp)
+    assert np.array_equal(ci, bci)
+    assert np.max(np.abs(v - bv)) <= 1e-15 * max(1.0, float(np.max(np.abs(bv))))
+    if case == "grid_real":
+        # the grid cell does exercise the rule: runs of one column through several
+        # images (fewer slots than pairs)
+        npairs = sum(len(og.pair_arrays(w)[0]) for w in (0, 1))
+        assert npairs > len(ci) + 1000
+    if case == "flake":
+        # ... and the flake keeps zero-coupling home-image slots
+        assert np.sum(v == 0) > 100
