@@ -177,9 +177,9 @@ struct leo2d_handle {
         int tile_rows = 0, ntiles = 0;          // current tiling
         std::vector<int32_t> tile_row0, job_tile0;
         DevBuf dtile_row0, djob_tile0;
-        // autotuned launch shape, per (R, precision)
+        // autotuned launch shape, per (R, precision, vector element type) of the current stack
         struct Tuned {
-            int R, precision, mode, tile_rows, grid;
+            int R, precision, vec_type, mode, tile_rows, grid;
         };
         std::vector<Tuned> tuned;
         // captured recursion (valid while the key matches)
@@ -782,6 +782,7 @@ void batch_stack(leo2d_handle* h)
     to_device(B.drow0, B.job_row0.data(), sizeof(int64_t) * B.job_row0.size(), s, "batch job rows");
     check(hipStreamSynchronize(s), "batch job rows");
     B.tile_rows = 0;
+    B.tuned.clear();   // shapes were tuned on the previous stack
     B.drop_graph();
 }
 
@@ -2150,7 +2151,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         if (p->tune) {
             const leo2d_handle::Batch::Tuned* hit = nullptr;
             for (const auto& t : B.tuned)
-                if (t.R == R && t.precision == p->precision) hit = &t;
+                if (t.R == R && t.precision == p->precision && t.vec_type == vec_type) hit = &t;
             if (hit) {
                 mode = hit->mode;
                 T = hit->tile_rows;
@@ -2196,7 +2197,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 float tms = 0.f;
                 check(hipEventElapsedTime(&tms, e0, e3), "event");
                 tune_ms = tms;
-                B.tuned.push_back({R, p->precision, mode, T, grid});
+                B.tuned.push_back({R, p->precision, vec_type, mode, T, grid});
             }
         }
         set_shape(mode, T, grid);

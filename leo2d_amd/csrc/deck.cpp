@@ -174,7 +174,9 @@ Deck parse_deck(const std::string& text)
             else if (key == "MAT_FROM_FILE") o.i["mat_from_file"] = std::atoi(val().c_str());
             else if (key == "MLMC") o.i["mlmc"] = std::atoi(val().c_str());
             else if (key == "GSFE_Z_ON") o.i["gsfe_z_on"] = std::atoi(val().c_str());
-            else if (key == "TRILAYER_SUPERCELL") o.i["supercell_type"] = 3;   // main.cpp:294-302
+            else if (key == "TRILAYER_SUPERCELL") {   // main.cpp:294-302: the type changes only for the value 1
+                if (std::atoi(val().c_str()) == 1) o.i["supercell_type"] = 3;
+            }
             else if (key == "H_SUPERCELL_MODIFY") o.i["h_supercell_modify"] = std::atoi(val().c_str());
             else if (key == "HEX_SUPERCELL_MODIFY") o.i["hex_supercell_modify"] = std::atoi(val().c_str());
             else if (key == "SUPERCELL_M_N") {

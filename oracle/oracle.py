@@ -303,7 +303,10 @@ def parse_deck(text: str) -> Deck:
             elif tok in ("CHIRAL_ON", "BALLISTIC_TRANSPORT", "MAT_SOC_ON", "MAT_FROM_FILE", "MLMC", "GSFE_Z_ON"):
                 nxt(); opts[tok.lower()] = atoi(nxt())
             elif tok == "TRILAYER_SUPERCELL":
-                opts["supercell_type"] = 3
+                # main.cpp:294-302: the supercell type changes only for the value 1
+                nxt()
+                if atoi(nxt()) == 1:
+                    opts["supercell_type"] = 3
             elif tok == "H_SUPERCELL_MODIFY":
                 nxt(); opts["h_supercell_modify"] = atoi(nxt())
             elif tok == "HEX_SUPERCELL_MODIFY":

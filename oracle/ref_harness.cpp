@@ -6,7 +6,9 @@
 // public functions.
 #include <chrono>
 #include <cstdio>
+#include <complex>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "geom/sdata.h"
@@ -182,6 +184,108 @@ double ref_spmv_time(int n, int nnz, const int* row_ptr, const int* col, const d
                              reinterpret_cast<std::complex<double>*>(y), std::complex<double>(1.0),
                              std::complex<double>(0.0));
     }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The reference's KPM loop (Locality::computeDosKPM / computeDosTraceKPM,
+// locality.cpp:5990-6041 and 6138-6346) restated around its own
+// SpMatrix::vectorMultiply, for R start vectors on `nthreads` threads (vector r
+// on thread r % nthreads; the matrix is built once and only read): per step one
+// SpMV T_next = 2 H T_j, the subtract of T_prev and the two buffer copies, with
+// mu_j = Re<v|T_j> for j < M/2 and the doubling identities mu_2j = 2<T_j|T_j> -
+// mu_0, mu_2j+1 = 2 Re<T_j+1|T_j> - mu_1 for j >= M/4, as the reference fills
+// cheb_coeffs.  vecs is [R][n] (real) or [R][2n] (complex, interleaved).
+// Returns the seconds spent in the recursions (matrix copy outside).
+double ref_kpm_time(int n, int nnz, const int* row_ptr, const int* col, const double* val, int is_complex,
+                    const double* vecs, int R, int M, int nthreads, double* mu /*[R][M]*/)
+{
+    SpMatrix H;
+    H.setup(nnz, n, n);
+    int* ci = H.allocColIndx();
+    int* rp = H.allocRowPtr();
+    std::memcpy(rp, row_ptr, sizeof(int) * (n + 1));
+    std::memcpy(ci, col, sizeof(int) * nnz);
+    if (!is_complex) std::memcpy(H.allocRealVal(), val, sizeof(double) * nnz);
+    else std::memcpy(H.allocCpxVal(), val, sizeof(double) * 2 * nnz);
+
+    auto run_one = [&](int r) {
+        double* m = mu + (size_t)r * M;
+        if (is_complex) {
+            typedef std::complex<double> cd;
+            const cd* v = reinterpret_cast<const cd*>(vecs) + (size_t)r * n;
+            std::vector<cd> T_prev(v, v + n), T_j(n), T_next(n);
+            H.vectorMultiply(T_prev.data(), T_j.data(), cd(1.0), cd(0.0));
+            double m0 = 0, m1 = 0;
+            for (int c = 0; c < n; ++c) {
+                m0 += std::real(std::conj(v[c]) * v[c]);
+                m1 += std::real(std::conj(v[c]) * T_j[c]);
+            }
+            m[0] = m0;
+            m[1] = m1;
+            H.vectorMultiply(T_j.data(), T_next.data(), cd(2.0), cd(0.0));
+            for (int c = 0; c < n; ++c) T_next[c] = T_next[c] - T_prev[c];
+            for (int j = 2; j < M / 2; ++j) {
+                for (int c = 0; c < n; ++c) {
+                    T_prev[c] = T_j[c];
+                    T_j[c] = T_next[c];
+                }
+                double mj = 0;
+                for (int c = 0; c < n; ++c) mj += std::real(std::conj(v[c]) * T_j[c]);
+                m[j] = mj;
+                H.vectorMultiply(T_j.data(), T_next.data(), cd(2.0), cd(0.0));
+                for (int c = 0; c < n; ++c) T_next[c] = T_next[c] - T_prev[c];
+                if (j >= M / 4) {
+                    double an_an = 0, anp_an = 0;
+                    for (int c = 0; c < n; ++c) {
+                        an_an += std::real(std::conj(T_j[c]) * T_j[c]);
+                        anp_an += std::real(std::conj(T_next[c]) * T_j[c]);
+                    }
+                    m[2 * j] = 2 * an_an - m[0];
+                    m[2 * j + 1] = 2 * anp_an - m[1];
+                }
+            }
+        } else {
+            const double* v = vecs + (size_t)r * n;
+            std::vector<double> T_prev(v, v + n), T_j(n), T_next(n);
+            H.vectorMultiply(T_prev.data(), T_j.data(), 1.0, 0.0);
+            double m0 = 0, m1 = 0;
+            for (int c = 0; c < n; ++c) {
+                m0 += v[c] * v[c];
+                m1 += v[c] * T_j[c];
+            }
+            m[0] = m0;
+            m[1] = m1;
+            H.vectorMultiply(T_j.data(), T_next.data(), 2.0, 0.0);
+            for (int c = 0; c < n; ++c) T_next[c] = T_next[c] - T_prev[c];
+            for (int j = 2; j < M / 2; ++j) {
+                for (int c = 0; c < n; ++c) {
+                    T_prev[c] = T_j[c];
+                    T_j[c] = T_next[c];
+                }
+                double mj = 0;
+                for (int c = 0; c < n; ++c) mj += v[c] * T_j[c];
+                m[j] = mj;
+                H.vectorMultiply(T_j.data(), T_next.data(), 2.0, 0.0);
+                for (int c = 0; c < n; ++c) T_next[c] = T_next[c] - T_prev[c];
+                if (j >= M / 4) {
+                    double an_an = 0, anp_an = 0;
+                    for (int c = 0; c < n; ++c) {
+                        an_an += T_j[c] * T_j[c];
+                        anp_an += T_next[c] * T_j[c];
+                    }
+                    m[2 * j] = 2 * an_an - m[0];
+                    m[2 * j + 1] = 2 * anp_an - m[1];
+                }
+            }
+        }
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&, t]() {
+            for (int r = t; r < R; r += nthreads) run_one(r);
+        });
+    for (auto& th : pool) th.join();
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -283,3 +283,25 @@ def test_conductivity_deck_keys(oracle_mod):
     assert L.Deck(base).get_int("cond_poly_par") == 0
     dos = text.replace("OBSERVABLE_TYPE = COND", "OBSERVABLE_TYPE = DOS")
     assert L.Deck(dos).get_int("observable_type") == 0
+
+
+def test_trilayer_supercell_value(oracle_mod):
+    """TRILAYER_SUPERCELL changes the supercell type only when its value is 1
+    (main.cpp:294-302): '= 0' leaves the (M, N) cell as it is, '= 1' selects the
+    trilayer cell, which this path refuses (library and oracle alike)."""
+    base = deck_text("tblg_sc_7_6_k.in")
+    g0 = L.Geometry(L.Deck(base))
+    text = base.replace("POLY_ORDER", "TRILAYER_SUPERCELL = 0\nPOLY_ORDER", 1)
+    g = L.Geometry(L.Deck(text))
+    assert g.n == g0.n and g.info.angle[1] == g0.info.angle[1]
+    od = oracle_mod.parse_deck(text)
+    oracle_mod.setup_supercell(od)
+    assert od.opts["supercell_type"] == 1
+    text = base.replace("POLY_ORDER", "TRILAYER_SUPERCELL = 1\nPOLY_ORDER", 1)
+    with pytest.raises(L.Leo2dError) as e:
+        L.Geometry(L.Deck(text))
+    assert e.value.code == L.ERR_DECK
+    od = oracle_mod.parse_deck(text)
+    assert od.opts["supercell_type"] == 3
+    with pytest.raises(oracle_mod.DeckError):
+        oracle_mod.setup_supercell(od)

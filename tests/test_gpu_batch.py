@@ -175,3 +175,105 @@ def test_batch_target_starts(batch, oracle_mod):
         bad = targets.copy()
         bad[0, 0] = jobs[0][0]          # one past the first job's last orbital
         dev.batch_kpm(M, Rt, targets=bad)
+
+
+def _gamma_job(dev, geom):
+    """Gamma-point job: real H (no Bloch phase), Gershgorin interval with 1 % padding."""
+    job = geom.jobs()[0]
+    job.complex_matrix = 0
+    job.k[0], job.k[1] = 0.0, 0.0
+    job.energy_rescale, job.energy_shift = 1.0, 0.0
+    dev.assemble(job, L.PATTERN_PRUNED)
+    lo, hi = dev.spectral_bounds()
+    job.energy_rescale, job.energy_shift = 1.01 * (hi - lo) / 2.0, -(hi + lo) / 2.0
+    dev.assemble(job, L.PATTERN_PRUNED)
+
+
+@pytest.mark.parametrize("Rt", [4, 16, 32])
+def test_batch_real_target_starts(oracle_mod, Rt):
+    """Real H (Gamma point) with unit-vector starts runs on real vector blocks
+    (FP64 and FP32 elements): every lane layout, both precisions, against the
+    oracle's faithful KPM from the same targets (FP64 within 1e-12, FP32
+    recursion within 2e-5 of mu_0 = 1)."""
+    dev = L.Device(0)
+    try:
+        rng = np.random.default_rng(Rt)
+        want, tg = [], []
+        for i in (3, 5):
+            geom = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=M)))
+            dev.set_geometry(geom)
+            dev.pairs_build()
+            _gamma_job(dev, geom)
+            rp, ci, v = dev.csr_copy()
+            assert not np.iscomplexobj(v)
+            t = rng.choice(geom.n, size=Rt, replace=False).astype(np.int32)
+            tg.append(t)
+            want.append(oracle_mod.kpm(rp, ci, v, M, targets=t, doubling_all=True, nthreads=8))
+            dev.batch_append()
+        tg = np.array(tg)
+        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS, L.BATCH_ROW_CHUNKS):
+            if mode == L.BATCH_ROW_CHUNKS and Rt < 16:
+                continue
+            for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+                print("case R", Rt, "mode", mode, "prec", prec, flush=True)
+                mu = dev.batch_kpm(M, Rt, targets=tg, precision=prec, mode=mode, graph=False)
+                for p, omu in enumerate(want):
+                    assert np.max(np.abs(mu[p] - omu)) <= tol, (Rt, mode, prec, p)
+    finally:
+        dev.close()
+
+
+def test_batch_tuning_cache_per_vector_type():
+    """On a real-H stack, random-phase starts run complex vector blocks and
+    target starts real ones: a shape tuned for one element type is not reused
+    for the other (the cache is keyed on it), a repeat of either is, and
+    restacking the batch drops the cached shapes."""
+    d2 = L.Device(0)
+    try:
+        geom = L.Geometry(L.Deck(W.tblg_cell_deck(4, 3, M=M)))
+        d2.set_geometry(geom)
+        d2.pairs_build()
+        _gamma_job(d2, geom)
+        d2.batch_append()
+        tg = np.array([[0, 5, 9, 11, 20, 31, 40, 41, 50, 60, 70, 80, 90, 100, 110, 120]], dtype=np.int32)
+        d2.batch_kpm(M, 16, seed=1, tune=True)
+        assert d2.batch_timing().candidates > 0
+        d2.batch_kpm(M, 16, targets=tg, tune=True)
+        assert d2.batch_timing().candidates > 0          # real blocks: tuned afresh
+        d2.batch_kpm(M, 16, targets=tg, tune=True)
+        assert d2.batch_timing().candidates == 0         # cached
+        d2.batch_kpm(M, 16, seed=1, tune=True)
+        assert d2.batch_timing().candidates == 0         # cached
+        d2.batch_append()                                # restack: cached shapes dropped
+        d2.batch_kpm(M, 16, seed=1, tune=True)
+        assert d2.batch_timing().candidates > 0
+    finally:
+        d2.close()
+
+
+def test_batch_c2_cells(oracle_mod):
+    """Cells from the batched workload's range (i = 9 and 13: 1,084 and 2,188
+    orbitals), four Bloch k-points each, 16 random-phase vectors per job,
+    M = 1024, tuned launch shape: every job's FP64 moments within 1e-12 mu_0
+    of the oracle."""
+    dev = L.Device(0)
+    try:
+        rng = np.random.default_rng(1)
+        want = []
+        Mb, Rb = 1024, 16
+        for i in (9, 13):
+            geom = L.Geometry(L.Deck(W.tblg_cell_deck(i + 1, i, M=Mb)))
+            assert geom.n == W.cell_orbitals(i)
+            dev.set_geometry(geom)
+            dev.pairs_build()
+            for kf in rng.random((4, 2)):
+                W.assemble_rescaled(dev, geom, kf)
+                rp, ci, v = dev.csr_copy()
+                p = dev.batch_append()
+                vecs = oracle_mod.random_phase(geom.n, Rb, 5, r0=p * Rb)
+                want.append((geom.n, oracle_mod.kpm(rp, ci, v, Mb, vecs=vecs, doubling_all=True, nthreads=16)))
+        mu = dev.batch_kpm(Mb, Rb, seed=5, tune=True)
+        for p, (n, omu) in enumerate(want):
+            assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, p
+    finally:
+        dev.close()

@@ -1171,3 +1171,32 @@ def test_slot_rule_against_pairwise_restatement(oracle_mod, case):
     if case == "flake":
         # ... and the flake keeps zero-coupling home-image slots
         assert np.sum(v == 0) > 100
+
+
+@pytest.mark.parametrize("deckname,vec", [("tblg_test1.in", "real"), ("tblg_test2.in", "phase")])
+def test_kpm_loop_matches_reference_loop(oracle_mod, ref_lib, deckname, vec):
+    """The oracle's faithful KPM (orc_kpm, reference scheme with vector starts)
+    equals, bit for bit, the reference's own loop around its own
+    SpMatrix::vectorMultiply (oracle/_ref ref_kpm_time, locality.cpp:5990-6041,
+    6138-6346), run on three threads: real H with real vectors, complex H(B)
+    with random-phase vectors.  The same library call is bench.py's CPU
+    baseline."""
+    orc = oracle_mod
+    L = ref_lib
+    L.ref_kpm_time.restype = C.c_double
+    L.ref_kpm_time.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                               C.c_int, C.c_int, C.c_int, C.c_void_p]
+    deck, geom, jobs, _ = orc.run_deck(open(golden_path(deckname)).read())
+    rp, ci, v = geom.assemble(orc.make_job(deck, jobs[0]))
+    n, M, R = geom.n, 120, 5
+    cpx = np.iscomplexobj(v)
+    vecs = orc.random_phase(n, R, 3) if vec == "phase" else np.random.default_rng(1).standard_normal((R, n))
+    mu = np.zeros((R, M))
+    rp32 = np.ascontiguousarray(rp, dtype=np.int32)
+    ci32 = np.ascontiguousarray(ci, dtype=np.int32)
+    vv = np.ascontiguousarray(v.view(np.float64) if cpx else v)
+    vf = np.ascontiguousarray(vecs).view(np.float64) if np.iscomplexobj(vecs) else np.ascontiguousarray(vecs)
+    assert np.iscomplexobj(vecs) == cpx
+    L.ref_kpm_time(n, len(ci), _p(rp32), _p(ci32), _p(vv), int(cpx), _p(vf), R, M, 3, _p(mu))
+    omu = orc.kpm(rp, ci, v, M, vecs=vecs, nthreads=2)
+    assert np.array_equal(mu, omu)
