@@ -655,6 +655,7 @@ def cpu_baseline(dev, nnz, n, mu_gpu, log):
         t1 = L.ref_spmv_time(n, len(ci), rp32.ctypes.data, ci32.ctypes.data, vv.ctypes.data, int(cpx),
                              x.ctypes.data, y.ctypes.data, 2) / 2
         steps = int(max(16, min(1024, 12.0 / max(t1 * 1.6, 1e-6))))
+    steps -= steps % 2          # M = 2 steps, a multiple of 4 (POLY_ORDER % 4, main.cpp:914)
     M_s = 2 * steps
     vecs = orc.random_phase(n, R_s, 0)
     out = {}

@@ -1635,6 +1635,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 }
             }
             leo2d::SellStepArgs ss{};
+            const bool sell_alternate = env_int("LEO2D_SELL_ALTERNATE", 1) != 0;
             if (sa.variant == kVariantSell) {
                 // one workgroup per CU with two LDS buffers of kSellUcap x 256 B; tiles of at most
                 // tmax rows whose union fits ucap staged rows
@@ -1672,7 +1673,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     ss.eu = (const uint16_t*)S.eu.p;
                     ss.tile_rows = S.tile_rows;
                     ss.max_len = S.max_len;
-                    ss.sliced = env_int("LEO2D_SELL_SLICED", 1);
+                    ss.sliced = env_int("LEO2D_SELL_SLICED", 0);
                 }
             }
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
@@ -1727,6 +1728,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     ss.out = sa.out;
                     ss.alpha = sa.alpha;
                     ss.beta = sa.beta;
+                    ss.reverse = sell_alternate && (j & 1);
                     check(leo2d::launch_kpm_step_sell(mat_type, vec_type, ss, s), "kpm step");
                 } else {
                     check(leo2d::launch_kpm_step(mat_type, vec_type, sa, s), "kpm step");

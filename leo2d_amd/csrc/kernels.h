@@ -106,6 +106,7 @@ struct SellStepArgs {
     int tile_rows;              // most rows in a tile (<= kSellMaxRows)
     int max_len;                // most steps in a group (<= kSellMaxLen)
     int sliced;                 // 1: XCD x walks its own contiguous eighth of the tiles
+    int reverse;                // 1: every workgroup walks its tiles last to first (alternate steps)
     int half;                   // 1: half-size workgroups (kSellWaves / 2 wavefronts, kSellUcap / 2 rows
                                 //    per buffer, tiles of at most 32 rows), two per CU
 };

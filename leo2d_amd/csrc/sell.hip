@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 #include "kpm_common.h"
@@ -90,12 +91,61 @@ __device__ inline void ld_block(XT (&x)[4][VPL], const GroupRegs<HT, QS>& g, con
     }
 }
 
+// Complex products with the entry broadcast folded into the multiply-adds: the
+// DPP form of v_fmac reads src0 from lane K of each 16-lane row, so one step is
+// four (FP64, one vector) or eight (FP32, two vectors) fused multiply-adds in
+// the order of mac(): re += hr xr, re -= hi xi, im += hr xi, im += hi xr.  DPP
+// needs two wait states after a VALU write of its source (the entry registers
+// are rewritten when a tile starts), hence one s_nop 1 per step.
+template <int K> struct DppMac;
+#define LEO2D_DPP_MAC(K)                                                                                          \
+    template <> struct DppMac<K> {                                                                                \
+        __device__ static inline void f64(double& ax, double& ay, double hr, double hi, double xx, double xy)     \
+        {                                                                                                          \
+            asm("s_nop 1\n\t"                                                                                    \
+                "v_fmac_f64_dpp %0, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f64_dpp %0, -%3, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
+                "v_fmac_f64_dpp %1, %2, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f64_dpp %1, %3, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf"                       \
+                : "+v"(ax), "+v"(ay)                                                                               \
+                : "v"(hr), "v"(hi), "v"(xx), "v"(xy));                                                             \
+        }                                                                                                          \
+        __device__ static inline void f32(float& a0x, float& a0y, float& a1x, float& a1y, float hr, float hi,    \
+                                          float x0x, float x0y, float x1x, float x1y)                              \
+        {                                                                                                          \
+            asm("s_nop 1\n\t"                                                                                    \
+                "v_fmac_f32_dpp %0, %4, %6 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %0, -%5, %7 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
+                "v_fmac_f32_dpp %1, %4, %7 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %1, %5, %6 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %2, %4, %8 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %2, -%5, %9 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
+                "v_fmac_f32_dpp %3, %4, %9 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %3, %5, %8 row_newbcast:" #K " row_mask:0xf bank_mask:0xf"                       \
+                : "+v"(a0x), "+v"(a0y), "+v"(a1x), "+v"(a1y)                                                       \
+                : "v"(hr), "v"(hi), "v"(x0x), "v"(x0y), "v"(x1x), "v"(x1y));                                       \
+        }                                                                                                          \
+    };
+LEO2D_DPP_MAC(0) LEO2D_DPP_MAC(1) LEO2D_DPP_MAC(2) LEO2D_DPP_MAC(3)
+LEO2D_DPP_MAC(4) LEO2D_DPP_MAC(5) LEO2D_DPP_MAC(6) LEO2D_DPP_MAC(7)
+LEO2D_DPP_MAC(8) LEO2D_DPP_MAC(9) LEO2D_DPP_MAC(10) LEO2D_DPP_MAC(11)
+LEO2D_DPP_MAC(12) LEO2D_DPP_MAC(13) LEO2D_DPP_MAC(14) LEO2D_DPP_MAC(15)
+#undef LEO2D_DPP_MAC
+
 template <int E, typename HT, typename XT, typename AT, int VPL, int QS>
 __device__ inline void mac_step(AT (&acc)[VPL], const XT (&x)[VPL], const GroupRegs<HT, QS>& g)
 {
-    const HT h = bc16<E & 15>(g.ev[E >> 4]);
+    if constexpr (std::is_same<HT, double2>::value && VPL == 1) {
+        const double2 h = g.ev[E >> 4];
+        DppMac<E & 15>::f64(acc[0].x, acc[0].y, h.x, h.y, x[0].x, x[0].y);
+    } else if constexpr (std::is_same<HT, float2>::value && VPL == 2) {
+        const float2 h = g.ev[E >> 4];
+        DppMac<E & 15>::f32(acc[0].x, acc[0].y, acc[1].x, acc[1].y, h.x, h.y, x[0].x, x[0].y, x[1].x, x[1].y);
+    } else {
+        const HT h = bc16<E & 15>(g.ev[E >> 4]);
 #pragma unroll
-    for (int v = 0; v < VPL; ++v) mac(acc[v], h, x[v]);
+        for (int v = 0; v < VPL; ++v) mac(acc[v], h, x[v]);
+    }
 }
 
 // steps 4B .. L-1 of the group (L a multiple of 4, at most 16 QS); xa holds the
@@ -165,6 +215,17 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
         t_end = a.ntiles;
         t_stride = G;
     }
+    // the workgroup's tiles are t_first + k t_stride < t_end; on reversed steps
+    // they are walked last to first, so a step begins with the rows the previous
+    // step wrote last (still in L2 and the memory-side cache)
+    if (a.reverse && t_first < t_end) {
+        const int last = t_first + (t_end - 1 - t_first) / t_stride * t_stride;
+        t_end = t_first - 1;                       // walk down to t_first
+        t_first = last;
+        t_stride = -t_stride;
+    }
+    // tile t of the walk exists (t_end is exclusive in the walk's direction)
+    auto live = [&](int t) { return t_stride > 0 ? t < t_end : t > t_end; };
 
     const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
     const XT* prev = static_cast<const XT*>(a.prev);
@@ -231,12 +292,12 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     int ucs[SPW];
     int t = t_first, c = 0, p = 0;
     int r0 = 0, r0n = 0;
-    if (t < t_end) {
+    if (live(t)) {
         load_groups(t, ec, gc_row, gc_len);
         r0 = a.tile_row0[t];
         load_ucols(t, ucs);
         stage(xs0, 0, ucs);
-        if (NCH == 1) load_ucols(t + t_stride < t_end ? t + t_stride : t, ucs);
+        if (NCH == 1) load_ucols(live(t + t_stride) ? t + t_stride : t, ucs);
     }
 
     auto pair = [&](XT* buf, XT* nbuf) {
@@ -273,7 +334,7 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
             c1 = 0;
             t1 = t + t_stride;
         }
-        const bool has1 = t1 < t_end;
+        const bool has1 = live(t1);
         stage(nbuf, has1 ? c1 : 0, ucs);
         // products of every group of the pair
         AT acc[GPW][VPL];
@@ -329,7 +390,7 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
             c2 = 0;
             t2 = t1 + t_stride;
         }
-        if (c2 == 0) load_ucols(t2 < t_end ? t2 : t, ucs);
+        if (c2 == 0) load_ucols(live(t2) ? t2 : t, ucs);
         if (c1 == 0 && has1) {
             load_groups(t1, en, gn_row, gn_len);
             r0n = a.tile_row0[t1];
@@ -338,9 +399,9 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
         c = c1;
         ++p;
     };
-    while (t < t_end) {
+    while (live(t)) {
         pair(xs0, xs1);
-        if (t >= t_end) break;
+        if (!live(t)) break;
         pair(xs1, xs0);
     }
     wait_vm0();
