@@ -1639,11 +1639,8 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             if (sa.variant == kVariantSell) {
                 // one workgroup per CU with two LDS buffers of kSellUcap x 256 B; tiles of at most
                 // tmax rows whose union fits ucap staged rows
-                const int half = env_int("LEO2D_SELL_HALF", 0) && B == 64 && !keep_v;
-                const int ucmax = half ? leo2d::kSellUcap / 2 : leo2d::kSellUcap;
-                const int tmmax = half ? leo2d::kSellMaxRows / 2 : leo2d::kSellMaxRows;
-                const int ucap = std::max(32, std::min(ucmax, env_int("LEO2D_SELL_UCAP", ucmax)));
-                const int tmax = std::max(4, std::min(tmmax, env_int("LEO2D_SELL_TMAX", tmmax)));
+                const int ucap = std::max(32, std::min(leo2d::kSellUcap, env_int("LEO2D_SELL_UCAP", leo2d::kSellUcap)));
+                const int tmax = std::max(4, std::min(leo2d::kSellMaxRows, env_int("LEO2D_SELL_TMAX", leo2d::kSellMaxRows)));
                 if (!spatial || !leo2d::sell_supported(mat_type, vec_type, B) ||
                     !build_sell(h, drp, dcol, ucap, tmax, spatial)) {
                     sa.variant = B == 16 || B == 32 ? 14 : (mixed ? 15 : 1);
@@ -1655,8 +1652,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     int dev_id = 0, cus = 256;
                     if (hipGetDevice(&dev_id) == hipSuccess)
                         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-                    ss.half = half && S.max_len <= 48;
-                    ss.grid = std::max(8, std::min(G, cus * (half ? 2 : 1)) / 8 * 8);
+                    ss.grid = std::max(8, std::min(G, cus) / 8 * 8);
                     sa.grid = ss.grid;   // the reduce reads this many partials
                     ss.n = n;
                     ss.R = B;

@@ -107,8 +107,6 @@ struct SellStepArgs {
     int max_len;                // most steps in a group (<= kSellMaxLen)
     int sliced;                 // 1: XCD x walks its own contiguous eighth of the tiles
     int reverse;                // 1: every workgroup walks its tiles last to first (alternate steps)
-    int half;                   // 1: half-size workgroups (kSellWaves / 2 wavefronts, kSellUcap / 2 rows
-                                //    per buffer, tiles of at most 32 rows), two per CU
 };
 bool sell_supported(int mat_type, int vec_type, int R);
 hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s);

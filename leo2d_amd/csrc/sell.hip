@@ -466,19 +466,14 @@ hipError_t launch_sell_r(const SellStepArgs& a, hipStream_t s)
     }
 }
 
-// two groups of four rows per wavefront (tiles of at most 64 rows): with the
-// next tile's entries double-buffered in registers, four groups per wavefront
-// exceed the 256 registers a lane has at two wavefronts per SIMD.  The
-// half-size shape (four wavefronts, two workgroups per CU) exists for the
-// 64-vector complex blocks only.
+// Two groups of four rows per wavefront (tiles of at most 64 rows): with the
+// next tile's entries double-buffered in registers, four groups of complex
+// FP64 entries per wavefront exceed the 256 registers a lane has at two
+// wavefronts per SIMD.  (Four groups of complex FP32 entries fit, and were
+// measured slower on the 1.05 deg 8 x 8 cell: 580 against 558 us per step.)
 template <typename HT, typename XT, int VPL>
 hipError_t launch_sell_shape(const SellStepArgs& a, hipStream_t s)
 {
-    if (a.half) {
-        constexpr int NCH = 64 / (16 * VPL);
-        if (a.R != 64 || a.v || a.max_len > 48 || a.tile_rows > 32) return hipErrorInvalidValue;
-        return launch_sell<HT, XT, VPL, NCH, 2, 3, kSellWaves / 2, kSellUcap / 2>(a, s);
-    }
     if (a.max_len > 48) return launch_sell_r<HT, XT, VPL, 2, 4>(a, s);
     return launch_sell_r<HT, XT, VPL, 2, 3>(a, s);
 }

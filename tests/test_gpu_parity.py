@@ -531,15 +531,17 @@ def test_xcd_slice_schedule(dev, oracle_mod, monkeypatch, variant, R):
         assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n, (variant, R, grid)
 
 
-@pytest.mark.parametrize("cfg", [("288", "1", "64"), ("64", "0", "64"), ("128", "1", "13"), ("288", "0", "5")])
+@pytest.mark.parametrize("cfg", [("288", "1", "64"), ("64", "0", "64"), ("128", "1", "13"), ("288", "0", "5"),
+                                 ("200", "1", "48")])
 @pytest.mark.parametrize("R", [16, 32, 64, 128])
 def test_sell_variant_shapes(dev, oracle_mod, monkeypatch, cfg, R):
     """SELL-4-sigma step with LDS-staged tiles (variant 20) under LDS budgets
     of 64 to 288 staged rows (tiles cut short by the budget), tiles of at most
-    5 to 64 rows (partial groups, idle wavefronts) and both tile schedules (XCD
-    slices, interleaved): FP64 moments within 1e-12 mu_0 of the oracle in both
-    schemes, FP32 recursion within 2e-5 mu_0, complex H(k) and a real H.  R =
-    128 falls back to the CSR step."""
+    5 to 64 rows (partial groups, idle wavefronts) and both tile schedules
+    (XCD slices, interleaved), walked forwards and backwards on alternate
+    steps: FP64 moments within 1e-12 mu_0 of the oracle in both schemes, FP32
+    recursion within 2e-5 mu_0, complex H(k) and a real H.  R = 128 falls back
+    to the CSR step."""
     ucap, sliced, tmax = cfg
     monkeypatch.setenv("LEO2D_KPM_VARIANT", "20")
     monkeypatch.setenv("LEO2D_SELL_UCAP", ucap)
