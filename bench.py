@@ -589,12 +589,10 @@ def measured_traffic(kernel, nnz, R):
 
 def kernel_name(variant, t):
     """Name of the fused step kernel a KPM run launched (complex H, R = 64)."""
-    if variant == 8:
-        return f"kpm_step_smem_kernel<{t},{t},1,8>"
     if variant == 1:
-        return f"kpm_step_wave_kernel<{t},{t},1,8,256>"
-    if variant in (15, 16):
-        return f"kpm_step_pair_kernel<{t},{8 if variant == 15 else 16}>"
+        return f"kpm_step_wave_kernel<{t},{t},1,8>"
+    if variant == 15:
+        return f"kpm_step_pair_kernel<{t},8>"
     if variant == 14:
         return f"kpm_step_xchunk_kernel<{t},{t},4>"
     if variant == 20:

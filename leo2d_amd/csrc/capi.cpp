@@ -52,16 +52,18 @@ struct NanError : std::runtime_error {
 // the SELL-4-sigma step with LDS-staged row tiles (sell.hip)
 constexpr int kVariantSell = 20;
 
-// step kernel for a block of B vectors (LEO2D_KPM_VARIANT overrides the
-// default).  B >= 64: the wavefront-per-row kernel with readlane broadcasts (1)
-// for FP64; for the FP32 recursion, two rows per wavefront with 16-byte gathers
-// and the matrix on the scalar path (15; it falls back to the one-row
-// scalar-path kernel 8 where it does not apply).  B = 16 or 32 in the spatial
-// order: vector chunks split over the XCDs (14; 1.2-1.3x the lane-per-vector
-// kernel on a 4.7 M-orbital flake at B = 32).  Other B: lane per vector (0).
+// step kernel for a block of B vectors (LEO2D_KPM_VARIANT = 0, 1, 14, 15 or 20
+// overrides the default).  In the spatial order: B >= 64 runs the SELL-4-sigma
+// step with LDS-staged row tiles (20); B = 16 or 32 vector chunks split over
+// the XCDs (14).  In the natural order (or where 20 does not apply): B >= 64
+// the wavefront-per-row kernel (1), or for the FP32 recursion at B = 64 two rows
+// per wavefront with 16-byte gathers (15); other B lane per vector (0).
 int kpm_variant(bool mixed, bool spatial, int B)
 {
-    if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) return std::atoi(e);
+    if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) {
+        const int v = std::atoi(e);
+        if (v == 0 || v == 1 || v == 14 || v == 15 || v == kVariantSell) return v;
+    }
     if (spatial && (B == 16 || B == 32)) return 14;
     if (spatial && B >= 64) return kVariantSell;
     return mixed && spatial ? 15 : 1;
@@ -128,12 +130,6 @@ struct leo2d_handle {
     DevBuf perm, iperm;
     bool pcsr_valid = false, pval32_valid = false;
     DevBuf prp, pcol_k, pval, pval32;
-    // LDS tiles of the matrix the recursion runs on (built on first use)
-    struct Tiles {
-        bool valid = false, failed = false, spatial = false;
-        int ucap = 0, ntiles = 0;
-        DevBuf row0, u0, ucols, lcol;
-    } tiles;
     // SELL-4-sigma row tiles of the recursion matrix for the LDS-staged step
     // (sell.hip): pattern built on the host on first use, values gathered on the device
     struct Sell {
@@ -144,21 +140,6 @@ struct leo2d_handle {
         DevBuf row0, u0, ucols, g0, grow, glen, goff, eu, map, val, val32;
         bool val_valid = false, val32_valid = false;
     } sell;
-    // row groups: kGroupRows consecutive rows of the recursion matrix with their
-    // merged column list and a value per (column, row) (zero where absent)
-    struct Groups {
-        bool valid = false, spatial = false;
-        int ngroups = 0;
-        int64_t nent = 0;
-        DevBuf rp, col, val, val32;
-        bool val32_valid = false;
-    } groups;
-    // row tiles of the vector-chunked step (variant 13) and each workgroup's run of them
-    struct Chunks {
-        bool valid = false, spatial = false;
-        int ecap = 0, tmax = 0, grid = 0, ntiles = 0;
-        DevBuf row0, wg;
-    } chunks;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
     // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
@@ -330,128 +311,11 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->val32_valid = false;
     h->pcsr_valid = false;
     h->pval32_valid = false;
-    h->tiles.valid = false;
     h->sell.valid = false;
     h->sell.failed = false;
     h->sell.val_valid = false;
     h->sell.val32_valid = false;
-    h->groups.valid = false;
-    h->tiles.failed = false;
-    h->chunks.valid = false;
     h->has_dx = false;
-}
-
-// Row tiles for the vector-chunked step: consecutive rows, at most `tmax` rows
-// and `ecap` entries per tile; then `grid` contiguous runs of tiles with about
-// equal entry counts, one per workgroup.  Returns false when one row alone has
-// more than `ecap` entries.
-bool build_chunks(leo2d_handle* h, const int64_t* drp, int ecap, int tmax, int grid, bool spatial)
-{
-    auto& T = h->chunks;
-    if (T.valid && T.ecap == ecap && T.tmax == tmax && T.grid == grid && T.spatial == spatial) return true;
-    const int n = h->n;
-    hipStream_t s = h->stream;
-    std::vector<int64_t> rp(n + 1);
-    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "chunk tiles");
-    check(hipStreamSynchronize(s), "chunk tiles");
-    std::vector<int32_t> row0;
-    std::vector<int64_t> e0;
-    int r = 0;
-    while (r < n) {
-        const int start = r;
-        while (r < n && r - start < tmax && rp[r + 1] - rp[start] <= ecap) ++r;
-        if (r == start) {
-            T.valid = false;
-            return false;
-        }
-        row0.push_back(start);
-        e0.push_back(rp[start]);
-    }
-    const int ntiles = (int)row0.size();
-    row0.push_back(n);
-    const int64_t total = rp[n];
-    std::vector<int32_t> wg(grid + 1);
-    for (int g = 0; g < grid; ++g) {
-        const int64_t want = total * g / grid;
-        wg[g] = (int32_t)(std::lower_bound(e0.begin(), e0.end(), want) - e0.begin());
-    }
-    wg[grid] = ntiles;
-    to_device(T.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "chunk tiles");
-    to_device(T.wg, wg.data(), sizeof(int32_t) * wg.size(), s, "chunk tiles");
-    check(hipStreamSynchronize(s), "chunk tiles");
-    T.ecap = ecap;
-    T.tmax = tmax;
-    T.grid = grid;
-    T.spatial = spatial;
-    T.ntiles = ntiles;
-    T.valid = true;
-    return true;
-}
-
-// Greedy row tiles for the LDS-tiled step: consecutive rows (at most
-// kTileMaxRows) while the tile's distinct columns fit `ucap`.  Returns false
-// when a single row alone has more than `ucap` distinct columns.
-bool build_tiles(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, bool spatial)
-{
-    auto& T = h->tiles;
-    if (T.valid && T.ucap == ucap && T.spatial == spatial) return true;
-    if (T.failed && T.ucap == ucap && T.spatial == spatial) return false;
-    const int n = h->n;
-    hipStream_t s = h->stream;
-    std::vector<int64_t> rp(n + 1);
-    std::vector<int32_t> col((size_t)h->nnz);
-    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "tiles");
-    if (h->nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "tiles");
-    check(hipStreamSynchronize(s), "tiles");
-    std::vector<int32_t> row0, ucols;
-    std::vector<int64_t> u0;
-    std::vector<uint16_t> lcol((size_t)h->nnz);
-    std::vector<int32_t> stamp(n, -1), pos(n, 0);
-    std::vector<int32_t> set;
-    int r = 0, tile = 0;
-    bool ok = true;
-    while (r < n) {
-        const int start = r;
-        set.clear();
-        while (r < n && r - start < leo2d::kTileMaxRows) {
-            int fresh = 0;
-            for (int64_t q = rp[r]; q < rp[r + 1]; ++q)
-                if (stamp[col[q]] != tile) { stamp[col[q]] = tile; ++fresh; set.push_back(col[q]); }
-            if ((int)set.size() > ucap) {
-                // undo this row
-                for (int k = 0; k < fresh; ++k) { stamp[set.back()] = -1; set.pop_back(); }
-                break;
-            }
-            ++r;
-        }
-        if (r == start) { ok = false; break; }
-        std::sort(set.begin(), set.end());
-        for (size_t k = 0; k < set.size(); ++k) pos[set[k]] = (int32_t)k;
-        for (int rr = start; rr < r; ++rr)
-            for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) lcol[q] = (uint16_t)pos[col[q]];
-        row0.push_back(start);
-        u0.push_back((int64_t)ucols.size());
-        ucols.insert(ucols.end(), set.begin(), set.end());
-        ++tile;
-    }
-    T.ucap = ucap;
-    T.spatial = spatial;
-    if (!ok) {
-        T.failed = true;
-        T.valid = false;
-        return false;
-    }
-    row0.push_back(n);
-    u0.push_back((int64_t)ucols.size());
-    to_device(T.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "tile rows");
-    to_device(T.u0, u0.data(), sizeof(int64_t) * u0.size(), s, "tile offsets");
-    to_device(T.ucols, ucols.data(), sizeof(int32_t) * ucols.size(), s, "tile columns");
-    to_device(T.lcol, lcol.data(), sizeof(uint16_t) * lcol.size(), s, "tile local columns");
-    check(hipStreamSynchronize(s), "tiles");
-    T.ntiles = tile;
-    T.valid = true;
-    T.failed = false;
-    return true;
 }
 
 // SELL-4-sigma row tiles for the LDS-staged step (sell.hip).  Greedy tiles of
@@ -605,72 +469,6 @@ const void* sell_values(leo2d_handle* h, const void* csr_val, int elem_bytes, bo
         ok = true;
     }
     return buf.p;
-}
-
-// Row groups of the recursion matrix: rows g*K .. g*K+K-1 share one sorted
-// column list (the union of their columns) and store K values per column,
-// zero where a row has no entry; duplicate (row, column) entries are summed.
-// kCsrPad trailing entries (column 0, values 0) allow whole-chunk loads.
-void build_groups(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, const void* dval, bool spatial)
-{
-    auto& Gr = h->groups;
-    if (Gr.valid && Gr.spatial == spatial) return;
-    constexpr int K = leo2d::kGroupRows;
-    const int n = h->n;
-    const int w = h->is_complex ? 2 : 1;
-    hipStream_t s = h->stream;
-    std::vector<int64_t> rp(n + 1);
-    std::vector<int32_t> col((size_t)h->nnz);
-    std::vector<double> val((size_t)w * h->nnz);
-    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "groups");
-    if (h->nnz) {
-        check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "groups");
-        check(hipMemcpyAsync(val.data(), dval, sizeof(double) * w * h->nnz, hipMemcpyDeviceToHost, s), "groups");
-    }
-    check(hipStreamSynchronize(s), "groups");
-    const int ng = (n + K - 1) / K;
-    std::vector<int64_t> grp(ng + 1, 0);
-    std::vector<int32_t> gcol;
-    std::vector<double> gval;
-    gcol.reserve((size_t)h->nnz);
-    gval.reserve((size_t)h->nnz * K * w);
-    std::vector<std::pair<int32_t, int64_t>> ent;   // (column, k * nnz-index)
-    for (int g = 0; g < ng; ++g) {
-        ent.clear();
-        for (int k = 0; k < K && g * K + k < n; ++k) {
-            const int r = g * K + k;
-            for (int64_t q = rp[r]; q < rp[r + 1]; ++q) ent.push_back({col[q], (int64_t)k << 40 | q});
-        }
-        std::sort(ent.begin(), ent.end(),
-                  [](const std::pair<int32_t, int64_t>& a, const std::pair<int32_t, int64_t>& b) {
-                      return a.first < b.first;
-                  });
-        for (size_t e = 0; e < ent.size();) {
-            const int32_t c = ent[e].first;
-            const size_t base = gval.size();
-            gcol.push_back(c);
-            gval.resize(base + (size_t)K * w, 0.0);
-            for (; e < ent.size() && ent[e].first == c; ++e) {
-                const int k = (int)(ent[e].second >> 40);
-                const int64_t q = ent[e].second & ((1LL << 40) - 1);
-                for (int t = 0; t < w; ++t) gval[base + (size_t)k * w + t] += val[(size_t)w * q + t];
-            }
-        }
-        grp[g + 1] = (int64_t)gcol.size();
-    }
-    Gr.nent = (int64_t)gcol.size();
-    for (int p = 0; p < leo2d::kCsrPad; ++p) {
-        gcol.push_back(0);
-        gval.resize(gval.size() + (size_t)K * w, 0.0);
-    }
-    to_device(Gr.rp, grp.data(), sizeof(int64_t) * grp.size(), s, "group rows");
-    to_device(Gr.col, gcol.data(), sizeof(int32_t) * gcol.size(), s, "group columns");
-    to_device(Gr.val, gval.data(), sizeof(double) * gval.size(), s, "group values");
-    check(hipStreamSynchronize(s), "groups");
-    Gr.ngroups = ng;
-    Gr.spatial = spatial;
-    Gr.valid = true;
-    Gr.val32_valid = false;
 }
 
 // Morton (Z-order) rank of every orbital by in-plane position on a 1.42 A
@@ -1536,104 +1334,6 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             sa.v = keep_v ? V.p : nullptr;
             sa.variant = kpm_variant(mixed, spatial, B);
             sa.xcd_rows = kpm_sched() == 1 ? (n + 7) / 8 : 0;
-            const bool tiled = sa.variant == 3 || sa.variant == 4;
-            if (tiled && (B % leo2d::kTileGroup != 0 ||
-                          !build_tiles(h, drp, dcol, leo2d::tile_ucap(vec_type), spatial)))
-                sa.variant = 1;
-            int lds_bytes = leo2d::kTileWaveLdsBytes;
-            if (const char* kb = std::getenv("LEO2D_TILE_LDS_KB")) lds_bytes = std::max(16, std::min(150, std::atoi(kb))) * 1024;
-            const int ucap5 = lds_bytes / (B * (int)leo2d::vec_bytes(vec_type));
-            if (sa.variant == 5 && ((B != 64 && B != 128) || ucap5 < 8 || !build_tiles(h, drp, dcol, ucap5, spatial)))
-                sa.variant = 1;
-            sa.tile_lds_bytes = lds_bytes;
-            if (sa.variant == 5 && p->grid <= 0) {
-                // one resident round: workgroups per CU from the LDS budget
-                int dev_id = 0, cus = 256;
-                if (hipGetDevice(&dev_id) == hipSuccess)
-                    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-                const int per_cu = std::max(1, (160 * 1024) / (lds_bytes + 4096));
-                sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));
-            }
-            if ((sa.variant == 8 || sa.variant == 9) && (!spatial || B % 64 != 0)) sa.variant = 1;
-            if (sa.variant == 12) {
-                if (!spatial || B != 64) {
-                    sa.variant = 1;
-                } else {
-                    build_groups(h, drp, dcol, h->pval.p, spatial);
-                    auto& Gr = h->groups;
-                    const void* gv = Gr.val.p;
-                    if (mixed) {
-                        if (!Gr.val32_valid) {
-                            const size_t count =
-                                (size_t)(Gr.nent + leo2d::kCsrPad) * leo2d::kGroupRows * (h->is_complex ? 2 : 1);
-                            dalloc(Gr.val32, sizeof(float) * count, "fp32 group values");
-                            check(leo2d::launch_convert_f64_to_f32((const double*)Gr.val.p, (float*)Gr.val32.p,
-                                                                   (int64_t)count, s),
-                                  "fp32 group values");
-                            Gr.val32_valid = true;
-                        }
-                        gv = Gr.val32.p;
-                    }
-                    sa.ngroups = Gr.ngroups;
-                    sa.grp_rp = (const int64_t*)Gr.rp.p;
-                    sa.grp_col = (const int32_t*)Gr.col.p;
-                    sa.grp_val = gv;
-                }
-            }
-            if (sa.variant == 11) {
-                // LDS budget per workgroup (default 48 KiB: three workgroups per CU)
-                int kb = 48;
-                if (const char* e = std::getenv("LEO2D_TILE_LDS_KB")) kb = std::max(8, std::min(150, std::atoi(e)));
-                const int ucap11 = kb * 1024 / (16 * (int)leo2d::vec_bytes(vec_type));
-                if (B % 16 != 0 || B > 128 || ucap11 < 40 || ucap11 > 65535 ||
-                    !build_tiles(h, drp, dcol, ucap11, spatial)) {
-                    sa.variant = 1;
-                } else {
-                    sa.tile_lds_bytes = ucap11 * 16 * (int)leo2d::vec_bytes(vec_type);
-                    if (p->grid <= 0) {
-                        int dev_id = 0, cus = 256;
-                        if (hipGetDevice(&dev_id) == hipSuccess)
-                            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-                        const int per_cu = std::max(1, (160 * 1024) / (sa.tile_lds_bytes + 1024));
-                        sa.grid = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));   // part holds G slots
-                    }
-                }
-            }
-            if (sa.variant == 13) {
-                // workgroup size, LDS budget per workgroup and rows per tile
-                // (LEO2D_CHUNK_THREADS / _LDS_KB / _TMAX override the defaults)
-                auto env_int = [](const char* k, int d) {
-                    const char* e = std::getenv(k);
-                    return e ? std::atoi(e) : d;
-                };
-                const int nt = env_int("LEO2D_CHUNK_THREADS", 512);
-                const int kb = std::max(16, std::min(160, env_int("LEO2D_CHUNK_LDS_KB", 78)));
-                const int tmax = std::max(8, std::min(4096, env_int("LEO2D_CHUNK_TMAX", 256)));
-                const size_t hb = mat_type == leo2d::MAT_CPX_F64 ? 16 : (mat_type == leo2d::MAT_REAL_F32 ? 4 : 8);
-                const size_t fixed = sizeof(double) * (nt / 64) * B * 3 + sizeof(int32_t) * (tmax + 1);
-                const long ecap = ((long)kb * 1024 - (long)fixed) / (long)(hb + 4);
-                const bool type_ok = vec_type == leo2d::VEC_CPX_F64 || vec_type == leo2d::VEC_CPX_F32 ||
-                                     vec_type == leo2d::VEC_REAL_F64;
-                int grid13 = G;
-                if (p->grid <= 0) {
-                    int dev_id = 0, cus = 256;
-                    if (hipGetDevice(&dev_id) == hipSuccess)
-                        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
-                    const int per_cu = std::max(1, std::min((160 * 1024) / (kb * 1024), 2048 / std::max(64, nt)));
-                    grid13 = std::min(G, std::max(8, (cus * per_cu) / 8 * 8));   // part holds G slots
-                }
-                if (!spatial || (B != 64 && B != 128) || !type_ok || (nt != 256 && nt != 512 && nt != 1024) ||
-                    ecap < 64 || !build_chunks(h, drp, (int)ecap, tmax, grid13, spatial)) {
-                    sa.variant = 1;
-                } else {
-                    sa.grid = grid13;
-                    sa.chunk_row0 = (const int32_t*)h->chunks.row0.p;
-                    sa.chunk_wg = (const int32_t*)h->chunks.wg.p;
-                    sa.chunk_ecap = (int)ecap;
-                    sa.chunk_tmax = tmax;
-                    sa.chunk_threads = nt;
-                }
-            }
             leo2d::SellStepArgs ss{};
             const bool sell_alternate = env_int("LEO2D_SELL_ALTERNATE", 1) != 0;
             if (sa.variant == kVariantSell) {
@@ -1673,32 +1373,9 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 }
             }
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
-            if ((sa.variant == 15 || sa.variant == 16) && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32))
-                sa.variant = mixed && spatial && B % 64 == 0 ? 8 : 1;
-            if (sa.variant == 17) {
-                // 16-byte lanes: rows per wavefront = 1024 / (B x element); complex vectors only
-                const int per_row = B * (int)leo2d::vec_bytes(vec_type);
-                const bool cpx = vec_type == leo2d::VEC_CPX_F64 || vec_type == leo2d::VEC_CPX_F32;
-                // (a complex FP64 matrix does not fit the scalar registers of this kernel)
-                if (!spatial || !cpx || mat_type == leo2d::MAT_CPX_F64 || (1024 / per_row != 2 && 1024 / per_row != 4) ||
-                    1024 % per_row != 0)
-                    sa.variant = B >= 64 ? 1 : 0;
-            }
-            if (B < 64 && sa.variant != 3 && sa.variant != 4 && sa.variant != 11 && sa.variant != 14 && sa.variant != 17 &&
-                sa.variant != kVariantSell)
-                sa.variant = 0;
+            if (sa.variant == 15 && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32)) sa.variant = 1;
+            if (B < 64 && sa.variant != 14 && sa.variant != kVariantSell) sa.variant = 0;
             kernel_used = sa.variant;
-            if ((sa.variant == 6 || sa.variant == 7) && p->grid <= 0) {
-                // larger workgroups: keep the total number of resident wavefronts
-                sa.grid = std::max(8, (sa.variant == 6 ? G / 4 : G / 2) / 8 * 8);
-            }
-            if (sa.variant == 3 || sa.variant == 4 || sa.variant == 5 || sa.variant == 11) {
-                sa.ntiles = h->tiles.ntiles;
-                sa.tile_row0 = (const int32_t*)h->tiles.row0.p;
-                sa.tile_u0 = (const int64_t*)h->tiles.u0.p;
-                sa.ucols = (const int32_t*)h->tiles.ucols.p;
-                sa.lcol = (const uint16_t*)h->tiles.lcol.p;
-            }
             leo2d::ReduceArgs ra{};
             ra.part = (const double*)part.p;
             ra.grid = sa.grid;

@@ -33,45 +33,14 @@ struct StepArgs {
     int beta;
     double* part;   // [grid][R][3]
     int grid;       // number of workgroups (multiple of 8)
-    int variant;    // R >= 64: 0 = lane-per-vector loop, 1/2 = wavefront-per-row with 8/16 gathers in flight,
-                    // 8/9 = wavefront-per-row with H on the scalar path (needs kCsrPad padding)
-    // LDS-tiled variant (variant 3): rows grouped into tiles; each tile lists the
-    // distinct columns it reads (ucols) and every entry carries its position in
-    // that list (lcol), so the tile's neighbour rows are staged once in LDS.
-    int ntiles;
-    const int32_t* tile_row0;   // [ntiles + 1] first row of each tile
-    const int64_t* tile_u0;     // [ntiles + 1] offset of each tile's column list
-    const int32_t* ucols;       // concatenated column lists
-    const uint16_t* lcol;       // [nnz] tile-local column index
-    int tile_lds_bytes;         // variant 5: LDS bytes for the staged x rows
-    int xcd_rows;               // variants 1/2/8/9: rows per XCD slice (0 = interleave rows over the grid)
-    // variant 12: row groups of kGroupRows rows, merged columns, kGroupRows values per column
-    int ngroups;
-    const int64_t* grp_rp;      // [ngroups + 1]
-    const int32_t* grp_col;     // [entries + kCsrPad]
-    const void* grp_val;        // [entries + kCsrPad][kGroupRows]
-    // variant 13: row tiles of at most chunk_tmax rows and chunk_ecap entries;
-    // workgroup g (logical, XCD-aware) walks tiles [chunk_wg[g], chunk_wg[g+1])
-    const int32_t* chunk_row0;  // [tiles + 1]
-    const int32_t* chunk_wg;    // [grid + 1]
-    int chunk_ecap;
-    int chunk_tmax;
-    int chunk_threads;          // 256, 512 or 1024
+    int variant;    // 0 = lane per vector (any R), 1 = wavefront per row (R >= 64), 14 = vector chunks split
+                    // over the XCDs (R = 16, 32, 64, 128), 15 = two rows per wavefront, 16-byte FP32 gathers
+                    // (complex FP32, R = 64)
+    int xcd_rows;   // variant 1: rows per XCD slice (0 = interleave rows over the grid)
 };
 
 // trailing (column 0, value 0) entries after nnz in the spatially ordered CSR
 constexpr int kCsrPad = 16;
-// rows per row group (variant 12)
-constexpr int kGroupRows = 4;
-
-// LDS budget of one tiled workgroup (two workgroups per CU)
-constexpr int kTileLdsBytes = 72 * 1024;
-constexpr int kTileGroup = 16;     // vectors staged per pass
-constexpr int kTileMaxRows = 64;
-inline int tile_ucap(int vec_type) { return kTileLdsBytes / (kTileGroup * (int)vec_bytes(vec_type)); }
-// variant 5 stages all R vectors of each tile row (one workgroup per CU)
-constexpr int kTileWaveLdsBytes = 144 * 1024;
-inline int tile_wave_ucap(int vec_type, int R) { return kTileWaveLdsBytes / (R * (int)vec_bytes(vec_type)); }
 hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStream_t s);
 
 // ---- SELL-4-sigma step with LDS-staged row tiles (sell.hip) -----------------
