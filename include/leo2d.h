@@ -190,6 +190,22 @@ typedef struct {
 
 /* mu: host double[R][M]. */
 int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu);
+/* Allocation-free variant of leo2d_kpm_moments.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same parameters and output; every device buffer of the recursion (vector
+   blocks, start-vector staging, partials, moments, targets) is carved out of a
+   static device array of 64 MiB that each GPU holds from leo2d_create until the
+   process ends, so no device memory is allocated or freed.  Runs the FP64
+   recursion in the reference row order (natural_order is taken as 1,
+   time_steps as 0; the timing record is left unchanged); the moments equal
+   leo2d_kpm_moments' with natural_order = 1 bit for bit.  Limits:
+   precision != LEO2D_PREC_FP64 -> LEO2D_ERR_ARG; a workspace above 64 MiB
+   (about 2 n B 16 bytes for complex vectors, plus a third block for the
+   reference scheme with caller vectors) -> LEO2D_ERR_ARG with the size in
+   leo2d_last_error.  Calls on one GPU share its array: calls that overlap on
+   one GPU (several threads or handles on one device) race. */
+int leo2d_kpm_moments_noalloc(leo2d_handle* h, const leo2d_kpm_params* p, double* mu);
 int leo2d_last_timing(const leo2d_handle* h, leo2d_timing* t);
 
 /* ---- batched jobs: many small KPM jobs in one recursion -----------------

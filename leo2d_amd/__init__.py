@@ -48,7 +48,7 @@ EXPORTS = [
     "leo2d_geom_arrays", "leo2d_jobs_count", "leo2d_job_get", "leo2d_create", "leo2d_destroy",
     "leo2d_last_error", "leo2d_set_geometry", "leo2d_pairs_build", "leo2d_pairs_copy",
     "leo2d_assemble", "leo2d_csr_upload", "leo2d_csr_info", "leo2d_csr_copy",
-    "leo2d_spectral_bounds", "leo2d_kpm_moments", "leo2d_last_timing", "leo2d_dos",
+    "leo2d_spectral_bounds", "leo2d_kpm_moments", "leo2d_kpm_moments_noalloc", "leo2d_last_timing", "leo2d_dos",
     "leo2d_run_deck", "leo2d_batch_clear", "leo2d_batch_append", "leo2d_batch_info", "leo2d_batch_kpm",
     "leo2d_batch_last_timing", "leo2d_mlp_features", "leo2d_mlp_create", "leo2d_mlp_destroy",
     "leo2d_mlp_last_error", "leo2d_mlp_num_params", "leo2d_mlp_get_params", "leo2d_mlp_set_params",
@@ -185,6 +185,7 @@ def lib() -> C.CDLL:
         L.leo2d_cond_last_timing.argtypes = [vp, C.POINTER(CondTiming)]
         L.leo2d_spectral_bounds.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.leo2d_kpm_moments.argtypes = [vp, C.POINTER(KpmParams), vp]
+        L.leo2d_kpm_moments_noalloc.argtypes = [vp, C.POINTER(KpmParams), vp]
         L.leo2d_last_timing.argtypes = [vp, C.POINTER(Timing)]
         L.leo2d_dos.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, vp, vp]
         L.leo2d_run_deck.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int32]
@@ -404,8 +405,10 @@ class Device:
         return lo.value, hi.value
 
     def kpm(self, M, targets=None, vectors=None, random_phase=None, precision=PREC_FP64,
-            scheme=SCHEME_REFERENCE, block=0, grid=0, time_steps=False, natural_order=False):
-        """Chebyshev moments mu[R][M] on the device Hamiltonian."""
+            scheme=SCHEME_REFERENCE, block=0, grid=0, time_steps=False, natural_order=False, noalloc=False):
+        """Chebyshev moments mu[R][M] on the device Hamiltonian (noalloc: through
+        leo2d_kpm_moments_noalloc, FP64 in the reference row order, workspace in the
+        library's static device scratch)."""
         p = KpmParams()
         p.M = M
         p.precision = precision
@@ -435,7 +438,8 @@ class Device:
             p.start = START_RANDOM_PHASE
             p.seed = seed
         mu = np.zeros((p.R, M))
-        self.last_rc = lib().leo2d_kpm_moments(self.h, C.byref(p), _p(mu))
+        entry = lib().leo2d_kpm_moments_noalloc if noalloc else lib().leo2d_kpm_moments
+        self.last_rc = entry(self.h, C.byref(p), _p(mu))
         self._check(self.last_rc)
         return mu
 

@@ -18,7 +18,7 @@ BIN = os.path.join(HERE, "bin", "leo2d")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LEO2D_ARCH", "gfx950")
 
-SOURCES = ["kpm.hip", "sell.hip", "batch.hip", "mlp.hip", "dos.hip", "cond.hip", "assemble.hip", "capi.cpp", "surrogate.cpp", "deck.cpp", "geom.cpp"]
+SOURCES = ["kpm.hip", "sell.hip", "scratch.hip", "batch.hip", "mlp.hip", "dos.hip", "cond.hip", "assemble.hip", "capi.cpp", "surrogate.cpp", "deck.cpp", "geom.cpp"]
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off",                               # reference operation order, no contraction

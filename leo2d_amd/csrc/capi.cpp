@@ -764,6 +764,12 @@ int leo2d_create(int32_t device, leo2d_handle** out)
         delete h;
         return LEO2D_ERR_HIP;
     }
+    // this device's address of the static scratch (leo2d_kpm_moments_noalloc)
+    if (leo2d::scratch_init(device) != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return LEO2D_ERR_HIP;
+    }
     *out = h;
     return LEO2D_OK;
 }
@@ -1208,25 +1214,68 @@ int leo2d_spectral_bounds(leo2d_handle* h, double* lo, double* hi)
 }
 
 // ---- KPM --------------------------------------------------------------------
+namespace {
+
+// workgroups of a step launch when the caller leaves the grid to the library.
+// R >= 64: four workgroups per CU (one wavefront per row); fewer rows in flight
+// keep each XCD's gathered rows in its L2 (measured best on the 1.05 deg 8x8
+// cell: 1.66 ms per FP64 step at 1024 vs 1.79 ms at 2048)
+int auto_grid(int n, int B)
+{
+    const int rows_per_iter = std::max(1, 256 / B);   // 4 wavefronts x 64/B rows
+    const int want = (n + rows_per_iter - 1) / rows_per_iter;
+    return std::min(B >= 64 ? 1024 : 2048, std::max(8, (want + 7) / 8 * 8));
+}
+
+// moments of a spectrum inside [-1, 1] are finite and satisfy |mu_n| <= mu_0
+int check_moments(leo2d_handle* h, const double* mu, int R, int M)
+{
+    for (int r = 0; r < R; ++r) {
+        const double* m = mu + (size_t)r * M;
+        for (int j = 0; j < M; ++j)
+            if (!std::isfinite(m[j])) {
+                h->err = "non-finite moment (vector " + std::to_string(r) + ", n = " + std::to_string(j) + ")";
+                return LEO2D_ERR_NAN;
+            }
+        const double lim = std::fabs(m[0]) * (1.0 + 1e-6) + 1e-12;
+        for (int j = 0; j < M; ++j)
+            if (std::fabs(m[j]) > lim) {
+                h->err = "|mu_" + std::to_string(j) + "| > mu_0: rescaled spectrum leaves [-1, 1]";
+                return LEO2D_ERR_SPECTRUM;
+            }
+    }
+    return LEO2D_OK;
+}
+
+// the arguments leo2d_kpm_moments and its allocation-free variant check alike
+void check_kpm_params(const leo2d_handle* h, const leo2d_kpm_params* p)
+{
+    if (!h->has_csr) throw StateError("KPM moments before a Hamiltonian is installed");
+    const int M = p->M, R = p->R, n = h->n;
+    if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4 (POLY_ORDER % 4, main.cpp:914)");
+    if (R < 1) throw ArgError("R must be >= 1");
+    if (p->start < 0 || p->start > 2) throw ArgError("unknown start mode");
+    if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
+    if (p->scheme != LEO2D_SCHEME_REFERENCE && p->scheme != LEO2D_SCHEME_DOUBLING) throw ArgError("unknown scheme");
+    if (p->start == LEO2D_START_TARGETS) {
+        if (!p->targets) throw ArgError("targets missing");
+        for (int r = 0; r < R; ++r)
+            if (p->targets[r] < 0 || p->targets[r] >= n) throw ArgError("target index out of range");
+    }
+    if (p->start == LEO2D_START_VECTORS && !p->vectors) throw ArgError("vectors missing");
+    const int B = p->block ? p->block : std::min(64, next_pow2(R));
+    if (B < 1 || B > 128 || (B & (B - 1))) throw ArgError("block must be a power of two <= 128");
+}
+
+}  // namespace
+
 int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
 {
     if (!h || !p || !mu) return LEO2D_ERR_ARG;
     return guarded(h, [&]() {
-        if (!h->has_csr) throw StateError("leo2d_kpm_moments before a Hamiltonian is installed");
+        check_kpm_params(h, p);
         const int M = p->M, R = p->R, n = h->n;
-        if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4 (POLY_ORDER % 4, main.cpp:914)");
-        if (R < 1) throw ArgError("R must be >= 1");
-        if (p->start < 0 || p->start > 2) throw ArgError("unknown start mode");
-        if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
-        if (p->scheme != LEO2D_SCHEME_REFERENCE && p->scheme != LEO2D_SCHEME_DOUBLING) throw ArgError("unknown scheme");
-        if (p->start == LEO2D_START_TARGETS) {
-            if (!p->targets) throw ArgError("targets missing");
-            for (int r = 0; r < R; ++r)
-                if (p->targets[r] < 0 || p->targets[r] >= n) throw ArgError("target index out of range");
-        }
-        if (p->start == LEO2D_START_VECTORS && !p->vectors) throw ArgError("vectors missing");
-        int B = p->block ? p->block : std::min(64, next_pow2(R));
-        if (B < 1 || B > 128 || (B & (B - 1))) throw ArgError("block must be a power of two <= 128");
+        const int B = p->block ? p->block : std::min(64, next_pow2(R));
         const bool mixed = p->precision == LEO2D_PREC_MIXED;
         const bool real_vec = !h->is_complex &&
                               (p->start == LEO2D_START_TARGETS ||
@@ -1273,15 +1322,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         if (p->scheme == LEO2D_SCHEME_DOUBLING) dev_scheme = 1;
         else dev_scheme = p->start == LEO2D_START_TARGETS ? 0 : 2;
         const bool keep_v = dev_scheme == 2;
-        int G = p->grid;
-        if (G <= 0) {
-            // R >= 64: four workgroups per CU (one wavefront per row); fewer rows in
-            // flight keep each XCD's gathered rows in its L2 (measured best on the
-            // 1.05 deg 8x8 cell: 1.66 ms per FP64 step at 1024 vs 1.79 ms at 2048)
-            const int rows_per_iter = std::max(1, 256 / B);   // 4 wavefronts x 64/B rows
-            const int want = (n + rows_per_iter - 1) / rows_per_iter;
-            G = std::min(B >= 64 ? 1024 : 2048, std::max(8, (want + 7) / 8 * 8));
-        }
+        const int G = p->grid > 0 ? p->grid : auto_grid(n, B);
         if (G % 8 != 0) throw ArgError("grid must be a multiple of 8");
         const size_t vb = leo2d::vec_bytes(vec_type);
         DevBuf A, Bf, V, part, dmu, dtg, host_v;
@@ -1438,22 +1479,136 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
         h->timing.M = M;
         h->timing.grid = G;
         h->timing.kernel = kernel_used;
-        // moments of a spectrum inside [-1, 1] satisfy |mu_n| <= mu_0
-        for (int r = 0; r < R; ++r) {
-            const double* m = mu + (size_t)r * M;
-            for (int j = 0; j < M; ++j)
-                if (!std::isfinite(m[j])) {
-                    h->err = "non-finite moment (vector " + std::to_string(r) + ", n = " + std::to_string(j) + ")";
-                    return LEO2D_ERR_NAN;
-                }
-            const double lim = std::fabs(m[0]) * (1.0 + 1e-6) + 1e-12;
-            for (int j = 0; j < M; ++j)
-                if (std::fabs(m[j]) > lim) {
-                    h->err = "|mu_" + std::to_string(j) + "| > mu_0: rescaled spectrum leaves [-1, 1]";
-                    return LEO2D_ERR_SPECTRUM;
-                }
+        return check_moments(h, mu, R, M);
+    });
+}
+
+// The same moments as leo2d_kpm_moments (FP64, reference row order) with every
+// device buffer carved out of the library's static device scratch: no device
+// memory is allocated or freed.  Layout of one block of B vectors, each region
+// 256-byte aligned: T_j and T_{j+1} blocks [n][B], the start vectors (reference
+// scheme with caller vectors), the upload staging of caller vectors [B][w n],
+// the step partials [grid][B][3], the moments [B][M] and the targets [B].
+int leo2d_kpm_moments_noalloc(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
+{
+    if (!h || !p || !mu) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        check_kpm_params(h, p);
+        if (p->precision != LEO2D_PREC_FP64)
+            throw ArgError("leo2d_kpm_moments_noalloc runs the FP64 recursion only (the FP32 matrix copy of the "
+                           "mixed recursion is allocated on first use)");
+        char* base = static_cast<char*>(leo2d::scratch_ptr(h->device));
+        if (!base) throw StateError("static device scratch not initialised on this device");
+        const int M = p->M, R = p->R, n = h->n;
+        const int B = p->block ? p->block : std::min(64, next_pow2(R));
+        const int G = p->grid > 0 ? p->grid : auto_grid(n, B);
+        if (G % 8 != 0) throw ArgError("grid must be a multiple of 8");
+        const bool real_vec = !h->is_complex && (p->start == LEO2D_START_TARGETS ||
+                                                 (p->start == LEO2D_START_VECTORS && !p->vectors_complex));
+        if (real_vec && p->start == LEO2D_START_RANDOM_PHASE) throw ArgError("random phases need complex vectors");
+        const int mat_type = h->is_complex ? leo2d::MAT_CPX_F64 : leo2d::MAT_REAL_F64;
+        const int vec_type = real_vec ? leo2d::VEC_REAL_F64 : leo2d::VEC_CPX_F64;
+        int dev_scheme;
+        if (p->scheme == LEO2D_SCHEME_DOUBLING) dev_scheme = 1;
+        else dev_scheme = p->start == LEO2D_START_TARGETS ? 0 : 2;
+        const bool keep_v = dev_scheme == 2;
+        const int w = p->vectors_complex ? 2 : 1;
+
+        // carve the workspace; reject what does not fit before touching the device
+        const size_t vb = leo2d::vec_bytes(vec_type);
+        const size_t block_bytes = vb * (size_t)n * (size_t)B;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = off;
+            off += (bytes + 255) / 256 * 256;
+            return at;
+        };
+        const size_t oA = take(block_bytes), oB = take(block_bytes);
+        const size_t oV = keep_v ? take(block_bytes) : 0;
+        const size_t oStage = p->start == LEO2D_START_VECTORS ? take(sizeof(double) * (size_t)w * n * B) : 0;
+        const size_t oPart = take(sizeof(double) * 3 * (size_t)G * B);
+        const size_t oMu = take(sizeof(double) * (size_t)M * B);
+        const size_t oTg = take(sizeof(int32_t) * (size_t)B);
+        if (off > leo2d::kScratchBytes)
+            throw ArgError("leo2d_kpm_moments_noalloc needs " + std::to_string(off) + " bytes of workspace for n = " +
+                           std::to_string(n) + ", block = " + std::to_string(B) + ", M = " + std::to_string(M) +
+                           "; the static scratch holds " + std::to_string(leo2d::kScratchBytes) +
+                           " (use leo2d_kpm_moments)");
+        void* A = base + oA;
+        void* Bf = base + oB;
+        void* V = keep_v ? base + oV : nullptr;
+        double* stage = p->start == LEO2D_START_VECTORS ? reinterpret_cast<double*>(base + oStage) : nullptr;
+        double* part = reinterpret_cast<double*>(base + oPart);
+        double* dmu = reinterpret_cast<double*>(base + oMu);
+        int32_t* dtg = reinterpret_cast<int32_t*>(base + oTg);
+
+        // the kernel leo2d_kpm_moments picks for the reference row order in FP64
+        int variant = kpm_variant(false, false, B);
+        if (variant == kVariantSell) variant = B == 16 || B == 32 ? 14 : 1;
+        if (variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) variant = 1;
+        if (variant == 15) variant = 1;
+        if (B < 64 && variant != 14) variant = 0;
+
+        hipStream_t s = h->stream;
+        for (int b0 = 0; b0 < R; b0 += B) {
+            const int Rb = std::min(B, R - b0);
+            check(leo2d::launch_zero_f64(dmu, (int64_t)M * B, s), "moments");
+            if (p->start == LEO2D_START_TARGETS) {
+                int32_t tg[128];
+                for (int r = 0; r < B; ++r) tg[r] = p->targets[b0 + std::min(r, Rb - 1)];
+                check(hipMemcpyAsync(dtg, tg, sizeof(int32_t) * B, hipMemcpyHostToDevice, s), "targets");
+                check(hipStreamSynchronize(s), "targets");
+                check(leo2d::launch_init_targets(A, vec_type, n, B, dtg, s), "init");
+            } else if (p->start == LEO2D_START_RANDOM_PHASE) {
+                check(leo2d::launch_init_random_phase_perm(A, vec_type, n, B, p->seed, b0, nullptr, s), "init");
+            } else {
+                const size_t per = (size_t)w * n;
+                check(hipMemcpyAsync(stage, p->vectors + per * b0, sizeof(double) * per * Rb, hipMemcpyHostToDevice, s),
+                      "upload");
+                check(leo2d::launch_zero_f64(stage + per * Rb, (int64_t)(per * (B - Rb)), s), "upload");
+                check(leo2d::launch_init_from_host_layout_perm(A, vec_type, stage, p->vectors_complex, n, B, nullptr, s),
+                      "init");
+            }
+            if (keep_v) check(hipMemcpyAsync(V, A, block_bytes, hipMemcpyDeviceToDevice, s), "copy start");
+            leo2d::StepArgs sa{};
+            sa.row_ptr = (const int64_t*)h->rp.p;
+            sa.col = (const int32_t*)h->col.p;
+            sa.val = h->val.p;
+            sa.n = n;
+            sa.R = B;
+            sa.part = part;
+            sa.grid = G;
+            sa.v = V;
+            sa.variant = variant;
+            sa.xcd_rows = kpm_sched() == 1 ? (n + 7) / 8 : 0;
+            leo2d::ReduceArgs ra{};
+            ra.part = part;
+            ra.grid = G;
+            ra.R = B;
+            ra.M = M;
+            ra.scheme = dev_scheme;
+            ra.targets = dtg;
+            ra.vec_type = vec_type;
+            ra.mu = dmu;
+            for (int j = 0; j < M / 2; ++j) {
+                void* cur = (j % 2 == 0) ? A : Bf;
+                void* other = (j % 2 == 0) ? Bf : A;
+                sa.cur = cur;
+                sa.prev = other;
+                sa.out = other;
+                sa.alpha = j == 0 ? 1.0 : 2.0;
+                sa.beta = j == 0 ? 0 : 1;
+                check(leo2d::launch_kpm_step(mat_type, vec_type, sa, s), "kpm step");
+                ra.j = j;
+                ra.cur = cur;
+                ra.out = other;
+                check(leo2d::launch_kpm_reduce(ra, s), "kpm reduce");
+            }
+            check(hipMemcpyAsync(mu + (size_t)M * b0, dmu, sizeof(double) * (size_t)M * Rb, hipMemcpyDeviceToHost, s),
+                  "moments");
+            check(hipStreamSynchronize(s), "kpm recursion");
         }
-        return LEO2D_OK;
+        return check_moments(h, mu, R, M);
     });
 }
 

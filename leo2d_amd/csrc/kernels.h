@@ -167,6 +167,14 @@ hipError_t launch_dct3_matrix(int M, double* P, hipStream_t s);
 hipError_t launch_dos(const double* mu, int T, int M, double a, double b, double scale, double* E, double* rho,
                       hipStream_t s);
 
+// Static device workspace of the allocation-free KPM entry point (scratch.hip).
+// scratch_init(d) looks up device d's address of the array (d must be current);
+// scratch_ptr(d) returns it (nullptr before scratch_init succeeded).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+hipError_t scratch_init(int device);
+void* scratch_ptr(int device);
+hipError_t launch_zero_f64(double* p, int64_t count, hipStream_t s);
+
 // Gershgorin interval of a CSR matrix: lo = min_i (Re H_ii - R_i), hi = max_i (Re H_ii + R_i)
 hipError_t launch_gershgorin(const int64_t* rp, const int32_t* col, const void* val, int is_complex, int n,
                              double* lohi /*[2*blocks]*/, int blocks, hipStream_t s);
