@@ -18,7 +18,7 @@ BIN = os.path.join(HERE, "bin", "leo2d")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LEO2D_ARCH", "gfx950")
 
-SOURCES = ["kpm.hip", "sell.hip", "scratch.hip", "batch.hip", "mlp.hip", "dos.hip", "cond.hip", "assemble.hip", "capi.cpp", "surrogate.cpp", "deck.cpp", "geom.cpp"]
+SOURCES = ["kpm.hip", "sell.hip", "scratch.hip", "dgemm.hip", "batch.hip", "mlp.hip", "dos.hip", "cond.hip", "assemble.hip", "capi.cpp", "surrogate.cpp", "deck.cpp", "geom.cpp"]
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off",                               # reference operation order, no contraction
@@ -45,7 +45,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    r = subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lrocblas"], capture_output=True,
+    r = subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs], capture_output=True,
                        text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")

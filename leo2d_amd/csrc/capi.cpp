@@ -16,7 +16,6 @@
 #include <string>
 #include <vector>
 
-#include <rocblas/rocblas.h>
 
 #include "assemble.h"
 #include "devbuf.h"
@@ -119,7 +118,6 @@ struct leo2d_handle {
     // Fourier relaxation field (leo2d_set_relaxation) and the per-orbital local twist
     DevBuf relax, ltheta;
     int relax_max_k = -1;
-    rocblas_handle blas = nullptr;   // created on the first conductivity call, bound to the stream
     struct CondTiming {
         double recursion_ms = 0.0, gemm_ms = 0.0;
         int32_t block = 0;
@@ -206,6 +204,8 @@ int guarded(leo2d_handle* h, F&& f)
     try {
         if (h) {
             check(hipSetDevice(h->device), "hipSetDevice");
+            // launch checks read hipGetLastError: start from a clean slate
+            (void)hipGetLastError();
         }
         return f();
     } catch (const HipError& e) {
@@ -614,24 +614,6 @@ int batch_default_grid(int ntiles)
 
 }  // namespace
 
-// ---- KPM conductivity (observable type 1) -------------------------------------
-namespace {
-void blas_check(rocblas_status st, const char* what)
-{
-    if (st != rocblas_status_success)
-        throw std::runtime_error(std::string(what) + ": rocBLAS status " + std::to_string((int)st));
-}
-
-rocblas_handle blas_of(leo2d_handle* h)
-{
-    if (!h->blas) {
-        blas_check(rocblas_create_handle(&h->blas), "rocblas_create_handle");
-        blas_check(rocblas_set_stream(h->blas, h->stream), "rocblas_set_stream");
-    }
-    return h->blas;
-}
-}  // namespace
-
 extern "C" {
 
 // ---- deck -------------------------------------------------------------------
@@ -779,7 +761,6 @@ void leo2d_destroy(leo2d_handle* h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->blas) (void)rocblas_destroy_handle(h->blas);
     delete h;
 }
 
@@ -2152,10 +2133,12 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
         dalloc(dtg, sizeof(int32_t) * B, "targets");
         dalloc(Cdev, sizeof(double) * (size_t)B * M * M, "conductivity moments");
         // split the n-long GEMM reduction into S row chunks so that the M x M outputs of a pass
-        // give the GPU enough tiles (about 2048 of 128 x 128); partial products summed in order
+        // fill every CU twice over with 128 x 128 tiles; partial products summed in chunk order
+        int dev_id = 0, cus = 256;
+        if (hipGetDevice(&dev_id) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
         const int tiles = std::max(1, ((M + 127) / 128) * ((M + 127) / 128) * B);
-        const int S = std::max(1, std::min({64, 2048 / tiles, n / 1024}));
-        const int64_t L = n / S;                 // rows per chunk; the last chunk takes the rest
+        const int S = std::max(1, std::min({64, (2 * cus + tiles - 1) / tiles, n / 1024}));
         DevBuf Cpart;
         if (S > 1) dalloc(Cpart, sizeof(double) * (size_t)B * S * M * M, "conductivity partials");
         auto scatter = [&](const void* blk, int m, DevBuf& set) {
@@ -2165,7 +2148,6 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
         const void* dX = h->val_dx.p;
         hipEvent_t e0 = event_at(h, 0), e1 = event_at(h, 1), e2 = event_at(h, 2);
         float rec_ms = 0.f, gemm_ms = 0.f;
-        rocblas_handle bl = blas_of(h);
         for (int b0 = 0; b0 < R; b0 += B) {
             const int Bb = std::min(B, R - b0);
             std::vector<int32_t> tg(B);
@@ -2201,35 +2183,31 @@ int leo2d_cond_moments(leo2d_handle* h, const leo2d_cond_params* p, double* C)
             }
             dual(nullptr, dX, cur, nullptr, nullptr, 1.0, 0, M - 1, nullptr, &Dset);
             check(hipEventRecord(e1, s), "event");
-            // C_b[j][m] = sum_i D_b[j][i] Bset_b[m][i]: column-major (M x M) = Bset_b^T (M x n) Dset_b (n x M)
-            const double one = 1.0, zero = 0.0;
+            // C_b[j][m] = sum_i D_b[j][i] Bset_b[m][i], stored column-major (m + j M): the GEMM's
+            // rows are j (A = D_b, k = i contiguous), its columns m (B = Bset_b), so a wavefront's
+            // 16 output columns are 16 consecutive doubles
             const size_t mm = (size_t)M * M;
-            if (S == 1) {
-                blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M, M, n,
-                                                         &one, (const double*)Bset.p, n, (rocblas_stride)n * M,
-                                                         (const double*)Dset.p, n, (rocblas_stride)n * M, &zero,
-                                                         (double*)Cdev.p, M, (rocblas_stride)mm, Bb),
-                           "conductivity GEMM");
-            } else {
-                for (int b = 0; b < Bb; ++b) {
-                    const double* Bb_ = (const double*)Bset.p + (size_t)b * n * M;
-                    const double* Db_ = (const double*)Dset.p + (size_t)b * n * M;
-                    double* Cp = (double*)Cpart.p + (size_t)b * S * mm;
-                    // chunks 0 .. S-2 of L rows, then the last chunk of n - (S-1) L rows
-                    if (S > 1)
-                        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_transpose, rocblas_operation_none, M,
-                                                                 M, (rocblas_int)L, &one, Bb_, n, (rocblas_stride)L, Db_,
-                                                                 n, (rocblas_stride)L, &zero, Cp, M, (rocblas_stride)mm,
-                                                                 S - 1),
-                                   "conductivity GEMM");
-                    const int64_t r0 = (int64_t)(S - 1) * L;
-                    blas_check(rocblas_dgemm(bl, rocblas_operation_transpose, rocblas_operation_none, M, M,
-                                             (rocblas_int)(n - r0), &one, Bb_ + r0, n, Db_ + r0, n, &zero,
-                                             Cp + (size_t)(S - 1) * mm, M),
-                               "conductivity GEMM");
-                }
-                check(leo2d::launch_cond_chunk_sum((const double*)Cpart.p, S, mm, Bb, (double*)Cdev.p, s), "chunk sum");
-            }
+            leo2d::DgemmArgs ga{};
+            ga.M = M;
+            ga.N = M;
+            ga.K = n;
+            ga.batch = Bb;
+            ga.splits = S;
+            ga.A = (const double*)Dset.p;
+            ga.sam = n;
+            ga.sak = 1;
+            ga.sab = (int64_t)n * M;
+            ga.B = (const double*)Bset.p;
+            ga.sbn = n;
+            ga.sbk = 1;
+            ga.sbb = (int64_t)n * M;
+            ga.C = S == 1 ? (double*)Cdev.p : (double*)Cpart.p;
+            ga.scm = M;
+            ga.scn = 1;
+            ga.scs = (int64_t)mm;
+            ga.scb = (int64_t)(S * mm);
+            check(leo2d::launch_dgemm(ga, s), "conductivity GEMM");
+            if (S > 1) check(leo2d::launch_cond_chunk_sum((const double*)Cpart.p, S, mm, Bb, (double*)Cdev.p, s), "chunk sum");
             check(hipEventRecord(e2, s), "event");
             check(hipMemcpyAsync(C + (size_t)b0 * M * M, Cdev.p, sizeof(double) * (size_t)Bb * M * M,
                                  hipMemcpyDeviceToHost, s),
@@ -2265,17 +2243,34 @@ int leo2d_cond_transform(leo2d_handle* h, int32_t M, int32_t T, const double* C,
         check(leo2d::launch_cond_weight((const double*)Cd.p, M, T, (double*)X.p, s), "weights");
         check(leo2d::launch_dct3_matrix(M, (double*)P.p, s), "DCT-III matrix");
         // with the row-major X read as column-major X^T, Z = P X^T P^T (column-major)
-        // is the row-major 2D transform P X P^T
-        rocblas_handle bl = blas_of(h);
-        const double one = 1.0, zero = 0.0;
-        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_none, rocblas_operation_none, M, M, M, &one,
-                                                 (const double*)P.p, M, 0, (const double*)X.p, M, (rocblas_stride)mm,
-                                                 &zero, (double*)Yb.p, M, (rocblas_stride)mm, T),
-                   "transform GEMM");
-        blas_check(rocblas_dgemm_strided_batched(bl, rocblas_operation_none, rocblas_operation_transpose, M, M, M, &one,
-                                                 (const double*)Yb.p, M, (rocblas_stride)mm, (const double*)P.p, M, 0,
-                                                 &zero, (double*)X.p, M, (rocblas_stride)mm, T),
-                   "transform GEMM");
+        // is the row-major 2D transform P X P^T: Y_t = P X_t, then X_t = Y_t P^T
+        leo2d::DgemmArgs ga{};
+        ga.M = M;
+        ga.N = M;
+        ga.K = M;
+        ga.batch = T;
+        ga.splits = 1;
+        ga.A = (const double*)P.p;   // P(m, k) = P[m + k M]
+        ga.sam = 1;
+        ga.sak = M;
+        ga.sab = 0;
+        ga.B = (const double*)X.p;   // X_t(k, n) = X[t mm + k + n M]
+        ga.sbn = M;
+        ga.sbk = 1;
+        ga.sbb = (int64_t)mm;
+        ga.C = (double*)Yb.p;
+        ga.scm = 1;
+        ga.scn = M;
+        ga.scb = (int64_t)mm;
+        check(leo2d::launch_dgemm(ga, s), "transform GEMM");
+        ga.A = (const double*)Yb.p;  // Y_t(m, k) = Yb[t mm + m + k M]
+        ga.sab = (int64_t)mm;
+        ga.B = (const double*)P.p;   // P^T(k, n) = P[n + k M]
+        ga.sbn = 1;
+        ga.sbk = M;
+        ga.sbb = 0;
+        ga.C = (double*)X.p;
+        check(leo2d::launch_dgemm(ga, s), "transform GEMM");
         check(hipMemcpyAsync(out, X.p, sizeof(double) * mm * T, hipMemcpyDeviceToHost, s), "download");
         check(hipStreamSynchronize(s), "transform");
         return LEO2D_OK;

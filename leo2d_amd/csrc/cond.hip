@@ -7,7 +7,7 @@
 // product at a time: M^2 vector dots per target.  Here both vector sets are
 // produced for a block of targets at once, one kernel per Chebyshev index
 // writing straight into the [target][m][row] sets; C for every target is then
-// an FP64 GEMM (rocBLAS, MFMA) split into row chunks, and the 2D
+// an FP64 GEMM on the matrix cores (dgemm.hip) split into row chunks, and the 2D
 // Jackson/DCT-III transform is two more GEMMs against the cosine matrix.
 #include <hip/hip_runtime.h>
 

@@ -60,7 +60,14 @@ inline void dalloc(DevBuf& b, size_t bytes, const char* what)
 {
     b.release();
     if (bytes == 0) bytes = 16;
-    check(hipMalloc(&b.p, bytes), what);
+    const hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        // a failed allocation is not sticky, but it stays the runtime's last error:
+        // clear it, or the next launch check (hipGetLastError) would report it
+        (void)hipGetLastError();
+        b.p = nullptr;
+        check(e, what);
+    }
     b.bytes = bytes;
 }
 

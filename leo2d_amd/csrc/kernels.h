@@ -163,6 +163,25 @@ hipError_t launch_cond_weight(const double* C, int M, int T, double* X, hipStrea
 hipError_t launch_cond_chunk_sum(const double* part, int S, size_t mm, int B, double* C, hipStream_t s);
 hipError_t launch_dct3_matrix(int M, double* P, hipStream_t s);
 
+// FP64 GEMM on the matrix cores (dgemm.hip), batched and split along k:
+//   C[b][s](m, n) = sum_{k in split s} A_b(m, k) B_b(k, n)
+//   A_b(m, k) = A[b sab + m sam + k sak], B_b(k, n) = B[b sbb + n sbn + k sbk],
+//   C[b][s](m, n) at C[b scb + s scs + m scm + n scn]
+// split s covers k in [s kc, (s+1) kc), kc = ceil(K / splits) rounded up to 16
+struct DgemmArgs {
+    int M, N, K;
+    int batch;
+    int splits;
+    int kchunk;     // set by the launcher
+    const double* A;
+    int64_t sam, sak, sab;
+    const double* B;
+    int64_t sbn, sbk, sbb;
+    double* C;
+    int64_t scm, scn, scb, scs;
+};
+hipError_t launch_dgemm(const DgemmArgs& a, hipStream_t s);
+
 // Jackson-kernel DoS: rho[t][k] = DCT-III(g * mu[t]) / (pi sqrt(a^2 - (E_k - b)^2)) * scale
 hipError_t launch_dos(const double* mu, int T, int M, double a, double b, double scale, double* E, double* rho,
                       hipStream_t s);

@@ -4,7 +4,7 @@ The oracle restates computeCondKPM (locality.cpp:6348-6727) and
 matrixResponseTransform (param_tools.cpp:838-905) and is pinned by a dense
 eigendecomposition and scipy's DCT-III (tests/test_oracle_pins.py).  Bars:
 the velocity matrix's pattern bit-exact and its values within 1e-12 of the
-largest; moments within 1e-12 of the largest (FP64; the GEMM sums rows in
+largest; moments within 1e-12 of the largest (FP64; the matrix-core GEMM sums rows in
 another order than the reference's dot loop); the transform within 1e-12 of
 its largest value.
 """
@@ -124,6 +124,21 @@ def test_cond_split_reduction(dev, oracle_mod):
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
 
 
+def test_cond_moments_multi_tile(dev, oracle_mod):
+    """M = 160: two 128-wide output tiles per direction, the second partial, and
+    the 3068-long reduction split into two row chunks (the second ending inside
+    a 16-deep k tile)."""
+    text = W.tblg_flake_deck(2.0, 36, M=16)
+    (orp, oci, ov, odx, _), tl = _oracle_setup(oracle_mod, text, 1)
+    geom, _ = _device_setup(dev, text, L.PATTERN_PRUNED)
+    assert geom.n == 3068
+    M = 160
+    targets = list(tl)[:2]
+    want = oracle_mod.cond(orp, oci, ov, odx, M, targets)
+    got = dev.cond_moments(M, targets)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
 def test_cond_periodic_cell(dev, oracle_mod):
     text = W.tblg_cell_deck(3, 2, M=16)
     (orp, oci, ov, odx, _), tl = _oracle_setup(oracle_mod, text, 1)
@@ -134,12 +149,13 @@ def test_cond_periodic_cell(dev, oracle_mod):
     assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
 
 
-@pytest.mark.parametrize("M", [16, 33, 128])
+@pytest.mark.parametrize("M", [16, 33, 128, 200])
 def test_cond_transform_matches_oracle(dev, oracle_mod, M):
     rng = np.random.default_rng(M)
-    Cm = rng.standard_normal((2, M, M))
+    T = 3 if M == 200 else 2
+    Cm = rng.standard_normal((T, M, M))
     got = dev.cond_transform(Cm)
-    for t in range(2):
+    for t in range(T):
         want = oracle_mod.cond_transform(Cm[t])
         assert np.max(np.abs(got[t] - want)) <= 1e-12 * np.max(np.abs(want)), (M, t)
 

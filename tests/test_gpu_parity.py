@@ -307,6 +307,22 @@ def test_spectrum_and_nan_errors(dev):
     assert e.value.code == L.ERR_ARG
 
 
+def test_oversized_vector_block_is_alloc_error(dev):
+    """A vector block larger than the GPU's memory: 128 complex FP64 vectors on
+    a 150 M-row diagonal H are 307 GB per block (288 GB of HBM).  The library
+    returns LEO2D_ERR_ALLOC (not a crash or a hang) and the handle keeps
+    working afterwards."""
+    n = 150_000_000
+    dev.csr_upload(np.arange(n + 1, dtype=np.int64), np.arange(n, dtype=np.int32), np.full(n, 0.25))
+    with pytest.raises(L.Leo2dError) as e:
+        dev.kpm(8, random_phase=(128, 1), block=128, natural_order=True)
+    assert e.value.code == L.ERR_ALLOC, str(e.value)
+    m = 64
+    dev.csr_upload(np.arange(m + 1, dtype=np.int64), np.arange(m, dtype=np.int32), np.full(m, 0.25))
+    mu = dev.kpm(8, random_phase=(4, 1), scheme=L.SCHEME_DOUBLING)
+    assert np.allclose(mu, np.cos(np.arange(8) * np.arccos(0.25))[None, :] * m, rtol=1e-12, atol=1e-12 * m)
+
+
 @pytest.mark.parametrize("name", ["tblg_c0_twin.in", "tblg_sc_7_6_k.in"])
 def test_spatial_order_matches_natural_order(dev, name):
     """The recursion runs on P H P^T (Morton order) by default; moments must
