@@ -172,7 +172,23 @@ __device__ inline void run_blocks(AT (&acc)[VPL], const GroupRegs<HT, QS>& g, co
     }
 }
 
-__device__ inline void wait_vm0() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// s_waitcnt vmcnt(N) through the builtin, so the compiler's own wait insertion
+// knows what has completed (expcnt and lgkmcnt left at their maxima, 7 and 15)
+template <int N>
+__device__ inline void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// per-wavefront-uniform metadata read through the constant address space: scalar
+// loads, counted by lgkmcnt, so waiting for them never waits for the staging
+// copies or the stores in flight (the arrays are not written during a step)
+template <typename T>
+__device__ inline T ldu(const T* p, int64_t i)
+{
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
 
 // Pipeline over (tile, chunk) pairs p of one workgroup, two LDS buffers:
 //   top of pair p: wait for everything issued during pair p-1, barrier;
@@ -196,6 +212,10 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     static_assert(UC % (4 * NW) == 0, "LDS rows must split evenly over the wavefronts");
     __shared__ __align__(16) XT xs0[UC * C];
     __shared__ __align__(16) XT xs1[UC * C];
+    // T_{j-1} of each wavefront's group rows, copied straight into LDS (lane l of
+    // group j at slot j, bytes 16 l .. 16 l + 15: row s = l / 16, vectors of lane k = l % 16)
+    constexpr bool TDMA = !DIRECT;
+    __shared__ __align__(16) XT tp[TDMA ? NW * GPW * 64 * VPL : 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: group loops branch on scalars
     const int s = lane >> 4;
@@ -248,13 +268,13 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
 
     // entries, rows and lengths of this wavefront's groups of tile t
     auto load_groups = [&](int t, GroupRegs<HT, QS> (&gr)[GPW], int (&grow)[GPW], int (&glen)[GPW]) {
-        const int g0 = a.tile_g0[t], g1 = a.tile_g0[t + 1];
+        const int g0 = ldu(a.tile_g0, t), g1 = ldu(a.tile_g0, t + 1);
 #pragma unroll
         for (int j = 0; j < GPW; ++j) {
             const int g = g0 + wave + NW * j;
             const bool live = g < g1;
-            const int L = live ? a.glen[g] : 0;
-            const int64_t go = live ? a.goff[g] : 0;
+            const int L = live ? ldu(a.glen, g) : 0;
+            const int64_t go = live ? ldu(a.goff, g) : 0;
             glen[j] = L;
             grow[j] = live ? a.grow[4 * g + s] : -1;
 #pragma unroll
@@ -269,8 +289,8 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     };
     // union rows this lane copies for tile t (a valid dummy row past the union)
     auto load_ucols = [&](int t, int (&uc)[SPW]) {
-        const int64_t u0 = a.tile_u0[t];
-        const int U = (int)(a.tile_u0[t + 1] - u0);
+        const int64_t u0 = ldu(a.tile_u0, t);
+        const int U = (int)(ldu(a.tile_u0, t + 1) - u0);
 #pragma unroll
         for (int m = 0; m < SPW; ++m) {
             const int u = 4 * (wave + NW * m) + s;
@@ -294,14 +314,14 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     int r0 = 0, r0n = 0;
     if (live(t)) {
         load_groups(t, ec, gc_row, gc_len);
-        r0 = a.tile_row0[t];
+        r0 = ldu(a.tile_row0, t);
         load_ucols(t, ucs);
         stage(xs0, 0, ucs);
         if (NCH == 1) load_ucols(live(t + t_stride) ? t + t_stride : t, ucs);
     }
 
     auto pair = [&](XT* buf, XT* nbuf) {
-        wait_vm0();
+        wait_vm<0>();
         __syncthreads();
         if (c == 0 && p > 0) {
 #pragma unroll
@@ -312,17 +332,28 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
             }
             r0 = r0n;
         }
-        // T_{j-1} and v of this pair's rows
+        // T_{j-1} (and v) of this pair's rows.  Copied into LDS, T_{j-1} is waited for
+        // with a count that leaves the staging copies issued after it in flight; a
+        // register load would be waited for together with them.
         XT pv[GPW][VPL], vv[GPW][VPL];
+        if constexpr (TDMA) {
 #pragma unroll
-        for (int j = 0; j < GPW; ++j) {
-            const int row = gc_row[j];
-            const size_t idx = (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+            for (int j = 0; j < GPW; ++j) {
+                const int row = gc_row[j];
+                const XT* src = prev + (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+                __builtin_amdgcn_global_load_lds((const void*)src,
+                                                 (__attribute__((address_space(3))) void*)(tp + (wave * GPW + j) * 64 * VPL),
+                                                 16, 0, 0);
+            }
+        } else {
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                if (beta && row >= 0) pv[j][v] = prev[idx + v];
-                else zero(pv[j][v]);
-                if (DIRECT) {
+            for (int j = 0; j < GPW; ++j) {
+                const int row = gc_row[j];
+                const size_t idx = (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    if (beta && row >= 0) pv[j][v] = prev[idx + v];
+                    else zero(pv[j][v]);
                     if (row >= 0) vv[j][v] = vin[idx + v];
                     else zero(vv[j][v]);
                 }
@@ -349,6 +380,18 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
                 XT xa[4][VPL];
                 ld_block<0, HT, XT, VPL, QS>(xa, ec[j], buf, koff);
                 run_blocks<0, HT, XT, AT, VPL, QS>(acc[j], ec[j], buf, koff, L, xa);
+            }
+        }
+        if constexpr (TDMA) {
+            // everything but the SPW staging copies issued after T_{j-1} has landed
+            wait_vm<SPW>();
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                lds_ld<XT, VPL>(pv[j], tp + (wave * GPW + j) * 64 * VPL + lane * VPL);
+                if (!beta || gc_row[j] < 0) {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) zero(pv[j][v]);
+                }
             }
         }
         // outputs and moment terms, then the stores
@@ -393,7 +436,7 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
         if (c2 == 0) load_ucols(live(t2) ? t2 : t, ucs);
         if (c1 == 0 && has1) {
             load_groups(t1, en, gn_row, gn_len);
-            r0n = a.tile_row0[t1];
+            r0n = ldu(a.tile_row0, t1);
         }
         t = t1;
         c = c1;
@@ -404,7 +447,7 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
         if (!live(t)) break;
         pair(xs1, xs0);
     }
-    wait_vm0();
+    wait_vm<0>();
     __syncthreads();
 
     // fold the four DPP rows (same vectors), then the wavefronts
