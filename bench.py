@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--c4-theta", type=float, default=0.3, help="c4: inference flake twist (deg)")
     ap.add_argument("--c4-fallback-sample", type=int, default=32,
                     help="c4: flagged sites computed exactly to score the networks and time the fallback")
+    ap.add_argument("--mixed-error-Ms", type=str, default="",
+                    help="c1 only: instead of the bench line, print max|dDoS| of the FP32 recursion against FP64 "
+                         "for each of these comma-separated moment counts (e.g. 1024,2048,4096,8192)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -167,6 +170,11 @@ def main():
     n = geom.n
     log(f"assembly: nnz={nnz} ({nnz / n:.2f}/row), Gershgorin [{lo:.4f}, {hi:.4f}] eV, "
         f"a={a_res:.4f} b={b_cen:.4f}, fill {t_fill2 * 1e3:.1f} ms")
+
+    if args.mixed_error_Ms:
+        mixed_error_study(dev, n, nnz, R, a_res, b_cen, [int(x) for x in args.mixed_error_Ms.split(",")], log)
+        dev.close()
+        return
 
     def run(precision, steps, warmup, time_steps):
         for _ in range(warmup):
@@ -245,6 +253,38 @@ def main():
 
     dev.close()
     print(json.dumps(out), flush=True)
+
+
+def mixed_error_study(dev, n, nnz, R, a_res, b_cen, Ms, log):
+    """DoS difference of the FP32 recursion against FP64 on the same job, random
+    phases and seed, for several moment counts M: the largest over all M
+    energies, its position x = (E - b)/a on the rescaled axis, the largest over
+    |x| <= 0.99 and over |x| <= 0.9, and the moment difference relative to mu_0."""
+    import leo2d_amd as L
+
+    rows = []
+    for M in Ms:
+        t0 = time.perf_counter()
+        mu64 = dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING)
+        t64 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mu32 = dev.kpm(M, random_phase=(R, 0), scheme=L.SCHEME_DOUBLING, precision=L.PREC_MIXED)
+        t32 = time.perf_counter() - t0
+        E, r64 = dev.dos(mu64.mean(axis=0) / n, a_res, b_cen)
+        _, r32 = dev.dos(mu32.mean(axis=0) / n, a_res, b_cen)
+        r64, r32 = r64[0], r32[0]
+        x = (E - b_cen) / a_res
+        d = np.abs(r32 - r64)
+        k = int(np.argmax(d))
+        row = {"M": M, "max_dDoS_all": float(d.max()), "x_at_max": float(x[k]),
+               "max_dDoS_abs_x_le_0.99": float(d[np.abs(x) <= 0.99].max()),
+               "max_dDoS_abs_x_le_0.9": float(d[np.abs(x) <= 0.9].max()),
+               "max_dmu_over_mu0": float(np.max(np.abs(mu32 - mu64)) / n),
+               "fp64_s": t64, "mixed_s": t32, "speedup": t64 / t32}
+        log(json.dumps(row))
+        rows.append(row)
+    print(json.dumps({"mixed_error_vs_M": rows, "orbitals": n, "nnz": nnz, "R": R,
+                      "units": "states/eV/atom"}), flush=True)
 
 
 def run_c2(args):
