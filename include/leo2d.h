@@ -230,6 +230,9 @@ int leo2d_batch_info(const leo2d_handle* h, int32_t* jobs, int64_t* rows, int64_
 #define LEO2D_BATCH_ROW_GROUPS 0    /* 64/R rows per wavefront */
 #define LEO2D_BATCH_ENTRY_SLOTS 1   /* one row per wavefront, 64/R entries in flight */
 #define LEO2D_BATCH_ROW_CHUNKS 2    /* R = 16, 32: 64/R rows per wavefront, 16-entry chunks broadcast in 16-lane rows */
+#define LEO2D_BATCH_SELL 3          /* complex vectors, R = 16 or 32 (FP32: 32): SELL-4-sigma row tiles cut at job
+                                       boundaries, their neighbour rows staged in LDS, one workgroup per CU
+                                       (tile_rows and grid are chosen by the library) */
 
 typedef struct {
     int32_t M;            /* moments per vector, multiple of 4 */

@@ -40,6 +40,7 @@ SCHEME_DOUBLING = 1
 BATCH_ROW_GROUPS = 0
 BATCH_ENTRY_SLOTS = 1
 BATCH_ROW_CHUNKS = 2
+BATCH_SELL = 3
 
 # every symbol include/leo2d.h declares
 EXPORTS = [

@@ -89,6 +89,18 @@ int next_pow2(int x)
     return p;
 }
 
+// SELL-4-sigma row tiles of a matrix for the LDS-staged step (sell.hip)
+struct SellTiles {
+    bool valid = false, failed = false, spatial = false;
+    int ucap = 0, tmax = 0, ntiles = 0, ngroups = 0, tile_rows = 0, max_len = 0;
+    int64_t entries = 0;
+    double build_ms = 0.0;
+    DevBuf row0, u0, ucols, g0, grow, glen, goff, eu, map, val, val32;
+    bool val_valid = false, val32_valid = false;
+    std::vector<int32_t> job_tile0;   // first tile of each job when cut at job boundaries
+    DevBuf djob_tile0;
+};
+
 }  // namespace
 
 struct leo2d_handle {
@@ -130,14 +142,7 @@ struct leo2d_handle {
     DevBuf prp, pcol_k, pval, pval32;
     // SELL-4-sigma row tiles of the recursion matrix for the LDS-staged step
     // (sell.hip): pattern built on the host on first use, values gathered on the device
-    struct Sell {
-        bool valid = false, failed = false, spatial = false;
-        int ucap = 0, tmax = 0, ntiles = 0, ngroups = 0, tile_rows = 0, max_len = 0;
-        int64_t entries = 0;
-        double build_ms = 0.0;
-        DevBuf row0, u0, ucols, g0, grow, glen, goff, eu, map, val, val32;
-        bool val_valid = false, val32_valid = false;
-    } sell;
+    SellTiles sell;
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
     // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
@@ -155,6 +160,7 @@ struct leo2d_handle {
         bool val32_valid = false;
         int tile_rows = 0, ntiles = 0;          // current tiling
         std::vector<int32_t> tile_row0, job_tile0;
+        SellTiles sell;                         // SELL tiles of the stack, cut at job boundaries
         DevBuf dtile_row0, djob_tile0;
         // autotuned launch shape, per (R, precision, vector element type) of the current stack
         struct Tuned {
@@ -328,19 +334,23 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
 // stored entry's position in the CSR (-1 for padding) so the values can be
 // gathered on the device for every matrix with this pattern.  Returns false
 // when a row has more than kSellMaxLen entries or does not fit `ucap` alone.
-bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, int tmax, bool spatial)
+// With `breaks` (ascending first rows of stacked jobs, ending with n) no tile
+// crosses a job boundary, and job_tile0 lists each job's first tile.
+bool build_sell_tiles(SellTiles& S, hipStream_t s, int n, int64_t nnz, const int64_t* drp, const int32_t* dcol, int ucap,
+                      int tmax, bool spatial, const std::vector<int64_t>* breaks)
 {
-    auto& S = h->sell;
     if (S.valid && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return true;
     if (S.failed && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return false;
     const auto t0 = std::chrono::steady_clock::now();
-    const int n = h->n;
-    hipStream_t s = h->stream;
     std::vector<int64_t> rp(n + 1);
-    std::vector<int32_t> col((size_t)h->nnz);
+    std::vector<int32_t> col((size_t)nnz);
     check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "sell");
-    if (h->nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost, s), "sell");
+    if (nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s), "sell");
     check(hipStreamSynchronize(s), "sell");
+    std::vector<uint8_t> brk(n + 1, 0);
+    if (breaks)
+        for (int64_t b : *breaks)
+            if (b >= 0 && b <= n) brk[b] = 1;
     S.ucap = ucap;
     S.tmax = tmax;
     S.spatial = spatial;
@@ -357,7 +367,7 @@ bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int uc
     while (r < n) {
         const int start = r;
         int size = 0;   // distinct columns of the tile so far (own rows included)
-        while (r < n && r - start < tmax) {
+        while (r < n && r - start < tmax && !(r > start && brk[r])) {
             if (rp[r + 1] - rp[r] > leo2d::kSellMaxLen) return fail();
             added.clear();
             auto take = [&](int c) {
@@ -443,6 +453,12 @@ bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int uc
     to_device(S.goff, goff.data(), sizeof(int64_t) * goff.size(), s, "sell groups");
     to_device(S.eu, eu.data(), sizeof(uint16_t) * eu.size(), s, "sell entries");
     to_device(S.map, map.data(), sizeof(int64_t) * map.size(), s, "sell entries");
+    S.job_tile0.clear();
+    if (breaks) {
+        for (int64_t b : *breaks)
+            S.job_tile0.push_back((int32_t)(std::lower_bound(row0.begin(), row0.end() - 1, (int32_t)b) - row0.begin()));
+        to_device(S.djob_tile0, S.job_tile0.data(), sizeof(int32_t) * S.job_tile0.size(), s, "sell job tiles");
+    }
     check(hipStreamSynchronize(s), "sell");
     S.ntiles = tile;
     S.tile_rows = tile_rows;
@@ -456,15 +472,19 @@ bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int uc
     return true;
 }
 
-// the SELL values of the current matrix (FP64, or FP32 for the mixed recursion)
-const void* sell_values(leo2d_handle* h, const void* csr_val, int elem_bytes, bool fp32)
+bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, int tmax, bool spatial)
 {
-    auto& S = h->sell;
+    return build_sell_tiles(h->sell, h->stream, h->n, h->nnz, drp, dcol, ucap, tmax, spatial, nullptr);
+}
+
+// the SELL values of a matrix (FP64, or FP32 for the mixed recursion)
+const void* sell_values(SellTiles& S, hipStream_t stream, const void* csr_val, int elem_bytes, bool fp32)
+{
     DevBuf& buf = fp32 ? S.val32 : S.val;
     bool& ok = fp32 ? S.val32_valid : S.val_valid;
     if (!ok) {
         dalloc(buf, (size_t)elem_bytes * (size_t)S.entries, "sell values");
-        check(leo2d::launch_sell_fill((const int64_t*)S.map.p, csr_val, buf.p, elem_bytes, S.entries, h->stream),
+        check(leo2d::launch_sell_fill((const int64_t*)S.map.p, csr_val, buf.p, elem_bytes, S.entries, stream),
               "sell values");
         ok = true;
     }
@@ -580,6 +600,8 @@ void batch_stack(leo2d_handle* h)
     to_device(B.drow0, B.job_row0.data(), sizeof(int64_t) * B.job_row0.size(), s, "batch job rows");
     check(hipStreamSynchronize(s), "batch job rows");
     B.tile_rows = 0;
+    B.sell.valid = B.sell.failed = false;
+    B.sell.val_valid = B.sell.val32_valid = false;
     B.tuned.clear();   // shapes were tuned on the previous stack
     B.drop_graph();
 }
@@ -1370,7 +1392,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     auto& S = h->sell;
                     const int hb = mat_type == leo2d::MAT_CPX_F64 ? 16
                                    : (mat_type == leo2d::MAT_REAL_F32 ? 4 : 8);
-                    ss.ev = sell_values(h, dval, hb, mixed);
+                    ss.ev = sell_values(S, h->stream, dval, hb, mixed);
                     int dev_id = 0, cus = 256;
                     if (hipGetDevice(&dev_id) == hipSuccess)
                         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
@@ -1392,6 +1414,8 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     ss.tile_rows = S.tile_rows;
                     ss.max_len = S.max_len;
                     ss.sliced = env_int("LEO2D_SELL_SLICED", 0);
+                    ss.ucap = S.ucap;
+                    ss.tile_part = 0;
                 }
             }
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
@@ -1872,7 +1896,8 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         if (M < 4 || M % 4 != 0) throw ArgError("M must be a positive multiple of 4");
         if (R != 4 && R != 8 && R != 16 && R != 32) throw ArgError("R must be 4, 8, 16 or 32");
         if (p->precision != LEO2D_PREC_FP64 && p->precision != LEO2D_PREC_MIXED) throw ArgError("unknown precision");
-        if (p->mode != LEO2D_BATCH_ROW_GROUPS && p->mode != LEO2D_BATCH_ENTRY_SLOTS && p->mode != LEO2D_BATCH_ROW_CHUNKS)
+        if (p->mode != LEO2D_BATCH_ROW_GROUPS && p->mode != LEO2D_BATCH_ENTRY_SLOTS && p->mode != LEO2D_BATCH_ROW_CHUNKS &&
+            p->mode != LEO2D_BATCH_SELL)
             throw ArgError("unknown mode");
         if (p->mode == LEO2D_BATCH_ROW_CHUNKS && R != 16 && R != 32) throw ArgError("row chunks need R = 16 or 32");
         constexpr int kMinTile = 16;
@@ -1926,8 +1951,17 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 dalloc(b, bytes, what);
             }
         };
+        // SELL tiles of the stack (complex vectors, R = 16 or 32; FP32 only at 32)
+        const bool sell_ok = leo2d::sell_supported(mat_type, vec_type, R) && R <= 32 &&
+                             build_sell_tiles(B.sell, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
+                                              (const int32_t*)B.col.p, leo2d::kSellUcapTiled, leo2d::kSellMaxRows, false,
+                                              &B.job_row0);
+        if (!p->tune && p->mode == LEO2D_BATCH_SELL && !sell_ok)
+            throw ArgError("LEO2D_BATCH_SELL needs complex vectors with R = 16 or 32 (32 for FP32) and rows of at most "
+                           "64 entries");
         int64_t max_tiles = 0;   // tiles of the shortest allowed height bound every tiling's count
         for (int q = 0; q < P; ++q) max_tiles += (B.job_row0[q + 1] - B.job_row0[q] + kMinTile - 1) / kMinTile;
+        if (sell_ok) max_tiles = std::max<int64_t>(max_tiles, B.sell.ntiles);
         ensure(B.A, vbytes, "batch vectors");
         ensure(B.B, vbytes, "batch vectors");
         ensure(B.part, sizeof(double) * 2 * (size_t)std::max<int64_t>(max_tiles, 1) * R, "batch partials");
@@ -1939,14 +1973,61 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         sa.val = dval;
         sa.R = R;
         sa.part = (double*)B.part.p;
+        // the SELL step: one workgroup per CU, partials per tile
+        leo2d::SellStepArgs ss{};
+        int cus = 256;
+        {
+            int dev_id = 0;
+            if (hipGetDevice(&dev_id) == hipSuccess)
+                (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id);
+        }
+        if (sell_ok) {
+            auto& S = B.sell;
+            const int hb = mat_type == leo2d::MAT_CPX_F64 ? 16 : (mat_type == leo2d::MAT_REAL_F32 ? 4 : 8);
+            ss.ev = sell_values(S, s, dval, hb, mixed);
+            ss.n = (int)B.rows;
+            ss.R = R;
+            ss.part = (double*)B.part.p;
+            ss.grid = std::max(8, cus / 8 * 8);
+            ss.tile_part = 1;
+            ss.ucap = S.ucap;
+            ss.ntiles = S.ntiles;
+            ss.tile_row0 = (const int32_t*)S.row0.p;
+            ss.tile_u0 = (const int64_t*)S.u0.p;
+            ss.ucols = (const int32_t*)S.ucols.p;
+            ss.tile_g0 = (const int32_t*)S.g0.p;
+            ss.grow = (const int32_t*)S.grow.p;
+            ss.glen = (const int32_t*)S.glen.p;
+            ss.goff = (const int64_t*)S.goff.p;
+            ss.eu = (const uint16_t*)S.eu.p;
+            ss.tile_rows = S.tile_rows;
+            ss.max_len = S.max_len;
+        }
+        const int32_t* job_tiles = (const int32_t*)B.djob_tile0.p;
         auto set_shape = [&](int mode, int T, int grid) {
-            batch_tiles(h, T);
             sa.mode = mode;
+            if (mode == LEO2D_BATCH_SELL) {
+                sa.grid = ss.grid;
+                job_tiles = (const int32_t*)B.sell.djob_tile0.p;
+                return;
+            }
+            batch_tiles(h, T);
             sa.tile_row0 = (const int32_t*)B.dtile_row0.p;
             sa.ntiles = B.ntiles;
             sa.grid = grid > 0 ? grid : batch_default_grid(B.ntiles);
+            job_tiles = (const int32_t*)B.djob_tile0.p;
         };
         auto step = [&](void* cur, void* out, int j) {
+            if (sa.mode == LEO2D_BATCH_SELL) {
+                ss.cur = cur;
+                ss.prev = out;
+                ss.out = out;
+                ss.alpha = j == 0 ? 1.0 : 2.0;
+                ss.beta = j == 0 ? 0 : 1;
+                ss.reverse = j & 1;
+                check(leo2d::launch_kpm_step_sell(mat_type, vec_type, ss, s), "batch step");
+                return;
+            }
             sa.cur = cur;
             sa.out = out;
             sa.alpha = j == 0 ? 1.0 : 2.0;
@@ -1975,6 +2056,23 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 step(B.A.p, B.B.p, 0);
                 check(hipEventRecord(e0, s), "event");
                 float best = 1e30f;
+                if (sell_ok) {
+                    // SELL tiles: the shape is fixed by the LDS budget (one candidate)
+                    set_shape(LEO2D_BATCH_SELL, 0, 0);
+                    step(B.A.p, B.B.p, 1);
+                    check(hipEventRecord(e1, s), "event");
+                    for (int k = 0; k < 4; ++k) step(k % 2 ? B.B.p : B.A.p, k % 2 ? B.A.p : B.B.p, 1 + k);
+                    hipEvent_t e2 = event_at(h, 2);
+                    check(hipEventRecord(e2, s), "event");
+                    check(hipEventSynchronize(e2), "tune");
+                    float ms = 0.f;
+                    check(hipEventElapsedTime(&ms, e1, e2), "event");
+                    ++candidates;
+                    best = ms;
+                    mode = LEO2D_BATCH_SELL;
+                    T = B.sell.tile_rows;
+                    grid = ss.grid;
+                }
                 for (int tt : {64, 128, 256, 512}) {
                     for (int md : {0, 1, 2}) {
                         if (md == 2 && R != 16 && R != 32) continue;
@@ -2013,6 +2111,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         }
         set_shape(mode, T, grid);
         grid = sa.grid;
+        if (mode == LEO2D_BATCH_SELL) T = B.sell.tile_rows;
 
         auto record = [&]() {
             check(leo2d::launch_batch_init(B.A.p, vec_type, (const int64_t*)B.drow0.p, P, B.rows, R, p->seed, dtargets, s),
@@ -2021,14 +2120,14 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 void* cur = (j % 2 == 0) ? B.A.p : B.B.p;
                 void* other = (j % 2 == 0) ? B.B.p : B.A.p;
                 step(cur, other, j);
-                check(leo2d::launch_batch_reduce((const double*)B.part.p, (const int32_t*)B.djob_tile0.p, P, R, M, j,
-                                                 (double*)B.mu.p, s),
+                check(leo2d::launch_batch_reduce((const double*)B.part.p, job_tiles, P, R, M, j, (double*)B.mu.p, s),
                       "batch reduce");
             }
         };
         if (p->graph) {
             const leo2d_handle::Batch::GraphKey key{M, R, p->precision, mode, T, grid, p->seed,
-                                                     B.A.p, B.B.p, B.part.p, B.mu.p, dval, dtargets};
+                                                     B.A.p, B.B.p, B.part.p, B.mu.p,
+                                                     mode == LEO2D_BATCH_SELL ? ss.ev : dval, dtargets};
             if (!B.exec || !(key == B.key)) {
                 B.drop_graph();
                 hipGraph_t graph = nullptr;
@@ -2071,7 +2170,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         t.graph = p->graph ? 1 : 0;
         t.candidates = candidates;
         t.jobs = P;
-        t.ntiles = B.ntiles;
+        t.ntiles = mode == LEO2D_BATCH_SELL ? B.sell.ntiles : B.ntiles;
         t.rows = B.rows;
         t.nnz = B.nnz;
         for (int q = 0; q < P * R; ++q) {

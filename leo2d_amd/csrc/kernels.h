@@ -50,6 +50,7 @@ hipError_t launch_kpm_step(int mat_type, int vec_type, const StepArgs& a, hipStr
 constexpr int kSellMaxRows = 64;    // rows per tile (8 wavefronts x 2 groups x 4 rows)
 constexpr int kSellMaxLen = 64;     // longest row a group can hold
 constexpr int kSellUcap = 288;      // staged rows per LDS buffer (two buffers of 288 x 256 B)
+constexpr int kSellUcapTiled = 256; // the same with per-tile partials (their LDS takes the difference)
 constexpr int kSellWaves = 8;       // wavefronts per workgroup (one workgroup per CU)
 struct SellStepArgs {
     int n;
@@ -60,8 +61,10 @@ struct SellStepArgs {
     const void* v;
     double alpha;
     int beta;
-    double* part;               // [grid][R][3]
+    double* part;               // [grid][R][3], or [ntiles][R][2] with tile_part
     int grid;
+    int tile_part;              // 1: partial sums per tile (stacked jobs), at most kSellUcapTiled union rows
+    int ucap;                   // union rows the tiles were cut to
     int ntiles;
     const int32_t* tile_row0;   // [ntiles + 1]
     const int64_t* tile_u0;     // [ntiles + 1] offsets into ucols

@@ -202,7 +202,9 @@ __device__ inline T ldu(const T* p, int64_t i)
 // which are issued before the copies, so their wait leaves the copies in flight.
 // NW wavefronts per workgroup, UC staged rows per LDS buffer (SPW = UC / 4 NW
 // staging loads per wavefront and chunk)
-template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, bool DIRECT, int NW, int UC>
+// PT: partial sums per tile (part[tile][R][2], for stacked jobs whose moments add
+// up over their own tiles) instead of one per workgroup (part[workgroup][R][3]).
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, bool DIRECT, int NW, int UC, bool PT>
 __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs a)
 {
     using AT = typename acc_of<XT>::type;
@@ -216,6 +218,9 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     // group j at slot j, bytes 16 l .. 16 l + 15: row s = l / 16, vectors of lane k = l % 16)
     constexpr bool TDMA = !DIRECT;
     __shared__ __align__(16) XT tp[TDMA ? NW * GPW * 64 * VPL : 1];
+    // per-tile partials of each wavefront, two sets used by alternate tiles
+    __shared__ double tred[PT ? 2 * NW * R * 2 : 1];
+    static_assert(!(PT && DIRECT), "per-tile partials carry two sums");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: group loops branch on scalars
     const int s = lane >> 4;
@@ -320,9 +325,24 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
         if (NCH == 1) load_ucols(live(t + t_stride) ? t + t_stride : t, ucs);
     }
 
+    int pend_tile = -1, pend_set = 0, tset = 0;
+    // sum the wavefronts' partials of the tile finished in the previous pair
+    auto emit_tile = [&]() {
+        if (PT && pend_tile >= 0) {
+            for (int i = threadIdx.x; i < 2 * R; i += 64 * NW) {
+                double sum = tred[(pend_set * NW) * R * 2 + i];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) sum += tred[(pend_set * NW + w) * R * 2 + i];
+                a.part[(size_t)pend_tile * R * 2 + i] = sum;
+            }
+            pend_tile = -1;
+        }
+    };
+
     auto pair = [&](XT* buf, XT* nbuf) {
         wait_vm<0>();
         __syncthreads();
+        emit_tile();
         if (c == 0 && p > 0) {
 #pragma unroll
             for (int j = 0; j < GPW; ++j) {
@@ -427,6 +447,29 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
                 for (int v = 0; v < VPL; ++v) out[idx + v] = o[j][v];
             }
         }
+        if (PT && c == NCH - 1) {
+            // the tile's last chunk: fold its sums over the four DPP rows into this
+            // wavefront's slot of the tile partials, summed over the wavefronts next pair
+#pragma unroll
+            for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    d1[cc][v] += __shfl_xor(d1[cc][v], 16);
+                    d2[cc][v] += __shfl_xor(d2[cc][v], 16);
+                    d1[cc][v] += __shfl_xor(d1[cc][v], 32);
+                    d2[cc][v] += __shfl_xor(d2[cc][v], 32);
+                    if (s == 0) {
+                        const int r = cc * C + koff + v;
+                        tred[((tset * NW + wave) * R + r) * 2 + 0] = d1[cc][v];
+                        tred[((tset * NW + wave) * R + r) * 2 + 1] = d2[cc][v];
+                    }
+                    d1[cc][v] = 0.0;
+                    d2[cc][v] = 0.0;
+                }
+            pend_tile = t;
+            pend_set = tset;
+            tset ^= 1;
+        }
         // loads for the next pairs: union columns of pair p+2's tile, the next tile's groups
         int t2 = t1, c2 = c1 + 1;
         if (c2 == NCH) {
@@ -449,6 +492,10 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     }
     wait_vm<0>();
     __syncthreads();
+    if constexpr (PT) {
+        emit_tile();
+        return;
+    }
 
     // fold the four DPP rows (same vectors), then the wavefronts
 #pragma unroll
@@ -485,25 +532,36 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     }
 }
 
-template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NW, int UC>
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NW>
 hipError_t launch_sell(const SellStepArgs& a, hipStream_t s)
 {
-    if (a.v) hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, true, NW, UC>), dim3(a.grid),
-                                dim3(64 * NW), 0, s, a);
-    else hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false, NW, UC>), dim3(a.grid),
-                            dim3(64 * NW), 0, s, a);
+    constexpr int UC = kSellUcap, UCT = kSellUcapTiled;
+    if (a.tile_part) {
+        if (a.v || a.ucap > UCT) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false, NW, UCT, true>), dim3(a.grid),
+                           dim3(64 * NW), 0, s, a);
+    } else if (a.v) {
+        hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, true, NW, UC, false>), dim3(a.grid),
+                           dim3(64 * NW), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((kpm_step_sell_kernel<HT, XT, VPL, NCH, GPW, QS, false, NW, UC, false>), dim3(a.grid),
+                           dim3(64 * NW), 0, s, a);
+    }
     return hipGetLastError();
 }
 
 template <typename HT, typename XT, int VPL, int GPW, int QS>
 hipError_t launch_sell_r(const SellStepArgs& a, hipStream_t s)
 {
-    constexpr int NW = kSellWaves, UC = kSellUcap;
+    constexpr int NW = kSellWaves;
     switch (a.R / (16 * VPL)) {
-    case 1: return launch_sell<HT, XT, VPL, 1, GPW, QS, NW, UC>(a, s);
-    case 2: return launch_sell<HT, XT, VPL, 2, GPW, QS, NW, UC>(a, s);
+    case 1: return launch_sell<HT, XT, VPL, 1, GPW, QS, NW>(a, s);
+    case 2: return launch_sell<HT, XT, VPL, 2, GPW, QS, NW>(a, s);
     case 4:
-        if constexpr (VPL == 1) return launch_sell<HT, XT, VPL, 4, GPW, QS, NW, UC>(a, s);
+        if constexpr (VPL == 1) {
+            if (a.tile_part) return hipErrorInvalidValue;   // stacked jobs carry 16 or 32 vectors
+            return launch_sell<HT, XT, VPL, 4, GPW, QS, NW>(a, s);
+        }
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }

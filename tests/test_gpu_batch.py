@@ -53,6 +53,30 @@ def test_batch_fp64_matches_oracle(batch, mode, tile_rows, graph):
     assert t.mode == mode and t.tile_rows == tile_rows and t.graph == int(graph)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_sell_matches_oracle(batch, graph):
+    """SELL-4-sigma tiles cut at the job boundaries (six jobs of 76 to 364
+    orbitals), partial sums per tile: FP64 moments within 1e-12 mu_0 of the
+    oracle for every job, eager and captured; the FP32 recursion needs 32
+    vectors per job for this layout (16-byte lanes of two vectors)."""
+    dev, jobs = batch
+    print("case SELL graph", graph, flush=True)
+    mu = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, graph=graph)
+    for p, (n, omu) in enumerate(jobs):
+        assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (p, graph)
+    t = dev.batch_timing()
+    assert t.mode == L.BATCH_SELL and 0 < t.tile_rows <= 64 and t.ntiles >= len(jobs)
+    with pytest.raises(L.Leo2dError) as e:
+        dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, precision=L.PREC_MIXED)
+    assert e.value.code == L.ERR_ARG
+    if graph:
+        eager = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, graph=False)
+        for rep in range(3):
+            g = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, graph=True)
+            assert np.array_equal(g, eager), rep
+            dev.kpm(16, random_phase=(4, rep), scheme=L.SCHEME_DOUBLING)
+
+
 @pytest.mark.parametrize("Rv", [4, 8, 32])
 def test_batch_other_block_widths(oracle_mod, Rv):
     """R = 4, 8, 32 vectors per job on two jobs of different size."""
@@ -78,6 +102,16 @@ def test_batch_other_block_widths(oracle_mod, Rv):
             mu = dev.batch_kpm(32, Rv, seed=11, mode=mode, graph=False)
             for p, (n, omu) in enumerate(want):
                 assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (Rv, mode, p)
+        # SELL tiles: 16 or 32 vectors per job; the FP32 recursion at 32
+        if Rv < 16:
+            with pytest.raises(L.Leo2dError) as e:
+                dev.batch_kpm(32, Rv, seed=11, mode=L.BATCH_SELL, graph=False)
+            assert e.value.code == L.ERR_ARG
+        else:
+            for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+                mu = dev.batch_kpm(32, Rv, seed=11, mode=L.BATCH_SELL, precision=prec, graph=False)
+                for p, (n, omu) in enumerate(want):
+                    assert np.max(np.abs(mu[p] - omu)) <= tol * n, (Rv, prec, p)
     finally:
         dev.close()
 
@@ -273,6 +307,11 @@ def test_batch_c2_cells(oracle_mod):
                 vecs = oracle_mod.random_phase(geom.n, Rb, 5, r0=p * Rb)
                 want.append((geom.n, oracle_mod.kpm(rp, ci, v, Mb, vecs=vecs, doubling_all=True, nthreads=16)))
         mu = dev.batch_kpm(Mb, Rb, seed=5, tune=True)
+        for p, (n, omu) in enumerate(want):
+            assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, p
+        # SELL tiles, several per job, walked in both directions over the 512 steps
+        mu = dev.batch_kpm(Mb, Rb, seed=5, mode=L.BATCH_SELL, graph=True)
+        assert dev.batch_timing().ntiles > len(want)
         for p, (n, omu) in enumerate(want):
             assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, p
     finally:
