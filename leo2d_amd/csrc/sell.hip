@@ -94,16 +94,23 @@ __device__ inline void ld_block(XT (&x)[4][VPL], const GroupRegs<HT, QS>& g, con
 // Complex products with the entry broadcast folded into the multiply-adds: the
 // DPP form of v_fmac reads src0 from lane K of each 16-lane row, so one step is
 // four (FP64, one vector) or eight (FP32, two vectors) fused multiply-adds in
-// the order of mac(): re += hr xr, re -= hi xi, im += hr xi, im += hi xr.  DPP
-// needs two wait states after a VALU write of its source (the entry registers
-// are rewritten when a tile starts), hence one s_nop 1 per step.
+// the order of mac(): re += hr xr, re -= hi xi, im += hr xi, im += hi xr.
+//
+// A DPP source needs two wait states after a VALU write of it.  The entry values
+// are written only when a tile's groups are taken over, so one s_nop 1 before a
+// group's products, tied to the value registers (they cannot move past it),
+// covers every step of the group.  (An address formed by one DPP add in place
+// of the broadcast and the shift-or measured slower: 872 against 823 us FP64.)
+__device__ inline void pin_values(double& a) { asm volatile("s_nop 1" : "+v"(a)); }
+__device__ inline void pin_values(float& a) { asm volatile("s_nop 1" : "+v"(a)); }
+__device__ inline void pin_values(double2& a) { asm volatile("s_nop 1" : "+v"(a.x), "+v"(a.y)); }
+__device__ inline void pin_values(float2& a) { asm volatile("s_nop 1" : "+v"(a.x), "+v"(a.y)); }
 template <int K> struct DppMac;
 #define LEO2D_DPP_MAC(K)                                                                                          \
     template <> struct DppMac<K> {                                                                                \
         __device__ static inline void f64(double& ax, double& ay, double hr, double hi, double xx, double xy)     \
         {                                                                                                          \
-            asm("s_nop 1\n\t"                                                                                    \
-                "v_fmac_f64_dpp %0, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+            asm("v_fmac_f64_dpp %0, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
                 "v_fmac_f64_dpp %0, -%3, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
                 "v_fmac_f64_dpp %1, %2, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
                 "v_fmac_f64_dpp %1, %3, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf"                       \
@@ -113,8 +120,7 @@ template <int K> struct DppMac;
         __device__ static inline void f32(float& a0x, float& a0y, float& a1x, float& a1y, float hr, float hi,    \
                                           float x0x, float x0y, float x1x, float x1y)                              \
         {                                                                                                          \
-            asm("s_nop 1\n\t"                                                                                    \
-                "v_fmac_f32_dpp %0, %4, %6 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+            asm("v_fmac_f32_dpp %0, %4, %6 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
                 "v_fmac_f32_dpp %0, -%5, %7 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
                 "v_fmac_f32_dpp %1, %5, %6 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
@@ -398,6 +404,8 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
             asm volatile("" : "+s"(L));
             if (L > 0) {
                 XT xa[4][VPL];
+#pragma unroll
+                for (int q = 0; q < QS; ++q) pin_values(ec[j].ev[q]);
                 ld_block<0, HT, XT, VPL, QS>(xa, ec[j], buf, koff);
                 run_blocks<0, HT, XT, AT, VPL, QS>(acc[j], ec[j], buf, koff, L, xa);
             }
