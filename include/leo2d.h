@@ -314,6 +314,13 @@ int leo2d_mlp_gradients(leo2d_mlp* m, const float* x, const float* t, int32_t n,
 /* Adam over shuffled mini-batches; epoch_loss[e] = mean squared error of epoch e (may be NULL). */
 int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, const leo2d_mlp_train_params* p,
                     double* epoch_loss);
+/* Trains count networks of one shape together on the same data: network k shuffles
+ * with seeds[k] and takes exactly the updates leo2d_mlp_train with that seed would
+ * (bit for bit), but each network's epoch is one captured graph replayed on its own
+ * stream, so the networks' small launches overlap.  p->seed is not used.
+ * losses (may be NULL): [count][epochs] mean squared error per epoch. */
+int leo2d_mlp_train_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, const float* t, int64_t n,
+                             const leo2d_mlp_train_params* p, const uint64_t* seeds, double* losses);
 /* Ensemble of count networks of one shape on one device: mean[n][out] (may be NULL) and
  * spread[n][blocks]: for each of `blocks` equal slices of the outputs (one per
  * sublattice, say), the root mean square over the slice of the ensemble

@@ -53,7 +53,7 @@ EXPORTS = [
     "leo2d_run_deck", "leo2d_batch_clear", "leo2d_batch_append", "leo2d_batch_info", "leo2d_batch_kpm",
     "leo2d_batch_last_timing", "leo2d_mlp_features", "leo2d_mlp_create", "leo2d_mlp_destroy",
     "leo2d_mlp_last_error", "leo2d_mlp_num_params", "leo2d_mlp_get_params", "leo2d_mlp_set_params",
-    "leo2d_mlp_forward", "leo2d_mlp_gradients", "leo2d_mlp_train", "leo2d_mlp_ensemble",
+    "leo2d_mlp_forward", "leo2d_mlp_gradients", "leo2d_mlp_train", "leo2d_mlp_train_ensemble", "leo2d_mlp_ensemble",
     "leo2d_assemble_velocity", "leo2d_velocity_copy", "leo2d_set_relaxation", "leo2d_read_relaxation", "leo2d_cond_moments", "leo2d_cond_transform",
     "leo2d_cond_last_timing",
 ]
@@ -203,6 +203,7 @@ def lib() -> C.CDLL:
         L.leo2d_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
         L.leo2d_mlp_gradients.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]
         L.leo2d_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(MlpTrainParams), vp]
+        L.leo2d_mlp_train_ensemble.argtypes = [vp, C.c_int32, vp, vp, C.c_int64, C.POINTER(MlpTrainParams), vp, vp]
         L.leo2d_mlp_ensemble.argtypes = [C.POINTER(vp), C.c_int32, vp, C.c_int64, C.c_int32, vp, vp]
         L.leo2d_batch_clear.argtypes = [vp]
         L.leo2d_batch_append.argtypes = [vp, C.POINTER(C.c_int32)]
@@ -583,6 +584,22 @@ class Mlp:
         losses = np.zeros(max(epochs, 1))
         self._check(lib().leo2d_mlp_train(self.h, _p(x), _p(t), x.shape[0], C.byref(p), _p(losses)))
         return losses[:epochs]
+
+
+def mlp_train_ensemble(nets, x, t, seeds, epochs=10, batch=256, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """Train the networks together (leo2d_mlp_train_ensemble): network k shuffles
+    with seeds[k]; returns the per-epoch losses [len(nets)][epochs]."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    p = MlpTrainParams()
+    p.epochs, p.batch, p.lr, p.beta1, p.beta2, p.eps, p.seed = epochs, batch, lr, beta1, beta2, eps, 0
+    arr = (C.c_void_p * len(nets))(*[m.h for m in nets])
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    losses = np.zeros((len(nets), max(epochs, 1)))
+    rc = lib().leo2d_mlp_train_ensemble(arr, len(nets), _p(x), _p(t), x.shape[0], C.byref(p), _p(sd), _p(losses))
+    if rc != OK:
+        raise Leo2dError(rc, lib().leo2d_mlp_last_error(nets[0].h).decode())
+    return losses[:, :epochs]
 
 
 def mlp_ensemble(nets, x, with_mean=True, blocks=1):

@@ -45,5 +45,10 @@ hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int ro
 // Adam step `step` (1-based) on FP32 master weights, refreshing their bf16 copy
 hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
                        float eps, int step, uint16_t* p16, hipStream_t s);
+// Adam's bias corrections 1 - b1^step, 1 - b2^step (host, the values launch_adam uses)
+hipError_t adam_corrections(float b1, float b2, int step, float* c1, float* c2);
+// Adam with the corrections of step *ctr taken from corr[2 *ctr .. 2 *ctr + 1], then *ctr += 1
+hipError_t launch_adam_table(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                             float eps, const float* corr, int* ctr, uint16_t* p16, hipStream_t s);
 
 }  // namespace leo2d

@@ -351,6 +351,33 @@ __global__ __launch_bounds__(kT) void adam_kernel(float* p, const float* grad, f
     }
 }
 
+// The same update with the bias corrections of step *ctr read from corr[2 *ctr],
+// corr[2 *ctr + 1] (a captured training epoch replays unchanged while the step
+// advances); count_kernel advances *ctr after it.
+__global__ __launch_bounds__(kT) void adam_table_kernel(float* p, const float* grad, float* m, float* v, int64_t n,
+                                                        float lr, float b1, float b2, float eps, const float* corr,
+                                                        const int* ctr, uint16_t* p16)
+{
+    const int k = *ctr;
+    const float c1 = corr[2 * (int64_t)k], c2 = corr[2 * (int64_t)k + 1];
+    for (int64_t i = blockIdx.x * (int64_t)kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+        const float gr = grad[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gr;
+        const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+        m[i] = mi;
+        v[i] = vi;
+        const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        const float pn = p[i] - upd;
+        p[i] = pn;
+        if (p16) p16[i] = f2bf(pn);
+    }
+}
+
+__global__ void count_kernel(int* ctr)
+{
+    if (threadIdx.x == 0) *ctr += 1;
+}
+
 // per row: mean over the E ensemble members and, for each of nblk equal blocks
 // of the outputs, the root mean square over the block of the ensemble standard
 // deviation (block grid: blockIdx.y)
@@ -477,9 +504,28 @@ hipError_t launch_ensemble_stats(const float* Ys, int E, int64_t estride, int ro
 hipError_t launch_adam(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
                        float eps, int step, uint16_t* p16, hipStream_t s)
 {
-    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    float c1, c2;
+    adam_corrections(b1, b2, step, &c1, &c2);
     hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, p, grad, m, v, n, lr, b1, b2, eps, c1, c2,
                        p16);
+    return hipGetLastError();
+}
+
+hipError_t adam_corrections(float b1, float b2, int step, float* c1, float* c2)
+{
+    *c1 = 1.f - powf(b1, (float)step);
+    *c2 = 1.f - powf(b2, (float)step);
+    return hipSuccess;
+}
+
+hipError_t launch_adam_table(float* p, const float* grad, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                             float eps, const float* corr, int* ctr, uint16_t* p16, hipStream_t s)
+{
+    hipLaunchKernelGGL(adam_table_kernel, dim3(blocks_for(n)), dim3(kT), 0, s, p, grad, m, v, n, lr, b1, b2, eps, corr,
+                       ctr, p16);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(count_kernel, dim3(1), dim3(64), 0, s, ctr);
     return hipGetLastError();
 }
 

@@ -77,6 +77,12 @@ const uint16_t* bf(const DevBuf& b, int64_t off = 0) { return static_cast<const 
 uint16_t* bfw(DevBuf& b, int64_t off = 0) { return static_cast<uint16_t*>(b.p) + off; }
 float* f32(DevBuf& b, int64_t off = 0) { return static_cast<float*>(b.p) + off; }
 
+void to_device_f32(DevBuf& b, const std::vector<float>& v, hipStream_t s)
+{
+    dalloc(b, sizeof(float) * v.size(), "adam corrections");
+    check(hipMemcpyAsync(b.p, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice, s), "adam corrections");
+}
+
 void ensure_capacity(leo2d_mlp* m, int rows)
 {
     if (rows <= m->cap) return;
@@ -132,13 +138,14 @@ void forward(leo2d_mlp* m, int rows, float* Y)
     }
 }
 
-// gradient of the mean squared error over rows x out into m->G; row losses in m->rowloss
-void backward(leo2d_mlp* m, int rows)
+// gradient of the mean squared error over rows x out into m->G; row losses in
+// row_loss (m->rowloss when null)
+void backward(leo2d_mlp* m, int rows, double* row_loss = nullptr)
 {
     hipStream_t s = m->stream;
     const int L = m->layers();
     check(leo2d::launch_mse_grad(f32(m->Y), f32(m->T), rows, m->out(), 2.0f / ((float)rows * (float)m->out()),
-                                 bfw(m->dY16), static_cast<double*>(m->rowloss.p), s),
+                                 bfw(m->dY16), row_loss ? row_loss : static_cast<double*>(m->rowloss.p), s),
           "mlp loss");
     const uint16_t* gz = bf(m->dY16);
     DevBuf* next = &m->dZa;
@@ -409,6 +416,137 @@ int leo2d_mlp_train(leo2d_mlp* m, const float* x, const float* t, int64_t n, con
             check(hipStreamSynchronize(s), "epoch");
             if (epoch_loss) epoch_loss[e] = std::accumulate(rl.begin(), rl.end(), 0.0) / ((double)n * m->out());
         }
+        return LEO2D_OK;
+    });
+}
+
+// Several networks trained together: each epoch of each network is captured once
+// as a graph (gather, forward, backward, Adam for every mini-batch, the Adam step
+// read from a device counter) and replayed per epoch on the network's own stream,
+// so the networks' short launches run side by side and the host issues one launch
+// per network and epoch.  Each network's arithmetic, shuffles and updates are
+// those of leo2d_mlp_train with its own seed, bit for bit.
+int leo2d_mlp_train_ensemble(leo2d_mlp* const* nets, int32_t count, const float* x, const float* t, int64_t n,
+                             const leo2d_mlp_train_params* p, const uint64_t* seeds, double* losses)
+{
+    if (!nets || count < 1 || !x || !t || !p || !seeds || n <= 0 || p->epochs < 0 || p->batch <= 0)
+        return LEO2D_ERR_ARG;
+    leo2d_mlp* m0 = nets[0];
+    return guarded(m0, [&]() {
+        for (int k = 0; k < count; ++k) {
+            if (!nets[k]) throw ArgError("null network");
+            if (nets[k]->dims != m0->dims || nets[k]->device != m0->device)
+                throw ArgError("ensemble members differ in shape or device");
+        }
+        const int B = (int)std::min<int64_t>(p->batch, n);
+        const int64_t steps = (n + B - 1) / B;
+        const int64_t total = steps * (int64_t)p->epochs;
+        if (total + 1 > 0x7fffffffLL) throw ArgError("too many training steps");
+        if (p->epochs == 0) return LEO2D_OK;
+        hipStream_t s0 = m0->stream;
+        DevBuf X, T;
+        dalloc(X, sizeof(float) * (size_t)n * m0->in(), "training inputs");
+        dalloc(T, sizeof(float) * (size_t)n * m0->out(), "training targets");
+        check(hipMemcpyAsync(X.p, x, sizeof(float) * (size_t)n * m0->in(), hipMemcpyHostToDevice, s0), "training inputs");
+        check(hipMemcpyAsync(T.p, t, sizeof(float) * (size_t)n * m0->out(), hipMemcpyHostToDevice, s0),
+              "training targets");
+        check(hipStreamSynchronize(s0), "training data");
+        struct Member {
+            DevBuf idx, loss, corr, ctr;
+            std::vector<int64_t> order;
+            std::vector<double> rl;
+            std::mt19937_64 rng;
+            hipGraphExec_t exec = nullptr;
+        };
+        std::vector<Member> mem((size_t)count);
+        auto release = [&]() {
+            for (auto& q : mem)
+                if (q.exec) (void)hipGraphExecDestroy(q.exec);
+        };
+        try {
+            for (int k = 0; k < count; ++k) {
+                leo2d_mlp* m = nets[k];
+                Member& q = mem[(size_t)k];
+                hipStream_t s = m->stream;
+                ensure_capacity(m, B);
+                dalloc(q.idx, sizeof(int64_t) * (size_t)n, "training order");
+                dalloc(q.loss, sizeof(double) * (size_t)n, "training losses");
+                dalloc(q.ctr, sizeof(int), "adam step");
+                // bias corrections of every step of this call, from the network's own step count
+                std::vector<float> corr((size_t)(2 * (total + 1)));
+                for (int64_t i = 0; i <= total; ++i)
+                    leo2d::adam_corrections(p->beta1, p->beta2, (int)(m->adam_step + 1 + i), &corr[2 * (size_t)i],
+                                            &corr[2 * (size_t)i + 1]);
+                to_device_f32(q.corr, corr, s);
+                const int zero = 0;
+                check(hipMemcpyAsync(q.ctr.p, &zero, sizeof(int), hipMemcpyHostToDevice, s), "adam step");
+                check(hipStreamSynchronize(s), "training setup");
+                q.order.resize((size_t)n);
+                std::iota(q.order.begin(), q.order.end(), (int64_t)0);
+                q.rng.seed(seeds[k]);
+                q.rl.resize((size_t)n);
+                // one epoch, captured
+                hipGraph_t graph = nullptr;
+                check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "epoch capture");
+                try {
+                    for (int64_t b0 = 0; b0 < n; b0 += B) {
+                        const int rows = (int)std::min<int64_t>(B, n - b0);
+                        check(leo2d::launch_gather(static_cast<const float*>(X.p), m->in(),
+                                                   static_cast<const float*>(T.p), m->out(),
+                                                   static_cast<const int64_t*>(q.idx.p) + b0, rows, bfw(m->X16),
+                                                   f32(m->T), s),
+                              "batch gather");
+                        forward(m, rows, f32(m->Y));
+                        backward(m, rows, static_cast<double*>(q.loss.p) + b0);
+                        check(leo2d::launch_adam_table(f32(m->P32), f32(m->G), f32(m->Mom), f32(m->Var), m->nparams,
+                                                       p->lr, p->beta1, p->beta2, p->eps, f32(q.corr),
+                                                       static_cast<int*>(q.ctr.p), bfw(m->P16), s),
+                              "adam");
+                    }
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                check(hipStreamEndCapture(s, &graph), "epoch capture");
+                const hipError_t ie = hipGraphInstantiate(&q.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    q.exec = nullptr;
+                    check(ie, "epoch graph");
+                }
+            }
+            for (int e = 0; e < p->epochs; ++e) {
+                // every member's epoch is queued before any loss is copied back: a copy into
+                // pageable host memory returns only when its stream has drained
+                for (int k = 0; k < count; ++k) {
+                    Member& q = mem[(size_t)k];
+                    hipStream_t s = nets[k]->stream;
+                    std::shuffle(q.order.begin(), q.order.end(), q.rng);
+                    check(hipMemcpyAsync(q.idx.p, q.order.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, s),
+                          "order");
+                    check(hipGraphLaunch(q.exec, s), "epoch launch");
+                }
+                for (int k = 0; k < count; ++k) {
+                    Member& q = mem[(size_t)k];
+                    check(hipMemcpyAsync(q.rl.data(), q.loss.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                                         nets[k]->stream),
+                          "loss");
+                }
+                for (int k = 0; k < count; ++k) {
+                    Member& q = mem[(size_t)k];
+                    check(hipStreamSynchronize(nets[k]->stream), "epoch");
+                    if (losses)
+                        losses[(size_t)k * p->epochs + e] =
+                            std::accumulate(q.rl.begin(), q.rl.end(), 0.0) / ((double)n * nets[k]->out());
+                }
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
+        for (int k = 0; k < count; ++k) nets[k]->adam_step += total;
         return LEO2D_OK;
     });
 }

@@ -246,10 +246,11 @@ def train_ensemble(x, t, members=4, hidden=256, layers=3, epochs=60, batch=256, 
     mu = t.mean(axis=0)
     sd = np.maximum(t.std(axis=0), 1e-6).astype(np.float32)
     ts = ((t - mu) / sd).astype(np.float32)
-    nets, losses = [], []
-    for e in range(members):
-        net = L.Mlp(device, x.shape[1], hidden, layers, t.shape[1], seed=seed * 1000 + e)
-        losses.append(net.train(x, ts, epochs=epochs, batch=batch, lr=lr, seed=seed * 1000 + 100 + e))
+    nets = [L.Mlp(device, x.shape[1], hidden, layers, t.shape[1], seed=seed * 1000 + e) for e in range(members)]
+    # trained side by side; each member's updates are those of Mlp.train with its seed
+    losses = list(L.mlp_train_ensemble(nets, x, ts, [seed * 1000 + 100 + e for e in range(members)], epochs=epochs,
+                                       batch=batch, lr=lr))
+    for net in nets:
         p = net.params()
         dims = net.dims
         nw = dims[-1] * dims[-2]
@@ -259,7 +260,6 @@ def train_ensemble(x, t, members=4, hidden=256, layers=3, epochs=60, batch=256, 
         p[w0:w0 + nw] = (W * sd[:, None]).ravel()
         p[w0 + nw:] = b * sd + mu
         net.set_params(p)
-        nets.append(net)
     return nets, losses
 
 

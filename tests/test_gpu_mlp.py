@@ -109,3 +109,27 @@ def test_mlp_ensemble_mean_and_spread():
     finally:
         for m in nets:
             m.close()
+
+
+def test_train_ensemble_matches_sequential_training():
+    """Three networks trained together (one captured graph per network and epoch,
+    replayed on each network's stream, the Adam step from a device counter) end
+    with the parameters and per-epoch losses of the same networks trained one
+    after another by leo2d_mlp_train with the same seeds, bit for bit; a fourth
+    epoch count exercises the partial last mini-batch (1000 rows, batches of 96)."""
+    rng = np.random.default_rng(17)
+    x, t = _data(rng, 1000, 64, 40)
+    seeds = [101, 202, 303]
+    together = [L.Mlp(0, 64, 96, 2, 40, seed=s) for s in (1, 2, 3)]
+    alone = [L.Mlp(0, 64, 96, 2, 40, seed=s) for s in (1, 2, 3)]
+    try:
+        for rep in range(2):
+            lt = L.mlp_train_ensemble(together, x, t, seeds, epochs=3, batch=96, lr=2e-3)
+            for k, net in enumerate(alone):
+                la = net.train(x, t, epochs=3, batch=96, lr=2e-3, seed=seeds[k])
+                assert np.array_equal(lt[k], la), (rep, k)
+                assert np.array_equal(together[k].params(), net.params()), (rep, k)
+            assert np.all(lt[:, -1] < lt[:, 0])
+    finally:
+        for net in together + alone:
+            net.close()
