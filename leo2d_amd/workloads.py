@@ -93,6 +93,39 @@ POLY_ORDER = {M}
 """
 
 
+def theta_mn(m: int, n: int) -> float:
+    """Twist (degrees) of the (m, n) commensurate cell:
+    cos theta = (n^2 + 4 n m + m^2) / (2 (n^2 + n m + m^2)), negative when
+    m < n (Locality::setupSupercell type 1, src/locality.cpp:205-215)."""
+    c = (n * n + 4 * n * m + m * m) / (2.0 * (n * n + n * m + m * m))
+    t = math.degrees(math.acos(min(1.0, c)))
+    return -t if m < n else t
+
+
+# The small case: the (7, 6) cell (508 orbitals, 5.086 deg) four times over in
+# each direction, 8,128 orbitals, M = 500, R = 16 random-phase vectors (seed 0),
+# Gershgorin rescale with 1 % padding.  The reference builds open geometries as
+# disks (MAX_R, app/src/main.cpp:116-120) and tiles a cell only periodically
+# (BOUNDARY_CONDITION = 1), so the case exists in two reference-expressible forms.
+C0_M, C0_R = 500, 16
+C0_FLAKE_RADIUS = 58.54   # A: 8,108 orbitals, the disk count nearest 8,128 (2 pi r^2 x 0.3817 / A^2)
+
+
+def c0_flake_deck(M: int = C0_M) -> str:
+    """Open disk bilayer flake at the (7, 6) cell's twist whose orbital count
+    is close to the 4 x 4 tiling's 8,128 (GrapheneSimple, real H)."""
+    deck = tblg_flake_deck(theta_mn(7, 6), C0_FLAKE_RADIUS, M=M, name="tblg_c0_flake", rescale=1.0)
+    S = int(math.ceil(2 * C0_FLAKE_RADIUS / 2.46)) + 1   # the index rhombus covers the disk
+    deck = deck.replace(f"MAXSIZE = {int(math.ceil(C0_FLAKE_RADIUS))}\n",
+                        f"MAXSIZE = {S}\nMAX_R = {C0_FLAKE_RADIUS!r}\n")
+    return deck.replace(f"MINSIZE = -{int(math.ceil(C0_FLAKE_RADIUS))}\n", f"MINSIZE = -{S}\n")
+
+
+def c0_twin_deck(M: int = C0_M) -> str:
+    """The (7, 6) cell tiled 4 x 4 with periodic boundaries (SUPERCELL_M_N = 28 24)."""
+    return tblg_cell_deck(7, 6, 4, M=M, name="tblg_c0_twin")
+
+
 def reciprocal(geom) -> np.ndarray:
     """Reciprocal vectors b1, b2 (rows, 1/A) of the geometry's supercell."""
     a = np.array(geom.info.supercell)
@@ -101,13 +134,17 @@ def reciprocal(geom) -> np.ndarray:
 
 
 def assemble_rescaled(dev, geom, k_frac, pattern=L.PATTERN_PRUNED, pad: float = 1.01):
-    """Fill H(k) for k = k_frac[0] b1 + k_frac[1] b2 and rescale it into [-1, 1]:
+    """Fill H(k) for k = k_frac[0] b1 + k_frac[1] b2 (k_frac None: the real H
+    of a flake or of a cell at Gamma) and rescale it into [-1, 1]:
     a = pad (hi - lo)/2, b = (hi + lo)/2 from the Gershgorin interval of the
     unscaled matrix.  Returns (job, a, b, nnz)."""
     job = geom.jobs()[0]
-    k = k_frac[0] * reciprocal(geom)[0] + k_frac[1] * reciprocal(geom)[1]
-    job.complex_matrix = 1
-    job.k[0], job.k[1] = float(k[0]), float(k[1])
+    if k_frac is None:
+        job.complex_matrix = 0
+    else:
+        k = k_frac[0] * reciprocal(geom)[0] + k_frac[1] * reciprocal(geom)[1]
+        job.complex_matrix = 1
+        job.k[0], job.k[1] = float(k[0]), float(k[1])
     job.energy_rescale, job.energy_shift = 1.0, 0.0
     dev.assemble(job, pattern)
     lo, hi = dev.spectral_bounds()

@@ -305,3 +305,22 @@ def test_trilayer_supercell_value(oracle_mod):
     assert od.opts["supercell_type"] == 3
     with pytest.raises(oracle_mod.DeckError):
         oracle_mod.setup_supercell(od)
+
+
+def test_small_case_decks():
+    """The two forms of the 8,128-orbital 5.086 deg case: the periodic (7, 6) x 4
+    x 4 twin has exactly 8,128 orbitals at the twist setupSupercell gives the
+    (28, 24) cell, and the open disk flake at that twist has 8,108."""
+    from leo2d_amd import workloads as W
+
+    # textbook: cos theta = (N^2 + 4NM + M^2) / (2 (N^2 + NM + M^2)) = 253/254 for (7, 6)
+    assert abs(W.theta_mn(7, 6) - math.degrees(math.acos(253.0 / 254.0))) <= 1e-12
+    assert abs(W.theta_mn(7, 6) - 5.0858) <= 1e-4 and abs(W.theta_mn(32, 31) - 1.0501) <= 1e-4
+    assert W.theta_mn(6, 7) == -W.theta_mn(7, 6)
+    twin = L.Geometry(L.Deck(W.c0_twin_deck()))
+    assert twin.n == 8128 and twin.info.boundary_condition == 1
+    assert abs(twin.info.angle[1] - math.radians(W.theta_mn(7, 6))) <= 1e-14
+    flake = L.Geometry(L.Deck(W.c0_flake_deck()))
+    assert flake.n == 8108 and flake.info.boundary_condition == 0
+    assert abs(flake.info.angle[1] - math.radians(W.theta_mn(7, 6))) <= 1e-14
+    assert list(flake.info.sheet_count)[:2] == [4054, 4054]
