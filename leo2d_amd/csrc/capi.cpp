@@ -1384,7 +1384,11 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 // one workgroup per CU with two LDS buffers of kSellUcap x 256 B; tiles of at most
                 // tmax rows whose union fits ucap staged rows
                 const int ucap = std::max(32, std::min(leo2d::kSellUcap, env_int("LEO2D_SELL_UCAP", leo2d::kSellUcap)));
-                const int tmax = std::max(4, std::min(leo2d::kSellMaxRows, env_int("LEO2D_SELL_TMAX", leo2d::kSellMaxRows)));
+                // a copy wavefront beside seven compute wavefronts (tiles of at most 56 rows), or
+                // eight wavefronts that each issue their share of the copies
+                const bool copy_wave = env_int("LEO2D_SELL_CW", 1) != 0;
+                const int tlim = copy_wave ? leo2d::kSellCwRows : leo2d::kSellMaxRows;
+                const int tmax = std::max(4, std::min(tlim, env_int("LEO2D_SELL_TMAX", tlim)));
                 if (!spatial || !leo2d::sell_supported(mat_type, vec_type, B) ||
                     !build_sell(h, drp, dcol, ucap, tmax, spatial)) {
                     sa.variant = B == 16 || B == 32 ? 14 : (mixed ? 15 : 1);
@@ -1416,6 +1420,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                     ss.sliced = env_int("LEO2D_SELL_SLICED", 0);
                     ss.ucap = S.ucap;
                     ss.tile_part = 0;
+                    ss.copy_wave = copy_wave ? 1 : 0;
                 }
             }
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;

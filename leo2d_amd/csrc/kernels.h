@@ -52,6 +52,8 @@ constexpr int kSellMaxLen = 64;     // longest row a group can hold
 constexpr int kSellUcap = 288;      // staged rows per LDS buffer (two buffers of 288 x 256 B)
 constexpr int kSellUcapTiled = 256; // the same with per-tile partials (their LDS takes the difference)
 constexpr int kSellWaves = 8;       // wavefronts per workgroup (one workgroup per CU)
+constexpr int kSellCwWaves = 7;     // compute wavefronts beside the copy wavefront (copy_wave)
+constexpr int kSellCwRows = 56;     // rows per tile with a copy wavefront (7 wavefronts x 2 groups x 4 rows)
 struct SellStepArgs {
     int n;
     int R;
@@ -79,6 +81,7 @@ struct SellStepArgs {
     int max_len;                // most steps in a group (<= kSellMaxLen)
     int sliced;                 // 1: XCD x walks its own contiguous eighth of the tiles
     int reverse;                // 1: every workgroup walks its tiles last to first (alternate steps)
+    int copy_wave;              // 1: staging copies issued by a wavefront of their own (tiles <= kSellCwRows)
 };
 bool sell_supported(int mat_type, int vec_type, int R);
 hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s);

@@ -540,9 +540,313 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
     }
 }
 
+// The same step with the staging copies issued by a wavefront of their own.
+//
+// NWC compute wavefronts and one copy wavefront per workgroup.  Each wavefront
+// has its own memory counters, so the compute wavefronts never have a direct
+// global->LDS copy outstanding: with one outstanding, the compiler's wait
+// insertion counts it as a possible LDS access of unknown order and turns every
+// wait for an LDS read into a wait for all of them (lgkmcnt(0)), which drains
+// the next block's operand reads before each block's products.  Without it,
+// each block waits for its own four reads only.
+//
+// Pair p = (tile, chunk) of the walk, one barrier per pair:
+//   copy wavefront:    barrier p; copies of pair p+1 into the buffer pair p-1
+//                      used; union rows of the next tile; wait for all of it;
+//   compute wavefront: barrier p; T_{j-1} (and v) of the pair's rows as register
+//                      loads; products from the buffer; outputs; the next tile's
+//                      groups, then the stores.
+// Both walk the same pairs, so they meet at the same barriers.  Tiles hold at
+// most 4 GPW NWC rows (GPW groups of four rows per compute wavefront).
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NWC, int UC>
+__global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(SellStepArgs a)
+{
+    using AT = typename acc_of<XT>::type;
+    constexpr int C = 16 * VPL;
+    constexpr int R = C * NCH;
+    constexpr int NCP = UC / 4;   // copies of 4 union rows (1 KiB) per pair, at most
+    constexpr int NT = 64 * (NWC + 1);
+    __shared__ __align__(16) XT xs0[UC * C];
+    __shared__ __align__(16) XT xs1[UC * C];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = lane >> 4;
+    const int k = lane & 15;
+    const int koff = k * VPL;
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * (G >> 3) + (b >> 3);
+    int t_first, t_end, t_stride;
+    if (a.sliced) {
+        const int x = b & 7;
+        t_first = (int)((int64_t)a.ntiles * x / 8) + (b >> 3);
+        t_end = (int)((int64_t)a.ntiles * (x + 1) / 8);
+        t_stride = G >> 3;
+    } else {
+        t_first = lb;
+        t_end = a.ntiles;
+        t_stride = G;
+    }
+    if (a.reverse && t_first < t_end) {
+        const int last = t_first + (t_end - 1 - t_first) / t_stride * t_stride;
+        t_end = t_first - 1;
+        t_first = last;
+        t_stride = -t_stride;
+    }
+    auto live = [&](int t) { return t_stride > 0 ? t < t_end : t > t_end; };
+    const XT* __restrict__ cur = static_cast<const XT*>(a.cur);
+
+    if (wave == NWC) {
+        // ---- copy wavefront ----------------------------------------------------
+        // byte offset of lane (s, k)'s 16 bytes of union row u within a chunk: the
+        // launcher checks n R sizeof(XT) < 2^32
+        uint32_t uo[NCP], un[NCP];
+        int U = 0, Un = 0;
+        auto load_union = [&](int t, uint32_t (&dst)[NCP], int& cnt) {
+            const int64_t u0 = ldu(a.tile_u0, t);
+            cnt = (int)(ldu(a.tile_u0, t + 1) - u0);
+#pragma unroll
+            for (int m = 0; m < NCP; ++m) {
+                const int u = 4 * m + s;
+                const int row = a.ucols[u0 + (u < cnt ? u : 0)];
+                dst[m] = (uint32_t)row * (uint32_t)(R * sizeof(XT)) + (uint32_t)(koff * sizeof(XT));
+            }
+        };
+        auto copies = [&](XT* buf, int c) {
+            const char* base = reinterpret_cast<const char*>(cur + c * C);
+            auto lbuf = (__attribute__((address_space(3))) XT*)buf;
+#pragma unroll
+            for (int m = 0; m < NCP; ++m) {
+                if (4 * m < U)
+                    __builtin_amdgcn_global_load_lds((const void*)(base + uo[m]),
+                                                     (__attribute__((address_space(3))) void*)(lbuf + 4 * m * C), 16, 0,
+                                                     0);
+            }
+        };
+        int t = t_first, c = 0;
+        if (live(t)) {
+            load_union(t, uo, U);
+            copies(xs0, 0);
+            if (live(t + t_stride)) load_union(t + t_stride, un, Un);
+            wait_vm<0>();
+        }
+        auto pair = [&](XT* nbuf) {
+            __syncthreads();
+            int t1 = t, c1 = c + 1;
+            if (c1 == NCH) {
+                c1 = 0;
+                t1 = t + t_stride;
+            }
+            if (live(t1)) {
+                if (c1 == 0) {
+#pragma unroll
+                    for (int m = 0; m < NCP; ++m) uo[m] = un[m];
+                    U = Un;
+                }
+                copies(nbuf, c1);
+                if (c1 == 0 && live(t1 + t_stride)) load_union(t1 + t_stride, un, Un);
+            }
+            wait_vm<0>();
+            t = t1;
+            c = c1;
+        };
+        while (live(t)) {
+            pair(xs1);
+            if (!live(t)) break;
+            pair(xs0);
+        }
+        __syncthreads();
+        __syncthreads();
+    } else {
+        // ---- compute wavefronts ------------------------------------------------
+        const XT* prev = static_cast<const XT*>(a.prev);
+        XT* out = static_cast<XT*>(a.out);
+        const XT* __restrict__ vin = static_cast<const XT*>(a.v);
+        const HT* __restrict__ ev = static_cast<const HT*>(a.ev);
+        const uint16_t* __restrict__ eu = a.eu;
+        const double alpha = a.alpha;
+
+        double d1[NCH][VPL], d2[NCH][VPL], d3[NCH][VPL];
+#pragma unroll
+        for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                d1[cc][v] = 0.0;
+                d2[cc][v] = 0.0;
+                d3[cc][v] = 0.0;
+            }
+        auto load_groups = [&](int t, GroupRegs<HT, QS> (&gr)[GPW], int (&grow)[GPW], int (&glen)[GPW]) {
+            const int g0 = ldu(a.tile_g0, t), g1 = ldu(a.tile_g0, t + 1);
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                const int g = g0 + wave + NWC * j;
+                const bool lv = g < g1;
+                const int L = lv ? ldu(a.glen, g) : 0;
+                const int64_t go = lv ? ldu(a.goff, g) : 0;
+                glen[j] = L;
+                grow[j] = lv ? a.grow[4 * g + s] : -1;
+#pragma unroll
+                for (int q = 0; q < QS; ++q) {
+                    const int e = 16 * q + k;
+                    const bool ok = e < L;
+                    gr[j].eu[q] = ok ? (int)eu[go + 4 * e + s] : 0;
+                    if (ok) gr[j].ev[q] = ev[go + 4 * e + s];
+                    else zero(gr[j].ev[q]);
+                }
+            }
+        };
+        GroupRegs<HT, QS> ec[GPW], en[GPW];
+        int gc_row[GPW], gn_row[GPW], gc_len[GPW], gn_len[GPW];
+        int t = t_first, c = 0, p = 0;
+        int r0 = 0, r0n = 0;
+        if (live(t)) {
+            load_groups(t, ec, gc_row, gc_len);
+            r0 = ldu(a.tile_row0, t);
+        }
+        auto pair = [&](const XT* buf) {
+            __syncthreads();
+            // opaque per pair: keeps the compiler from specialising the whole walk on them
+            int bflag = __builtin_amdgcn_readfirstlane(a.beta != 0 ? 1 : 0);
+            int vflag = __builtin_amdgcn_readfirstlane(vin != nullptr ? 1 : 0);
+            asm volatile("" : "+s"(bflag), "+s"(vflag));
+            const bool beta = bflag != 0, has_v = vflag != 0;
+            if (c == 0 && p > 0) {
+#pragma unroll
+                for (int j = 0; j < GPW; ++j) {
+                    ec[j] = en[j];
+                    gc_row[j] = gn_row[j];
+                    gc_len[j] = gn_len[j];
+                }
+                r0 = r0n;
+            }
+            XT pv[GPW][VPL], vv[GPW][VPL];
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                const int row = gc_row[j];
+                const size_t idx = (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    if (beta && row >= 0) pv[j][v] = prev[idx + v];
+                    else zero(pv[j][v]);
+                    if (has_v && row >= 0) vv[j][v] = vin[idx + v];
+                    else zero(vv[j][v]);
+                }
+            }
+            AT acc[GPW][VPL];
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) zero(acc[j][v]);
+                int L = __builtin_amdgcn_readfirstlane(gc_len[j]);
+                asm volatile("" : "+s"(L));
+                if (L > 0) {
+                    XT xa[4][VPL];
+#pragma unroll
+                    for (int q = 0; q < QS; ++q) pin_values(ec[j].ev[q]);
+                    ld_block<0, HT, XT, VPL, QS>(xa, ec[j], buf, koff);
+                    run_blocks<0, HT, XT, AT, VPL, QS>(acc[j], ec[j], buf, koff, L, xa);
+                }
+            }
+            XT o[GPW][VPL];
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                const int row = gc_row[j];
+                XT cv[VPL];
+                lds_ld<XT, VPL>(cv, buf + (row < 0 ? 0 : row - r0) * C + koff);
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    o[j][v] = comb(alpha, acc[j][v], pv[j][v]);
+                    const double t1v = row >= 0 ? nrm2(cv[v]) : 0.0;
+                    const double t2v = row >= 0 ? redot(o[j][v], cv[v]) : 0.0;
+                    const double t3v = has_v && row >= 0 ? redot(vv[j][v], cv[v]) : 0.0;
+#pragma unroll
+                    for (int cc = 0; cc < NCH; ++cc) {
+                        if (cc == c) {
+                            d1[cc][v] += t1v;
+                            d2[cc][v] += t2v;
+                            d3[cc][v] += t3v;
+                        }
+                    }
+                }
+            }
+            int t1 = t, c1 = c + 1;
+            if (c1 == NCH) {
+                c1 = 0;
+                t1 = t + t_stride;
+            }
+            // the next tile's groups before this pair's stores: waiting for them then
+            // leaves the stores in flight
+            if (c1 == 0 && live(t1)) {
+                load_groups(t1, en, gn_row, gn_len);
+                r0n = ldu(a.tile_row0, t1);
+            }
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                const int row = gc_row[j];
+                if (row >= 0) {
+                    const size_t idx = (size_t)row * R + c * C + koff;
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) out[idx + v] = o[j][v];
+                }
+            }
+            t = t1;
+            c = c1;
+            ++p;
+        };
+        while (live(t)) {
+            pair(xs0);
+            if (!live(t)) break;
+            pair(xs1);
+        }
+        // every wavefront is past its last read of the buffers: the partials go into xs0
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                d1[cc][v] += __shfl_xor(d1[cc][v], 16);
+                d2[cc][v] += __shfl_xor(d2[cc][v], 16);
+                d3[cc][v] += __shfl_xor(d3[cc][v], 16);
+                d1[cc][v] += __shfl_xor(d1[cc][v], 32);
+                d2[cc][v] += __shfl_xor(d2[cc][v], 32);
+                d3[cc][v] += __shfl_xor(d3[cc][v], 32);
+            }
+        double* red = reinterpret_cast<double*>(xs0);   // [compute waves][R][3]
+        if (s == 0) {
+#pragma unroll
+            for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    const int r = cc * C + koff + v;
+                    red[(wave * R + r) * 3 + 0] = d1[cc][v];
+                    red[(wave * R + r) * 3 + 1] = d2[cc][v];
+                    red[(wave * R + r) * 3 + 2] = d3[cc][v];
+                }
+        }
+        __syncthreads();
+    }
+    const double* red = reinterpret_cast<const double*>(xs0);
+    for (int i = threadIdx.x; i < R * 3; i += NT) {
+        double sum = red[i];
+#pragma unroll
+        for (int w = 1; w < NWC; ++w) sum += red[w * R * 3 + i];
+        a.part[(size_t)lb * R * 3 + i] = sum;
+    }
+}
+
 template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NW>
 hipError_t launch_sell(const SellStepArgs& a, hipStream_t s)
 {
+    if (a.copy_wave) {
+        // one copy wavefront beside kSellCwWaves compute wavefronts; 32-bit copy offsets
+        constexpr int NWC = kSellCwWaves;
+        if (a.tile_part || a.tile_rows > 4 * GPW * NWC || a.ucap > kSellUcap ||
+            (uint64_t)a.n * (uint64_t)a.R * sizeof(XT) >= (uint64_t(1) << 32))
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, XT, VPL, NCH, GPW, QS, NWC, kSellUcap>), dim3(a.grid),
+                           dim3(64 * (NWC + 1)), 0, s, a);
+        return hipGetLastError();
+    }
     constexpr int UC = kSellUcap, UCT = kSellUcapTiled;
     if (a.tile_part) {
         if (a.v || a.ucap > UCT) return hipErrorInvalidValue;

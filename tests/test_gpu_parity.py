@@ -467,15 +467,19 @@ def test_xcd_slice_schedule(dev, oracle_mod, monkeypatch, variant, R):
 @pytest.mark.parametrize("cfg", [("288", "1", "64"), ("64", "0", "64"), ("128", "1", "13"), ("288", "0", "5"),
                                  ("200", "1", "48")])
 @pytest.mark.parametrize("R", [16, 32, 64, 128])
-def test_sell_variant_shapes(dev, oracle_mod, monkeypatch, cfg, R):
-    """SELL-4-sigma step with LDS-staged tiles (variant 20) under LDS budgets
-    of 64 to 288 staged rows (tiles cut short by the budget), tiles of at most
-    5 to 64 rows (partial groups, idle wavefronts) and both tile schedules
-    (XCD slices, interleaved), walked forwards and backwards on alternate
-    steps: FP64 moments within 1e-12 mu_0 of the oracle in both schemes, FP32
-    recursion within 2e-5 mu_0, complex H(k) and a real H.  R = 128 falls back
-    to the CSR step."""
+@pytest.mark.parametrize("cw", ["1", "0"])
+def test_sell_variant_shapes(dev, oracle_mod, monkeypatch, cfg, R, cw):
+    """SELL-4-sigma step with LDS-staged tiles (variant 20), its staging copies
+    issued by a wavefront of their own (cw = 1, tiles of at most 56 rows) or by
+    every wavefront (cw = 0), under LDS budgets of 64 to 288 staged rows (tiles
+    cut short by the budget), tiles of at most 5 to 64 rows (partial groups,
+    idle wavefronts) and both tile schedules (XCD slices, interleaved), walked
+    forwards and backwards on alternate steps: FP64 moments within 1e-12 mu_0
+    of the oracle in both schemes, FP32 recursion within 2e-5 mu_0, complex
+    H(k) and a real H.  R = 128 falls back to the CSR step."""
     ucap, sliced, tmax = cfg
+    print("case", cfg, R, "copy wave", cw, flush=True)
+    monkeypatch.setenv("LEO2D_SELL_CW", cw)
     monkeypatch.setenv("LEO2D_KPM_VARIANT", "20")
     monkeypatch.setenv("LEO2D_SELL_UCAP", ucap)
     monkeypatch.setenv("LEO2D_SELL_SLICED", sliced)
