@@ -767,6 +767,8 @@ def kernel_name(variant, t):
         return f"kpm_step_xchunk_kernel<{t},{t},4>"
     if variant == 20:
         return f"kpm_step_sell_kernel<{t},{t},{1 if t == 'double2' else 2},{4 if t == 'double2' else 2}>"
+    if variant == 21:
+        return f"kpm_step_sell_cw_kernel<{t},{t},{1 if t == 'double2' else 2},{4 if t == 'double2' else 2}>"
     return f"kpm step variant {variant}"
 
 

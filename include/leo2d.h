@@ -184,7 +184,9 @@ typedef struct {
     int32_t n, R, M;
     int32_t grid;
     int32_t kernel;   /* step-kernel variant launched for the last block (0 = lane per vector,
-                         1 = wavefront per row, 8 = wavefront per row, matrix via scalar loads) */
+                         1 = wavefront per row, 14 = vector chunks split over the XCDs, 15 = two
+                         FP32 rows per wavefront, 20 = SELL-4-sigma LDS tiles, 21 = the same with
+                         a copy wavefront) */
     int32_t reserved;
 } leo2d_timing;
 

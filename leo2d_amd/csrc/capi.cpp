@@ -1426,7 +1426,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             if (sa.variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) sa.variant = 1;
             if (sa.variant == 15 && (!spatial || B != 64 || vec_type != leo2d::VEC_CPX_F32)) sa.variant = 1;
             if (B < 64 && sa.variant != 14 && sa.variant != kVariantSell) sa.variant = 0;
-            kernel_used = sa.variant;
+            kernel_used = sa.variant == kVariantSell && ss.copy_wave ? kVariantSell + 1 : sa.variant;
             leo2d::ReduceArgs ra{};
             ra.part = (const double*)part.p;
             ra.grid = sa.grid;
