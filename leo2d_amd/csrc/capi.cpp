@@ -173,11 +173,12 @@ struct leo2d_handle {
             int M, R, precision, mode, tile_rows, grid;
             uint64_t seed;
             const void *A, *B, *part, *mu, *val, *targets;
+            int copy_wave;
             bool operator==(const GraphKey& o) const
             {
                 return M == o.M && R == o.R && precision == o.precision && mode == o.mode && tile_rows == o.tile_rows &&
                        grid == o.grid && seed == o.seed && A == o.A && B == o.B && part == o.part && mu == o.mu &&
-                       val == o.val && targets == o.targets;
+                       val == o.val && targets == o.targets && copy_wave == o.copy_wave;
             }
         } key{};
         DevBuf A, B, part, mu, tgt;
@@ -1956,11 +1957,15 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
                 dalloc(b, bytes, what);
             }
         };
-        // SELL tiles of the stack (complex vectors, R = 16 or 32; FP32 only at 32)
+        // SELL tiles of the stack (complex vectors, R = 16 or 32; FP32 only at 32): tiles of at most
+        // 56 rows for the kernel with a copy wavefront, else 64 rows whose union fits the LDS the
+        // eight-wavefront kernel leaves beside its T_{j-1} copies and partials
+        const bool sell_cw = env_int("LEO2D_SELL_CW", 1) != 0;
         const bool sell_ok = leo2d::sell_supported(mat_type, vec_type, R) && R <= 32 &&
                              build_sell_tiles(B.sell, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
-                                              (const int32_t*)B.col.p, leo2d::kSellUcapTiled, leo2d::kSellMaxRows, false,
-                                              &B.job_row0);
+                                              (const int32_t*)B.col.p,
+                                              sell_cw ? leo2d::kSellUcap : leo2d::kSellUcapTiled,
+                                              sell_cw ? leo2d::kSellCwRows : leo2d::kSellMaxRows, false, &B.job_row0);
         if (!p->tune && p->mode == LEO2D_BATCH_SELL && !sell_ok)
             throw ArgError("LEO2D_BATCH_SELL needs complex vectors with R = 16 or 32 (32 for FP32) and rows of at most "
                            "64 entries");
@@ -1995,6 +2000,7 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
             ss.part = (double*)B.part.p;
             ss.grid = std::max(8, cus / 8 * 8);
             ss.tile_part = 1;
+            ss.copy_wave = sell_cw ? 1 : 0;
             ss.ucap = S.ucap;
             ss.ntiles = S.ntiles;
             ss.tile_row0 = (const int32_t*)S.row0.p;
@@ -2132,7 +2138,8 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         if (p->graph) {
             const leo2d_handle::Batch::GraphKey key{M, R, p->precision, mode, T, grid, p->seed,
                                                      B.A.p, B.B.p, B.part.p, B.mu.p,
-                                                     mode == LEO2D_BATCH_SELL ? ss.ev : dval, dtargets};
+                                                     mode == LEO2D_BATCH_SELL ? ss.ev : dval, dtargets,
+                                                     mode == LEO2D_BATCH_SELL ? ss.copy_wave : 0};
             if (!B.exec || !(key == B.key)) {
                 B.drop_graph();
                 hipGraph_t graph = nullptr;

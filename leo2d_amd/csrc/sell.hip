@@ -558,7 +558,8 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
 //                      groups, then the stores.
 // Both walk the same pairs, so they meet at the same barriers.  Tiles hold at
 // most 4 GPW NWC rows (GPW groups of four rows per compute wavefront).
-template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NWC, int UC>
+// PT: partial sums per tile (part[tile][R][2], stacked jobs) as in the kernel above.
+template <typename HT, typename XT, int VPL, int NCH, int GPW, int QS, int NWC, int UC, bool PT>
 __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(SellStepArgs a)
 {
     using AT = typename acc_of<XT>::type;
@@ -568,6 +569,8 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
     constexpr int NT = 64 * (NWC + 1);
     __shared__ __align__(16) XT xs0[UC * C];
     __shared__ __align__(16) XT xs1[UC * C];
+    // per-tile partials of each compute wavefront, two sets used by alternate tiles
+    __shared__ double tred[PT ? 2 * NWC * R * 2 : 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = lane >> 4;
@@ -656,6 +659,7 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
             pair(xs0);
         }
         __syncthreads();
+        if constexpr (PT) return;
         __syncthreads();
     } else {
         // ---- compute wavefronts ------------------------------------------------
@@ -703,8 +707,22 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
             load_groups(t, ec, gc_row, gc_len);
             r0 = ldu(a.tile_row0, t);
         }
+        int pend_tile = -1, pend_set = 0, tset = 0;
+        // sum the compute wavefronts' partials of the tile finished in the previous pair
+        auto emit_tile = [&]() {
+            if (PT && pend_tile >= 0) {
+                for (int i = threadIdx.x; i < 2 * R; i += 64 * NWC) {
+                    double sum = tred[(pend_set * NWC) * R * 2 + i];
+#pragma unroll
+                    for (int w = 1; w < NWC; ++w) sum += tred[(pend_set * NWC + w) * R * 2 + i];
+                    a.part[(size_t)pend_tile * R * 2 + i] = sum;
+                }
+                pend_tile = -1;
+            }
+        };
         auto pair = [&](const XT* buf) {
             __syncthreads();
+            emit_tile();
             // opaque per pair: keeps the compiler from specialising the whole walk on them
             int bflag = __builtin_amdgcn_readfirstlane(a.beta != 0 ? 1 : 0);
             int vflag = __builtin_amdgcn_readfirstlane(vin != nullptr ? 1 : 0);
@@ -769,6 +787,29 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                     }
                 }
             }
+            if (PT && c == NCH - 1) {
+                // the tile's last chunk: fold its sums over the four DPP rows into this
+                // wavefront's slot of the tile partials, summed over the wavefronts next pair
+#pragma unroll
+                for (int cc = 0; cc < NCH; ++cc)
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        d1[cc][v] += __shfl_xor(d1[cc][v], 16);
+                        d2[cc][v] += __shfl_xor(d2[cc][v], 16);
+                        d1[cc][v] += __shfl_xor(d1[cc][v], 32);
+                        d2[cc][v] += __shfl_xor(d2[cc][v], 32);
+                        if (s == 0) {
+                            const int r = cc * C + koff + v;
+                            tred[((tset * NWC + wave) * R + r) * 2 + 0] = d1[cc][v];
+                            tred[((tset * NWC + wave) * R + r) * 2 + 1] = d2[cc][v];
+                        }
+                        d1[cc][v] = 0.0;
+                        d2[cc][v] = 0.0;
+                    }
+                pend_tile = t;
+                pend_set = tset;
+                tset ^= 1;
+            }
             int t1 = t, c1 = c + 1;
             if (c1 == NCH) {
                 c1 = 0;
@@ -800,6 +841,10 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
         }
         // every wavefront is past its last read of the buffers: the partials go into xs0
         __syncthreads();
+        if constexpr (PT) {
+            emit_tile();
+            return;
+        }
 #pragma unroll
         for (int cc = 0; cc < NCH; ++cc)
 #pragma unroll
@@ -840,11 +885,15 @@ hipError_t launch_sell(const SellStepArgs& a, hipStream_t s)
     if (a.copy_wave) {
         // one copy wavefront beside kSellCwWaves compute wavefronts; 32-bit copy offsets
         constexpr int NWC = kSellCwWaves;
-        if (a.tile_part || a.tile_rows > 4 * GPW * NWC || a.ucap > kSellUcap ||
-            (uint64_t)a.n * (uint64_t)a.R * sizeof(XT) >= (uint64_t(1) << 32))
+        if (a.tile_rows > 4 * GPW * NWC || a.ucap > kSellUcap ||
+            (uint64_t)a.n * (uint64_t)a.R * sizeof(XT) >= (uint64_t(1) << 32) || (a.tile_part && a.v))
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, XT, VPL, NCH, GPW, QS, NWC, kSellUcap>), dim3(a.grid),
-                           dim3(64 * (NWC + 1)), 0, s, a);
+        if (a.tile_part)
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, XT, VPL, NCH, GPW, QS, NWC, kSellUcap, true>), dim3(a.grid),
+                               dim3(64 * (NWC + 1)), 0, s, a);
+        else
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, XT, VPL, NCH, GPW, QS, NWC, kSellUcap, false>),
+                               dim3(a.grid), dim3(64 * (NWC + 1)), 0, s, a);
         return hipGetLastError();
     }
     constexpr int UC = kSellUcap, UCT = kSellUcapTiled;

@@ -54,18 +54,22 @@ def test_batch_fp64_matches_oracle(batch, mode, tile_rows, graph):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_sell_matches_oracle(batch, graph):
+@pytest.mark.parametrize("cw", ["1", "0"])
+def test_batch_sell_matches_oracle(batch, graph, cw, monkeypatch):
     """SELL-4-sigma tiles cut at the job boundaries (six jobs of 76 to 364
-    orbitals), partial sums per tile: FP64 moments within 1e-12 mu_0 of the
-    oracle for every job, eager and captured; the FP32 recursion needs 32
-    vectors per job for this layout (16-byte lanes of two vectors)."""
+    orbitals), partial sums per tile, staging copies by a wavefront of their
+    own (cw = 1, tiles of at most 56 rows) or by every wavefront (cw = 0):
+    FP64 moments within 1e-12 mu_0 of the oracle for every job, eager and
+    captured; the FP32 recursion needs 32 vectors per job for this layout
+    (16-byte lanes of two vectors)."""
+    monkeypatch.setenv("LEO2D_SELL_CW", cw)
     dev, jobs = batch
-    print("case SELL graph", graph, flush=True)
+    print("case SELL graph", graph, "copy wave", cw, flush=True)
     mu = dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, graph=graph)
     for p, (n, omu) in enumerate(jobs):
         assert np.max(np.abs(mu[p] - omu)) <= 1e-12 * n, (p, graph)
     t = dev.batch_timing()
-    assert t.mode == L.BATCH_SELL and 0 < t.tile_rows <= 64 and t.ntiles >= len(jobs)
+    assert t.mode == L.BATCH_SELL and 0 < t.tile_rows <= (56 if cw == "1" else 64) and t.ntiles >= len(jobs)
     with pytest.raises(L.Leo2dError) as e:
         dev.batch_kpm(M, R, seed=SEED, mode=L.BATCH_SELL, precision=L.PREC_MIXED)
     assert e.value.code == L.ERR_ARG
