@@ -51,19 +51,25 @@ struct NanError : std::runtime_error {
 // the SELL-4-sigma step with LDS-staged row tiles (sell.hip)
 constexpr int kVariantSell = 20;
 
-// step kernel for a block of B vectors (LEO2D_KPM_VARIANT = 0, 1, 14, 15 or 20
-// overrides the default).  In the spatial order: B >= 64 runs the SELL-4-sigma
-// step with LDS-staged row tiles (20); B = 16 or 32 vector chunks split over
-// the XCDs (14).  In the natural order (or where 20 does not apply): B >= 64
+// Below this many rows the SELL tiles (at most 56 rows each) are fewer than the
+// GPU's CUs, and a matrix assembled for one short run pays the host-side tile
+// build for little: the vector-chunk kernel runs those.
+constexpr int kSellMinRows = 16384;
+
+// step kernel for a block of B vectors on n rows (LEO2D_KPM_VARIANT = 0, 1, 14,
+// 15 or 20 overrides the default).  In the spatial order: B = 16, 32 or 64 run
+// the SELL-4-sigma step with LDS-staged row tiles (20) where its layout applies
+// (complex vectors) and n >= kSellMinRows, else B = 16 or 32 vector chunks split
+// over the XCDs (14).  In the natural order (or where neither applies): B >= 64
 // the wavefront-per-row kernel (1), or for the FP32 recursion at B = 64 two rows
 // per wavefront with 16-byte gathers (15); other B lane per vector (0).
-int kpm_variant(bool mixed, bool spatial, int B)
+int kpm_variant(bool mixed, bool spatial, int B, int n)
 {
     if (const char* e = std::getenv("LEO2D_KPM_VARIANT")) {
         const int v = std::atoi(e);
         if (v == 0 || v == 1 || v == 14 || v == 15 || v == kVariantSell) return v;
     }
-    if (spatial && (B == 16 || B == 32)) return 14;
+    if (spatial && (B == 16 || B == 32)) return n >= kSellMinRows ? kVariantSell : 14;
     if (spatial && B >= 64) return kVariantSell;
     return mixed && spatial ? 15 : 1;
 }
@@ -1377,7 +1383,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
             sa.part = (double*)part.p;
             sa.grid = G;
             sa.v = keep_v ? V.p : nullptr;
-            sa.variant = kpm_variant(mixed, spatial, B);
+            sa.variant = kpm_variant(mixed, spatial, B, n);
             sa.xcd_rows = kpm_sched() == 1 ? (n + 7) / 8 : 0;
             leo2d::SellStepArgs ss{};
             const bool sell_alternate = env_int("LEO2D_SELL_ALTERNATE", 1) != 0;
@@ -1390,7 +1396,7 @@ int leo2d_kpm_moments(leo2d_handle* h, const leo2d_kpm_params* p, double* mu)
                 const bool copy_wave = env_int("LEO2D_SELL_CW", 1) != 0;
                 const int tlim = copy_wave ? leo2d::kSellCwRows : leo2d::kSellMaxRows;
                 const int tmax = std::max(4, std::min(tlim, env_int("LEO2D_SELL_TMAX", tlim)));
-                if (!spatial || !leo2d::sell_supported(mat_type, vec_type, B) ||
+                if (!spatial || !leo2d::sell_supported(mat_type, vec_type, B, copy_wave) ||
                     !build_sell(h, drp, dcol, ucap, tmax, spatial)) {
                     sa.variant = B == 16 || B == 32 ? 14 : (mixed ? 15 : 1);
                 } else {
@@ -1554,7 +1560,7 @@ int leo2d_kpm_moments_noalloc(leo2d_handle* h, const leo2d_kpm_params* p, double
         int32_t* dtg = reinterpret_cast<int32_t*>(base + oTg);
 
         // the kernel leo2d_kpm_moments picks for the reference row order in FP64
-        int variant = kpm_variant(false, false, B);
+        int variant = kpm_variant(false, false, B, n);
         if (variant == kVariantSell) variant = B == 16 || B == 32 ? 14 : 1;
         if (variant == 14 && B != 16 && B != 32 && B != 64 && B != 128) variant = 1;
         if (variant == 15) variant = 1;
@@ -1961,14 +1967,14 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         // 56 rows for the kernel with a copy wavefront, else 64 rows whose union fits the LDS the
         // eight-wavefront kernel leaves beside its T_{j-1} copies and partials
         const bool sell_cw = env_int("LEO2D_SELL_CW", 1) != 0;
-        const bool sell_ok = leo2d::sell_supported(mat_type, vec_type, R) && R <= 32 &&
+        const bool sell_ok = leo2d::sell_supported(mat_type, vec_type, R, sell_cw) && R <= 32 &&
                              build_sell_tiles(B.sell, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
                                               (const int32_t*)B.col.p,
                                               sell_cw ? leo2d::kSellUcap : leo2d::kSellUcapTiled,
                                               sell_cw ? leo2d::kSellCwRows : leo2d::kSellMaxRows, false, &B.job_row0);
         if (!p->tune && p->mode == LEO2D_BATCH_SELL && !sell_ok)
-            throw ArgError("LEO2D_BATCH_SELL needs complex vectors with R = 16 or 32 (32 for FP32) and rows of at most "
-                           "64 entries");
+            throw ArgError("LEO2D_BATCH_SELL needs complex vectors with R = 16 or 32 (FP32 at 16 only with the copy "
+                           "wavefront) and rows of at most 64 entries");
         int64_t max_tiles = 0;   // tiles of the shortest allowed height bound every tiling's count
         for (int q = 0; q < P; ++q) max_tiles += (B.job_row0[q + 1] - B.job_row0[q] + kMinTile - 1) / kMinTile;
         if (sell_ok) max_tiles = std::max<int64_t>(max_tiles, B.sell.ntiles);

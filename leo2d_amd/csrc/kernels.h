@@ -83,7 +83,8 @@ struct SellStepArgs {
     int reverse;                // 1: every workgroup walks its tiles last to first (alternate steps)
     int copy_wave;              // 1: staging copies issued by a wavefront of their own (tiles <= kSellCwRows)
 };
-bool sell_supported(int mat_type, int vec_type, int R);
+// copy_wave: 16 complex FP32 vectors (128-byte row chunks) need the copy-wavefront kernel
+bool sell_supported(int mat_type, int vec_type, int R, bool copy_wave);
 hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s);
 // dst[i] = src[map[i]] (zero where map[i] < 0), elements of elem_bytes (4, 8 or 16)
 hipError_t launch_sell_fill(const int64_t* map, const void* src, void* dst, int elem_bytes, int64_t count,

@@ -117,6 +117,15 @@ template <int K> struct DppMac;
                 : "+v"(ax), "+v"(ay)                                                                               \
                 : "v"(hr), "v"(hi), "v"(xx), "v"(xy));                                                             \
         }                                                                                                          \
+        __device__ static inline void f32x1(float& ax, float& ay, float hr, float hi, float xx, float xy)         \
+        {                                                                                                          \
+            asm("v_fmac_f32_dpp %0, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %0, -%3, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                \
+                "v_fmac_f32_dpp %1, %2, %5 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f32_dpp %1, %3, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf"                       \
+                : "+v"(ax), "+v"(ay)                                                                               \
+                : "v"(hr), "v"(hi), "v"(xx), "v"(xy));                                                             \
+        }                                                                                                          \
         __device__ static inline void f32(float& a0x, float& a0y, float& a1x, float& a1y, float hr, float hi,    \
                                           float x0x, float x0y, float x1x, float x1y)                              \
         {                                                                                                          \
@@ -147,6 +156,9 @@ __device__ inline void mac_step(AT (&acc)[VPL], const XT (&x)[VPL], const GroupR
     } else if constexpr (std::is_same<HT, float2>::value && VPL == 2) {
         const float2 h = g.ev[E >> 4];
         DppMac<E & 15>::f32(acc[0].x, acc[0].y, acc[1].x, acc[1].y, h.x, h.y, x[0].x, x[0].y, x[1].x, x[1].y);
+    } else if constexpr (std::is_same<HT, float2>::value && std::is_same<XT, float2>::value && VPL == 1) {
+        const float2 h = g.ev[E >> 4];
+        DppMac<E & 15>::f32x1(acc[0].x, acc[0].y, h.x, h.y, x[0].x, x[0].y);
     } else {
         const HT h = bc16<E & 15>(g.ev[E >> 4]);
 #pragma unroll
@@ -565,7 +577,13 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
     using AT = typename acc_of<XT>::type;
     constexpr int C = 16 * VPL;
     constexpr int R = C * NCH;
-    constexpr int NCP = UC / 4;   // copies of 4 union rows (1 KiB) per pair, at most
+    // a staged row chunk is RB = 256 bytes (16 complex FP64 or 32 complex FP32 vectors) or
+    // 128 bytes (16 complex FP32 vectors); one 1 KiB copy moves RPI of them
+    constexpr int RB = C * (int)sizeof(XT);
+    static_assert(RB == 256 || RB == 128, "row chunks of 256 or 128 bytes");
+    constexpr int LPR = RB / 16;        // lanes per staged row chunk (16 bytes each)
+    constexpr int RPI = 64 / LPR;       // staged rows per copy instruction
+    constexpr int NCP = UC / RPI;       // copies per pair, at most
     constexpr int NT = 64 * (NWC + 1);
     __shared__ __align__(16) XT xs0[UC * C];
     __shared__ __align__(16) XT xs1[UC * C];
@@ -601,8 +619,8 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
 
     if (wave == NWC) {
         // ---- copy wavefront ----------------------------------------------------
-        // byte offset of lane (s, k)'s 16 bytes of union row u within a chunk: the
-        // launcher checks n R sizeof(XT) < 2^32
+        // byte offset of lane l's 16 bytes (piece l % LPR of union row RPI m + l / LPR)
+        // within a chunk: the launcher checks n R sizeof(XT) < 2^32
         uint32_t uo[NCP], un[NCP];
         int U = 0, Un = 0;
         auto load_union = [&](int t, uint32_t (&dst)[NCP], int& cnt) {
@@ -610,9 +628,9 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
             cnt = (int)(ldu(a.tile_u0, t + 1) - u0);
 #pragma unroll
             for (int m = 0; m < NCP; ++m) {
-                const int u = 4 * m + s;
+                const int u = RPI * m + lane / LPR;
                 const int row = a.ucols[u0 + (u < cnt ? u : 0)];
-                dst[m] = (uint32_t)row * (uint32_t)(R * sizeof(XT)) + (uint32_t)(koff * sizeof(XT));
+                dst[m] = (uint32_t)row * (uint32_t)(R * sizeof(XT)) + (uint32_t)(16 * (lane % LPR));
             }
         };
         auto copies = [&](XT* buf, int c) {
@@ -620,10 +638,10 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
             auto lbuf = (__attribute__((address_space(3))) XT*)buf;
 #pragma unroll
             for (int m = 0; m < NCP; ++m) {
-                if (4 * m < U)
+                if (RPI * m < U)
                     __builtin_amdgcn_global_load_lds((const void*)(base + uo[m]),
-                                                     (__attribute__((address_space(3))) void*)(lbuf + 4 * m * C), 16, 0,
-                                                     0);
+                                                     (__attribute__((address_space(3))) void*)(lbuf + RPI * m * C), 16,
+                                                     0, 0);
             }
         };
         int t = t_first, c = 0;
@@ -940,6 +958,30 @@ hipError_t launch_sell_shape(const SellStepArgs& a, hipStream_t s)
     return launch_sell_r<HT, XT, VPL, 2, 3>(a, s);
 }
 
+// 16 complex FP32 vectors: 128-byte row chunks (eight staged rows per copy), the
+// copy-wavefront kernel only
+template <typename HT>
+hipError_t launch_sell_f32x16(const SellStepArgs& a, hipStream_t s)
+{
+    constexpr int NWC = kSellCwWaves, GPW = 2, UC = kSellUcap;
+    if (!a.copy_wave || a.tile_rows > 4 * GPW * NWC || a.ucap > UC || (a.tile_part && a.v) ||
+        (uint64_t)a.n * 16 * sizeof(float2) >= (uint64_t(1) << 32))
+        return hipErrorInvalidValue;
+    const dim3 grid(a.grid), block(64 * (NWC + 1));
+    if (a.max_len > 48) {
+        if (a.tile_part)
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, float2, 1, 1, GPW, 4, NWC, UC, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, float2, 1, 1, GPW, 4, NWC, UC, false>), grid, block, 0, s, a);
+    } else {
+        if (a.tile_part)
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, float2, 1, 1, GPW, 3, NWC, UC, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((kpm_step_sell_cw_kernel<HT, float2, 1, 1, GPW, 3, NWC, UC, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
 // dst[i] = src[map[i]] (0 where map[i] < 0): the SELL values from the CSR values
 template <typename E>
 __global__ void sell_fill_kernel(const int64_t* __restrict__ map, const E* __restrict__ src, E* __restrict__ dst,
@@ -956,24 +998,26 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ map, const E* __res
 
 }  // namespace
 
-bool sell_supported(int mat_type, int vec_type, int R)
+bool sell_supported(int mat_type, int vec_type, int R, bool copy_wave)
 {
     const bool cpx64 = vec_type == VEC_CPX_F64 && (mat_type == MAT_CPX_F64 || mat_type == MAT_REAL_F64);
     const bool cpx32 = vec_type == VEC_CPX_F32 && (mat_type == MAT_CPX_F32 || mat_type == MAT_REAL_F32);
     if (cpx64) return R == 16 || R == 32 || R == 64;
-    if (cpx32) return R == 32 || R == 64;
+    if (cpx32) return R == 32 || R == 64 || (copy_wave && R == 16);
     return false;
 }
 
 hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& a, hipStream_t s)
 {
-    if (a.grid % 8 != 0 || a.grid <= 0 || !sell_supported(mat_type, vec_type, a.R) || a.tile_rows > kSellMaxRows ||
-        a.max_len > kSellMaxLen)
+    if (a.grid % 8 != 0 || a.grid <= 0 || !sell_supported(mat_type, vec_type, a.R, a.copy_wave != 0) ||
+        a.tile_rows > kSellMaxRows || a.max_len > kSellMaxLen)
         return hipErrorInvalidValue;
     if (mat_type == MAT_CPX_F64) return launch_sell_shape<double2, double2, 1>(a, s);
     if (mat_type == MAT_REAL_F64) return launch_sell_shape<double, double2, 1>(a, s);
-    if (mat_type == MAT_CPX_F32) return launch_sell_shape<float2, float2, 2>(a, s);
-    if (mat_type == MAT_REAL_F32) return launch_sell_shape<float, float2, 2>(a, s);
+    if (mat_type == MAT_CPX_F32)
+        return a.R == 16 ? launch_sell_f32x16<float2>(a, s) : launch_sell_shape<float2, float2, 2>(a, s);
+    if (mat_type == MAT_REAL_F32)
+        return a.R == 16 ? launch_sell_f32x16<float>(a, s) : launch_sell_shape<float, float2, 2>(a, s);
     return hipErrorInvalidValue;
 }
 
