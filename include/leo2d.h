@@ -123,6 +123,11 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* g);
  * same per-row order (stable column sort for periodic decks), so the CSR
  * pattern is bit-identical to the reference's.  Counts returned (may be NULL). */
 int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64_t* n_slots);
+/* How the last leo2d_pairs_build wrote the lists: 1 = one wavefront per row with
+ * the row's lists staged in LDS, 0 = one thread per row through a scratch copy
+ * (rows longer than the LDS lists hold, or LEO2D_PAIRS_STAGED=0).  No reference
+ * counterpart: a diagnostic of this implementation. */
+int leo2d_pairs_method(leo2d_handle* h, int32_t* staged);
 /* Copy a pair list back: which 0 = intra, 1 = inter; ij int32[2P], sc int32[2P],
  * t double[P] (intra couplings, may be NULL). */
 int leo2d_pairs_copy(leo2d_handle* h, int32_t which, int32_t* ij, int32_t* sc, double* t);

@@ -47,7 +47,7 @@ EXPORTS = [
     "leo2d_deck_parse", "leo2d_deck_free", "leo2d_deck_get_int", "leo2d_deck_get_double",
     "leo2d_deck_get_string", "leo2d_geom_build", "leo2d_geom_free", "leo2d_geom_get_info",
     "leo2d_geom_arrays", "leo2d_jobs_count", "leo2d_job_get", "leo2d_create", "leo2d_destroy",
-    "leo2d_last_error", "leo2d_set_geometry", "leo2d_pairs_build", "leo2d_pairs_copy",
+    "leo2d_last_error", "leo2d_set_geometry", "leo2d_pairs_build", "leo2d_pairs_method", "leo2d_pairs_copy",
     "leo2d_assemble", "leo2d_csr_upload", "leo2d_csr_info", "leo2d_csr_copy",
     "leo2d_spectral_bounds", "leo2d_kpm_moments", "leo2d_kpm_moments_noalloc", "leo2d_last_timing", "leo2d_dos",
     "leo2d_run_deck", "leo2d_batch_clear", "leo2d_batch_append", "leo2d_batch_info", "leo2d_batch_kpm",
@@ -172,6 +172,7 @@ def lib() -> C.CDLL:
         L.leo2d_last_error.restype = C.c_char_p
         L.leo2d_set_geometry.argtypes = [vp, vp]
         L.leo2d_pairs_build.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.leo2d_pairs_method.argtypes = [vp, C.POINTER(C.c_int32)]
         L.leo2d_pairs_copy.argtypes = [vp, C.c_int32, vp, vp, vp]
         L.leo2d_assemble.argtypes = [vp, C.POINTER(Job), C.c_int32, C.POINTER(C.c_int64)]
         L.leo2d_csr_upload.argtypes = [vp, C.c_int32, C.c_int64, vp, vp, vp, C.c_int32]
@@ -318,6 +319,13 @@ class Device:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         self._check(lib().leo2d_pairs_build(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def pairs_method(self) -> int:
+        """1 when the last pairs_build staged each row's lists in LDS (one
+        wavefront per row), 0 for the thread-per-row kernels."""
+        v = C.c_int32()
+        self._check(lib().leo2d_pairs_method(self.h, C.byref(v)))
+        return v.value
 
     def pairs_copy(self, which: int, count: int):
         ij = np.zeros(2 * max(count, 1), dtype=np.int32)

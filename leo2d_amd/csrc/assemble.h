@@ -57,10 +57,17 @@ struct JobDev {
     double basis_cs[2];                  // cos, sin of the angle of b1 (host libm)
 };
 
+// pair counts per row (one wavefront per row) and their exclusive scan
 hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total, int64_t* pair_rp, hipStream_t s);
-// tcol/tsc: scratch of the same size as pcol/psc, holding the pairs as found
+// the pair lists, one thread per row; tcol/tsc: scratch of the same size as pcol/psc,
+// holding the pairs as found
 hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* tcol, int32_t* tsc,
                        int32_t* pcol, int32_t* psc, int64_t* nslots, int64_t* slot_rp, hipStream_t s);
+// the same lists, one wavefront per row with the lists staged in LDS (no scratch); only
+// when pairs_staged_fit holds for the longest intralayer and interlayer lists
+bool pairs_staged_fit(const GeomDev& g, int max_intra, int max_inter);
+hipError_t pairs_write_staged(const GeomDev& g, const int64_t* pair_rp, int32_t* pcol, int32_t* psc, int64_t* nslots,
+                              int64_t* slot_rp, hipStream_t s);
 hipError_t job_positions(const GeomDev& g, const JobDev& j, double* i2pos, hipStream_t s);
 // each orbital's local twist from its nearest neighbour on positions i2pos
 hipError_t local_theta(const GeomDev& g, const double* i2pos, double* theta, hipStream_t s);

@@ -1,6 +1,6 @@
 // assemble.hip -- GPU pair search and per-job Hamiltonian fill.
 //
-// Pair search: one thread per orbital row restates, candidate for candidate,
+// Pair search: every orbital row restates, candidate for candidate,
 //   * Sheet::getIntraPairs (sheet.cpp:624-765): 8x8 unit-cell window around
 //     the orbital's own cell in every periodic image; a candidate is kept when
 //     its tabulated hopping is non-zero or it lies in the home image;
@@ -11,9 +11,11 @@
 // (Sheet::orderPairs sheet.cpp:810-849, Hstruct::orderPairs hstruct.cpp:634-666).
 // Geometry arithmetic is compiled without FMA contraction and the rotation
 // cosines come from the host's libm, so every keep/reject decision and window
-// origin matches the reference bit for bit.  Two passes (count, then write)
-// with a device scan in between; the sort is an in-thread insertion sort
-// (rows are concatenations of per-image sorted runs, so it is near-linear).
+// origin matches the reference bit for bit.  Two passes with a device scan in
+// between: the counts, one thread per row; then the lists, one wavefront per row
+// with the row's lists staged in LDS and the per-image sorted runs merged there
+// (rows whose lists do not fit the LDS use the thread-per-row write pass, which
+// merges through a scratch copy in global memory).
 //
 // Fill: one thread per row walks its pairs in the reference order and
 // accumulates duplicate (row, col) pairs into one slot exactly as
@@ -387,6 +389,271 @@ __global__ __launch_bounds__(kThreads) void pair_write_kernel(GeomDev g, const i
     nslots[kh] = distinct_runs(pcol + base, ni) + distinct_runs(pcol + base + ni, nt - ni);
 }
 
+// ---- wavefront-per-row pair search with the lists staged in LDS -------------
+// One wavefront per orbital row walks the same candidates as search_intra /
+// search_inter, 64 at a time in the same order (image, i, j, l) and with the same
+// arithmetic, so every keep decision is theirs.  Kept pairs are compacted in
+// candidate order (ballot + lanes-below count) into the row's LDS lists, one run
+// per image.  The write pass then places each pair straight at its position of
+// the stable merge of the runs (periodic lists; earlier image first at equal
+// columns): its index in its own run plus, for every other run, the number of
+// entries that precede it there (<= its column for earlier runs, < for later
+// ones), found by binary search in LDS.  The slot count of a row is the number of
+// pairs whose column no earlier run holds.  No scratch copy of the pairs; rows
+// with more pairs than the lists hold use the thread-per-row kernels above.
+constexpr int kPairWaves = 4;                       // rows (wavefronts) per workgroup
+constexpr int kPairCapIntra = 512;                  // staged intralayer pairs per row
+constexpr int kPairCapInter = 1024;                 // staged interlayer pairs per row
+constexpr int kPairCap = kPairCapIntra + kPairCapInter;
+
+struct WaveLists {
+    int32_t* col;       // [kPairCap]: intralayer list at 0, interlayer list at kPairCapIntra
+    int32_t* sc;
+    int32_t* run_end;   // [2][kMaxRuns + 1]: run ends of the intralayer, then the interlayer list
+};
+
+__device__ inline int lanes_below(uint64_t mask, int lane)
+{
+    return __popcll(mask & ((uint64_t(1) << lane) - 1));
+}
+
+// candidate q = (i n1 + j) n0 + l of a window of n1 columns and n0 orbitals, walked
+// 64 at a time: lane k starts at q = k and advances by 64 per step with carries, so
+// the per-step cost is a few adds and compares (the step's digits are divided out
+// once per window)
+struct WindowWalk {
+    int l, j, i;       // this lane's digits
+    int sl, sj, si;    // digits of 64
+    int n0, n1;
+    __device__ inline WindowWalk(int lane, int n0_, int n1_) : n0(n0_), n1(n1_)
+    {
+        l = lane % n0;
+        j = (lane / n0) % n1;
+        i = lane / (n0 * n1);
+        sl = 64 % n0;
+        sj = (64 / n0) % n1;
+        si = 64 / (n0 * n1);
+    }
+    __device__ inline void next()
+    {
+        l += sl;
+        const int c0 = l >= n0;
+        l -= c0 ? n0 : 0;
+        j += sj + c0;
+        const int c1 = j >= n1;
+        j -= c1 ? n1 : 0;
+        i += si + c1;
+    }
+};
+
+// kept candidates of one chunk appended to a list at cnt, and counted
+__device__ inline void wave_push(bool keep, int col, int dx, int dy, int lane, int32_t* lc, int32_t* ls, int cap,
+                                 int& cnt)
+{
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+        const int pos = cnt + lanes_below(m, lane);
+        if (pos < cap) {
+            lc[pos] = col;
+            ls[pos] = (int32_t)((uint32_t)(dx & 0xffff) | ((uint32_t)dy << 16));
+        }
+    }
+    cnt += __popcll(m);
+}
+
+// the intralayer and interlayer candidates of row kh (search_intra, search_inter)
+__device__ void wave_search(const GeomDev& g, int kh, int lane, const WaveLists& L, int& n_intra, int& n_inter,
+                            int& runs_intra, int& runs_inter)
+{
+    const int sh = g.i2g[4 * (size_t)kh + 3];
+    {
+        const SheetDev& sd = g.s[sh];
+        const int ih = g.i2g[4 * (size_t)kh + 0], jh = g.i2g[4 * (size_t)kh + 1], l0 = g.i2g[4 * (size_t)kh + 2];
+        const double kp0 = g.lpos[3 * (size_t)kh], kp1 = g.lpos[3 * (size_t)kh + 1];
+        const int S = sd.bc == 1 ? g.sc_search : 0;
+        const int W = sd.win, norb = g.norb;
+        const int per = 4 * W * W * norb;   // candidates of one image: (2W)^2 cells x orbitals
+        int cnt = 0, nrun = 0;
+        for (int dx = -S; dx < S + 1; ++dx)
+            for (int dy = -S; dy < S + 1; ++dy) {
+                double scd0 = 0.0, scd1 = 0.0;
+                if (sd.bc == 1) {
+                    scd0 = dx * sd.sc[0][0] + dy * sd.sc[1][0];
+                    scd1 = dx * sd.sc[0][1] + dy * sd.sc[1][1];
+                }
+                const double p0 = kp0 + scd0, p1 = kp1 + scd1;
+                const int i0 = clamp_cell(g.ainv[0][0] * p0 + g.ainv[0][1] * p1, sd.min0, sd.max0);
+                const int j0 = clamp_cell(g.ainv[1][0] * p0 + g.ainv[1][1] * p1, sd.min1, sd.max1);
+                WindowWalk ww(lane, norb, 2 * W);
+                for (int b = 0; b < per; b += 64, ww.next()) {
+                    const int q = b + lane;
+                    bool keep = false;
+                    int col = 0;
+                    if (q < per) {
+                        const int l = ww.l;
+                        const int j = j0 - W + ww.j;
+                        const int i = i0 - W + ww.i;
+                        const int k2 = grid_at(sd, norb, i, j, l);
+                        if (k2 != -1) {
+                            int gd0 = i - ih, gd1 = j - jh;
+                            if (sd.bc == 1) {
+                                gd0 = gd0 - dx * sd.st[0][0] - dy * sd.st[1][0];
+                                gd1 = gd1 - dx * sd.st[0][1] - dy * sd.st[1][1];
+                            }
+                            const double t = intralayer(sd.mat, l0, l, gd0, gd1);
+                            keep = t != 0 || (dx == 0 && dy == 0);
+                            col = k2 + sd.base;
+                        }
+                    }
+                    wave_push(keep, col, dx, dy, lane, L.col, L.sc, kPairCapIntra, cnt);
+                }
+                if (lane == 0 && nrun < kMaxRuns) L.run_end[nrun] = cnt;
+                ++nrun;
+            }
+        n_intra = cnt;
+        runs_intra = nrun;
+    }
+    {
+        const double kp0 = g.i2p[3 * (size_t)kh], kp1 = g.i2p[3 * (size_t)kh + 1];
+        const int S = g.bc == 1 ? g.sc_search : 0;
+        const double cut2 = 12.5 * 12.5;
+        const double zero = 0.0;
+        const int norb = g.norb;
+        int cnt = 0, nrun = 0;
+        for (int dx = -S; dx < S + 1; ++dx)
+            for (int dy = -S; dy < S + 1; ++dy) {
+                double scd0 = 0.0, scd1 = 0.0;
+                if (g.bc == 1) {
+                    scd0 = dx * g.gsc[0][0] + dy * g.gsc[1][0];
+                    scd1 = dx * g.gsc[0][1] + dy * g.gsc[1][1];
+                }
+                const double p0 = kp0 + scd0, p1 = kp1 + scd1;
+                for (int side = 0; side < 2; ++side) {
+                    int s0;
+                    if (side == 0) {
+                        if (!(sh > 0)) continue;
+                        s0 = sh - 1;
+                    } else {
+                        if (!(sh < g.num_sheets - 1)) continue;
+                        s0 = sh + 1;
+                    }
+                    const SheetDev& t = g.s[s0];
+                    const double x_new = p0 * t.cneg - p1 * t.sneg + zero * g.lat[0][0] + zero * g.lat[1][0];
+                    const double y_new0 = p0 * t.sneg + p1 * t.cneg + zero * g.lat[0][1] + zero * g.lat[1][1];
+                    const double y_new1 = p1 * t.cneg + p0 * t.sneg + zero * g.lat[0][1] + zero * g.lat[1][1];
+                    const int i0 = clamp_cell(g.ainv[0][0] * x_new + g.ainv[0][1] * y_new0, t.min0, t.max0);
+                    const int j0 = clamp_cell(g.ainv[1][0] * x_new + g.ainv[1][1] * y_new1, t.min1, t.max1);
+                    const int ilo = max(0, i0 - g.inter_win), ihi = min(t.H, i0 + g.inter_win);
+                    const int jlo = max(0, j0 - g.inter_win), jhi = min(t.W, j0 + g.inter_win);
+                    const int nj = max(0, jhi - jlo);
+                    const int per = max(0, ihi - ilo) * nj * norb;
+                    WindowWalk ww(lane, norb, max(nj, 1));
+                    for (int b = 0; b < per; b += 64, ww.next()) {
+                        const int q = b + lane;
+                        bool keep = false;
+                        int col = 0;
+                        if (q < per) {
+                            const int l = ww.l;
+                            const int j = jlo + ww.j;
+                            const int i = ilo + ww.i;
+                            const int k2 = t.grid[((size_t)i * t.W + j) * norb + l];
+                            if (k2 != -1) {
+                                const size_t K = (size_t)k2 + t.base;
+                                const double x2 = g.i2p[3 * K], y2 = g.i2p[3 * K + 1];
+                                keep = (x2 - p0) * (x2 - p0) + (y2 - p1) * (y2 - p1) < cut2;
+                                col = (int)K;
+                            }
+                        }
+                        wave_push(keep, col, dx, dy, lane, L.col + kPairCapIntra, L.sc + kPairCapIntra,
+                                         kPairCapInter, cnt);
+                    }
+                }
+                if (lane == 0 && nrun < kMaxRuns) L.run_end[kMaxRuns + 1 + nrun] = cnt;
+                ++nrun;
+            }
+        n_inter = cnt;
+        runs_inter = nrun;
+    }
+}
+
+// entries of run [a, b) of lc below c (strict) or at most c
+__device__ inline int run_rank(const int32_t* lc, int a, int b, int32_t c, bool or_equal)
+{
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (lc[m] < c || (or_equal && lc[m] == c)) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+// one list of a row placed at out (merged when sorted); returns its slot count
+__device__ int wave_place(const int32_t* lc, const int32_t* ls, const int32_t* run_end, int cnt, int nrun, bool sorted,
+                          int lane, int32_t* ocol, int32_t* osc)
+{
+    int slots = 0;
+    for (int b = 0; b < cnt; b += 64) {
+        const int e = b + lane;
+        bool first = false;
+        if (e < cnt) {
+            const int32_t c = lc[e];
+            if (!sorted) {
+                ocol[e] = c;
+                osc[e] = ls[e];
+                first = e == 0 || lc[e - 1] != c;
+            } else {
+                // the run holding e
+                int r = 0;
+                while (run_end[r] <= e) ++r;
+                int pos = e - (r == 0 ? 0 : run_end[r - 1]);
+                int before_equal = 0;
+                int start = 0;
+                for (int q = 0; q < nrun; ++q) {
+                    const int end = run_end[q];
+                    if (q < r) {
+                        const int le = run_rank(lc, start, end, c, true) - start;
+                        const int lt = run_rank(lc, start, end, c, false) - start;
+                        pos += le;
+                        before_equal += le - lt;
+                    } else if (q > r) {
+                        pos += run_rank(lc, start, end, c, false) - start;
+                    }
+                    start = end;
+                }
+                ocol[pos] = c;
+                osc[pos] = ls[e];
+                first = before_equal == 0;
+            }
+        }
+        slots += __popcll(__ballot(first));
+    }
+    return slots;
+}
+
+__global__ __launch_bounds__(64 * kPairWaves) void pair_write_wave_kernel(GeomDev g, const int64_t* pair_rp,
+                                                                          int32_t* pcol, int32_t* psc, int64_t* nslots)
+{
+    __shared__ int32_t lcol[kPairWaves][kPairCap];
+    __shared__ int32_t lsc[kPairWaves][kPairCap];
+    __shared__ int32_t lrun[kPairWaves][2 * (kMaxRuns + 1)];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int kh = blockIdx.x * kPairWaves + w;
+    if (kh >= g.n) return;
+    WaveLists L = {lcol[w], lsc[w], lrun[w]};
+    int ni = 0, nx = 0, ri = 0, rx = 0;
+    wave_search(g, kh, lane, L, ni, nx, ri, rx);
+    const int64_t base = pair_rp[kh];
+    const int sh = g.i2g[4 * (size_t)kh + 3];
+    // the host runs this kernel only when every list fits and has at most kMaxRuns runs
+    if (ni > kPairCapIntra || nx > kPairCapInter || ri > kMaxRuns || rx > kMaxRuns || pair_rp[kh + 1] - base != ni + nx)
+        return;
+    const int si = wave_place(lcol[w], lsc[w], lrun[w], ni, ri, g.s[sh].bc == 1, lane, pcol + base, psc + base);
+    const int sx = wave_place(lcol[w] + kPairCapIntra, lsc[w] + kPairCapIntra, lrun[w] + kMaxRuns + 1, nx, rx,
+                              g.bc == 1, lane, pcol + base + ni, psc + base + ni);
+    if (lane == 0) nslots[kh] = (int64_t)si + sx;
+}
+
 // ---- per-job fill -----------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void positions_kernel(GeomDev g, JobDev j, double* i2pos)
 {
@@ -678,6 +945,22 @@ hipError_t pairs_count(const GeomDev& g, int32_t* cnt_intra, int64_t* cnt_total,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return scan_counts(cnt_total, pair_rp, g.n, s);
+}
+
+bool pairs_staged_fit(const GeomDev& g, int max_intra, int max_inter)
+{
+    return max_intra <= kPairCapIntra && max_inter <= kPairCapInter && g.sc_search >= 0 &&
+           (2 * g.sc_search + 1) * (2 * g.sc_search + 1) <= kMaxRuns;
+}
+
+hipError_t pairs_write_staged(const GeomDev& g, const int64_t* pair_rp, int32_t* pcol, int32_t* psc, int64_t* nslots,
+                              int64_t* slot_rp, hipStream_t s)
+{
+    hipLaunchKernelGGL(pair_write_wave_kernel, dim3((unsigned)((g.n + kPairWaves - 1) / kPairWaves)),
+                       dim3(64 * kPairWaves), 0, s, g, pair_rp, pcol, psc, nslots);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return scan_counts(nslots, slot_rp, g.n, s);
 }
 
 hipError_t pairs_write(const GeomDev& g, const int64_t* pair_rp, const int32_t* cnt_intra, int32_t* tcol, int32_t* tsc,

@@ -121,6 +121,7 @@ struct leo2d_handle {
     DevBuf i2g, i2p, lpos;
     // pairs
     bool has_pairs = false;
+    bool pairs_staged = false;   // the last pair lists came from the LDS-staged wavefront kernels
     int64_t n_pairs = 0, n_slots = 0, n_intra = 0;
     DevBuf cnt_intra, cnt_tmp, pair_rp, pcol, psc, nslots, slot_rp;
     // matrix
@@ -868,6 +869,16 @@ int leo2d_set_geometry(leo2d_handle* h, const leo2d_geom* gin)
     });
 }
 
+int leo2d_pairs_method(leo2d_handle* h, int32_t* staged)
+{
+    if (!h || !staged) return LEO2D_ERR_ARG;
+    return guarded(h, [&]() {
+        if (!h->has_pairs) throw StateError("no pair lists: call leo2d_pairs_build");
+        *staged = h->pairs_staged ? 1 : 0;
+        return LEO2D_OK;
+    });
+}
+
 int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64_t* n_slots)
 {
     if (!h) return LEO2D_ERR_ARG;
@@ -883,21 +894,39 @@ int leo2d_pairs_build(leo2d_handle* h, int64_t* n_intra, int64_t* n_inter, int64
         int64_t total = 0;
         check(hipMemcpyAsync(&total, (int64_t*)h->pair_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "pair total");
         std::vector<int32_t> ci(n);
+        std::vector<int64_t> ct(n);
         check(hipMemcpyAsync(ci.data(), h->cnt_intra.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s), "pair counts");
+        check(hipMemcpyAsync(ct.data(), h->cnt_tmp.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s), "pair counts");
         check(hipStreamSynchronize(s), "pair count");
-        int64_t ni = 0;
-        for (int32_t c : ci) ni += c;
+        int64_t ni = 0, max_intra = 0, max_inter = 0;
+        for (int i = 0; i < n; ++i) {
+            ni += ci[i];
+            max_intra = std::max<int64_t>(max_intra, ci[i]);
+            max_inter = std::max<int64_t>(max_inter, ct[i] - ci[i]);
+        }
         dalloc(h->pcol, sizeof(int32_t) * (size_t)total, "pair columns");
         dalloc(h->psc, sizeof(int32_t) * (size_t)total, "pair images");
         dalloc(h->nslots, sizeof(int64_t) * n, "slot counts");
         dalloc(h->slot_rp, sizeof(int64_t) * (n + 1), "slot offsets");
-        DevBuf tcol, tsc;   // the pairs as found, before the per-image runs are merged
-        dalloc(tcol, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
-        dalloc(tsc, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
-        check(leo2d::pairs_write(h->gdev, (const int64_t*)h->pair_rp.p, (const int32_t*)h->cnt_intra.p,
-                                 (int32_t*)tcol.p, (int32_t*)tsc.p, (int32_t*)h->pcol.p, (int32_t*)h->psc.p,
-                                 (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
-              "pair write");
+        // lists staged in LDS, one wavefront per row, where every row's lists fit; else
+        // one thread per row through a scratch copy of the pairs
+        h->pairs_staged = env_int("LEO2D_PAIRS_STAGED", 1) != 0 &&
+                          leo2d::pairs_staged_fit(h->gdev, (int)std::min<int64_t>(max_intra, INT32_MAX),
+                                                  (int)std::min<int64_t>(max_inter, INT32_MAX));
+        if (h->pairs_staged) {
+            check(leo2d::pairs_write_staged(h->gdev, (const int64_t*)h->pair_rp.p, (int32_t*)h->pcol.p,
+                                            (int32_t*)h->psc.p, (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
+                  "pair write");
+        } else {
+            DevBuf tcol, tsc;   // the pairs as found, before the per-image runs are merged
+            dalloc(tcol, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
+            dalloc(tsc, sizeof(int32_t) * (size_t)std::max<int64_t>(total, 1), "pair scratch");
+            check(leo2d::pairs_write(h->gdev, (const int64_t*)h->pair_rp.p, (const int32_t*)h->cnt_intra.p,
+                                     (int32_t*)tcol.p, (int32_t*)tsc.p, (int32_t*)h->pcol.p, (int32_t*)h->psc.p,
+                                     (int64_t*)h->nslots.p, (int64_t*)h->slot_rp.p, s),
+                  "pair write");
+            check(hipStreamSynchronize(s), "pair write");
+        }
         int64_t slots = 0;
         check(hipMemcpyAsync(&slots, (int64_t*)h->slot_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s), "slot total");
         check(hipStreamSynchronize(s), "pair write");

@@ -79,6 +79,27 @@ def test_pairs_bit_exact(dev, oracle_mod, name):
             assert np.array_equal(t, ot)
 
 
+@pytest.mark.parametrize("name", PARITY_DECKS + ["c1cell2"])
+def test_staged_pairs_equal_thread_pairs(dev, monkeypatch, name):
+    """The LDS-staged pair search (one wavefront per row) writes the same lists,
+    counts and slot counts as the thread-per-row kernels, bit for bit: every
+    parity deck (flakes, periodic cells with and without k-points, supercell
+    grids, strained and trilayer decks) and the 1.05 deg (32, 31) cell tiled
+    2 x 2."""
+    print("case", name, flush=True)
+    text = W.tblg_cell_deck(32, 31, 2) if name == "c1cell2" else deck_text(name)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("LEO2D_PAIRS_STAGED", staged)
+        deck, geom, counts = _setup(dev, text)
+        assert dev.pairs_method() == int(staged), name
+        lists = [dev.pairs_copy(which, cnt) for which, cnt in ((0, counts[0]), (1, counts[1]))]
+        out[staged] = (counts, lists)
+    assert out["1"][0] == out["0"][0]
+    for (a_ij, a_sc, a_t), (b_ij, b_sc, b_t) in zip(out["1"][1], out["0"][1]):
+        assert np.array_equal(a_ij, b_ij) and np.array_equal(a_sc, b_sc) and np.array_equal(a_t, b_t)
+
+
 def _value_ok(v, ov):
     tol = 1e-12 * np.abs(ov) + 1e-14 * max(1e-300, float(np.max(np.abs(ov))) if len(ov) else 1.0)
     return np.all(np.abs(v - ov) <= tol)
