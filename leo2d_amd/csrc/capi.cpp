@@ -1995,8 +1995,10 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         // SELL tiles of the stack (complex vectors, R = 16 or 32; FP32 only at 32): tiles of at most
         // 56 rows for the kernel with a copy wavefront, else 64 rows whose union fits the LDS the
         // eight-wavefront kernel leaves beside its T_{j-1} copies and partials
+        // (built on the host once per stack, so only when mode 3 is asked for or may be tuned in)
         const bool sell_cw = env_int("LEO2D_SELL_CW", 1) != 0;
-        const bool sell_ok = leo2d::sell_supported(mat_type, vec_type, R, sell_cw) && R <= 32 &&
+        const bool want_sell = p->mode == LEO2D_BATCH_SELL || p->tune;
+        const bool sell_ok = want_sell && leo2d::sell_supported(mat_type, vec_type, R, sell_cw) && R <= 32 &&
                              build_sell_tiles(B.sell, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
                                               (const int32_t*)B.col.p,
                                               sell_cw ? leo2d::kSellUcap : leo2d::kSellUcapTiled,

@@ -55,6 +55,11 @@ __device__ inline void lds_ld(XT (&x)[VPL], const XT* p)
 {
     if constexpr (VPL == 1) {
         x[0] = *p;
+    } else if constexpr (std::is_same<XT, double>::value) {
+        typedef double d2v __attribute__((ext_vector_type(2)));
+        const d2v q = *reinterpret_cast<const d2v*>(p);
+        x[0] = q.x;
+        x[1] = q.y;
     } else {
         const f4v q = *reinterpret_cast<const f4v*>(p);
         x[0] = make_float2(q.x, q.y);
@@ -117,6 +122,13 @@ template <int K> struct DppMac;
                 : "+v"(ax), "+v"(ay)                                                                               \
                 : "v"(hr), "v"(hi), "v"(xx), "v"(xy));                                                             \
         }                                                                                                          \
+        __device__ static inline void f64r(double& a0, double& a1, double h, double x0, double x1)                \
+        {                                                                                                          \
+            asm("v_fmac_f64_dpp %0, %2, %3 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
+                "v_fmac_f64_dpp %1, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf"                       \
+                : "+v"(a0), "+v"(a1)                                                                               \
+                : "v"(h), "v"(x0), "v"(x1));                                                                       \
+        }                                                                                                          \
         __device__ static inline void f32x1(float& ax, float& ay, float hr, float hi, float xx, float xy)         \
         {                                                                                                          \
             asm("v_fmac_f32_dpp %0, %2, %4 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n\t"                 \
@@ -159,6 +171,8 @@ __device__ inline void mac_step(AT (&acc)[VPL], const XT (&x)[VPL], const GroupR
     } else if constexpr (std::is_same<HT, float2>::value && std::is_same<XT, float2>::value && VPL == 1) {
         const float2 h = g.ev[E >> 4];
         DppMac<E & 15>::f32x1(acc[0].x, acc[0].y, h.x, h.y, x[0].x, x[0].y);
+    } else if constexpr (std::is_same<HT, double>::value && std::is_same<XT, double>::value && VPL == 2) {
+        DppMac<E & 15>::f64r(acc[0], acc[1], g.ev[E >> 4], x[0], x[1]);
     } else {
         const HT h = bc16<E & 15>(g.ev[E >> 4]);
 #pragma unroll
@@ -1002,8 +1016,10 @@ bool sell_supported(int mat_type, int vec_type, int R, bool copy_wave)
 {
     const bool cpx64 = vec_type == VEC_CPX_F64 && (mat_type == MAT_CPX_F64 || mat_type == MAT_REAL_F64);
     const bool cpx32 = vec_type == VEC_CPX_F32 && (mat_type == MAT_CPX_F32 || mat_type == MAT_REAL_F32);
+    const bool real64 = vec_type == VEC_REAL_F64 && mat_type == MAT_REAL_F64;
     if (cpx64) return R == 16 || R == 32 || R == 64;
     if (cpx32) return R == 32 || R == 64 || (copy_wave && R == 16);
+    if (real64) return R == 32 || R == 64;   // two FP64 vectors per lane, 32 per 256-byte chunk
     return false;
 }
 
@@ -1013,6 +1029,7 @@ hipError_t launch_kpm_step_sell(int mat_type, int vec_type, const SellStepArgs& 
         a.tile_rows > kSellMaxRows || a.max_len > kSellMaxLen)
         return hipErrorInvalidValue;
     if (mat_type == MAT_CPX_F64) return launch_sell_shape<double2, double2, 1>(a, s);
+    if (mat_type == MAT_REAL_F64 && vec_type == VEC_REAL_F64) return launch_sell_shape<double, double, 2>(a, s);
     if (mat_type == MAT_REAL_F64) return launch_sell_shape<double, double2, 1>(a, s);
     if (mat_type == MAT_CPX_F32)
         return a.R == 16 ? launch_sell_f32x16<float2>(a, s) : launch_sell_shape<float2, float2, 2>(a, s);

@@ -112,8 +112,9 @@ class ExactLocalKPM:
     """Exact local moments on one device handle: random samples (every
     target orbital of a cell is a sample) and on-demand configurations."""
 
-    def __init__(self, dev: L.Device, M: int, chunk: int = 256, precision=L.PREC_FP64):
+    def __init__(self, dev: L.Device, M: int, chunk: int = 256, precision=L.PREC_FP64, mode=None):
         self.dev, self.M, self.chunk, self.precision = dev, M, chunk, precision
+        self.mode = mode   # batched layout (None: row chunks for 16 or 32 vectors, else row groups)
         self._geoms = {}
 
     def _geom(self, i):
@@ -140,7 +141,7 @@ class ExactLocalKPM:
             if not pending:
                 return
             # 16- and 32-vector blocks use the row-chunk layout (16-entry chunks broadcast in 16-lane rows)
-            mode = L.BATCH_ROW_CHUNKS if R in (16, 32) else L.BATCH_ROW_GROUPS
+            mode = self.mode if self.mode is not None else (L.BATCH_ROW_CHUNKS if R in (16, 32) else L.BATCH_ROW_GROUPS)
             mu = dev.batch_kpm(self.M, R, targets=np.array(targets), precision=self.precision, graph=False, mode=mode)
             out[np.array(pending)] = mu
             dev.batch_clear()

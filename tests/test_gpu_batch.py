@@ -236,7 +236,8 @@ def _gamma_job(dev, geom):
 @pytest.mark.parametrize("Rt", [4, 16, 32])
 def test_batch_real_target_starts(oracle_mod, Rt):
     """Real H (Gamma point) with unit-vector starts runs on real vector blocks
-    (FP64 and FP32 elements): every lane layout, both precisions, against the
+    (FP64 and FP32 elements): every lane layout, both precisions, and the SELL
+    tiles with two FP64 vectors per lane at 32 vectors per job, against the
     oracle's faithful KPM from the same targets (FP64 within 1e-12, FP32
     recursion within 2e-5 of mu_0 = 1)."""
     dev = L.Device(0)
@@ -255,10 +256,12 @@ def test_batch_real_target_starts(oracle_mod, Rt):
             want.append(oracle_mod.kpm(rp, ci, v, M, targets=t, doubling_all=True, nthreads=8))
             dev.batch_append()
         tg = np.array(tg)
-        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS, L.BATCH_ROW_CHUNKS):
+        for mode in (L.BATCH_ROW_GROUPS, L.BATCH_ENTRY_SLOTS, L.BATCH_ROW_CHUNKS, L.BATCH_SELL):
             if mode == L.BATCH_ROW_CHUNKS and Rt < 16:
                 continue
             for prec, tol in ((L.PREC_FP64, 1e-12), (L.PREC_MIXED, 2e-5)):
+                if mode == L.BATCH_SELL and (Rt != 32 or prec != L.PREC_FP64):
+                    continue   # real SELL tiles: FP64 vectors, 32 per job
                 print("case R", Rt, "mode", mode, "prec", prec, flush=True)
                 mu = dev.batch_kpm(M, Rt, targets=tg, precision=prec, mode=mode, graph=False)
                 for p, omu in enumerate(want):
