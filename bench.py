@@ -643,7 +643,14 @@ def run_c4(args):
     n_jobs = (args.c4_train + n_val + R - 1) // R
     rng = np.random.default_rng(4)
     dev = L.Device(0)
-    ex = S.ExactLocalKPM(dev, M, chunk=128)
+    last = [time.perf_counter()]
+
+    def progress(done, total):
+        if time.perf_counter() - last[0] > 20 or done == total:
+            last[0] = time.perf_counter()
+            log(f"c4: exact moments for {done} of {total} cells")
+
+    ex = S.ExactLocalKPM(dev, M, chunk=128, progress=progress)
     t0 = time.perf_counter()
     d = ex.samples(n_jobs, rng, R=R, theta_range=(args.c4_theta_min, 3.0))
     t_gen = time.perf_counter() - t0

@@ -112,9 +112,10 @@ class ExactLocalKPM:
     """Exact local moments on one device handle: random samples (every
     target orbital of a cell is a sample) and on-demand configurations."""
 
-    def __init__(self, dev: L.Device, M: int, chunk: int = 256, precision=L.PREC_FP64, mode=None):
+    def __init__(self, dev: L.Device, M: int, chunk: int = 256, precision=L.PREC_FP64, mode=None, progress=None):
         self.dev, self.M, self.chunk, self.precision = dev, M, chunk, precision
-        self.mode = mode   # batched layout (None: row chunks for 16 or 32 vectors, else row groups)
+        self.mode = mode           # batched layout (None: row chunks for 16 or 32 vectors, else row groups)
+        self.progress = progress   # called as progress(jobs done, jobs) after every batch
         self._geoms = {}
 
     def _geom(self, i):
@@ -135,6 +136,7 @@ class ExactLocalKPM:
         dev = self.dev
         dev.batch_clear()
         pending, targets = [], []
+        done = [0]
         cur = None
 
         def flush():
@@ -144,6 +146,9 @@ class ExactLocalKPM:
             mode = self.mode if self.mode is not None else (L.BATCH_ROW_CHUNKS if R in (16, 32) else L.BATCH_ROW_GROUPS)
             mu = dev.batch_kpm(self.M, R, targets=np.array(targets), precision=self.precision, graph=False, mode=mode)
             out[np.array(pending)] = mu
+            done[0] += len(pending)
+            if self.progress:
+                self.progress(done[0], len(jobs))
             dev.batch_clear()
             pending.clear()
             targets.clear()
