@@ -733,7 +733,7 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
         };
         GroupRegs<HT, QS> ec[GPW], en[GPW];
         int gc_row[GPW], gn_row[GPW], gc_len[GPW], gn_len[GPW];
-        int t = t_first, c = 0, p = 0;
+        int t = t_first, p = 0;
         int r0 = 0, r0n = 0;
         if (live(t)) {
             load_groups(t, ec, gc_row, gc_len);
@@ -752,7 +752,11 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                 pend_tile = -1;
             }
         };
-        auto pair = [&](const XT* buf) {
+        // the chunk index is a constant of each pair's code: the walk is unrolled over the
+        // chunks of a tile, so the moment sums of chunk c are plain registers (a run-time
+        // chunk index makes every sum an add into each chunk's register and a select)
+        auto pair = [&](auto chunk, const XT* buf) {
+            constexpr int c = decltype(chunk)::value;
             __syncthreads();
             emit_tile();
             // opaque per pair: keeps the compiler from specialising the whole walk on them
@@ -806,16 +810,25 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
                     o[j][v] = comb(alpha, acc[j][v], pv[j][v]);
+                    // on a padded row (row < 0) the products, T_{j-1} and v are all zero (padded
+                    // entries hold zero values), so o and vv are zero and only the norm needs the test
                     const double t1v = row >= 0 ? nrm2(cv[v]) : 0.0;
-                    const double t2v = row >= 0 ? redot(o[j][v], cv[v]) : 0.0;
-                    const double t3v = has_v && row >= 0 ? redot(vv[j][v], cv[v]) : 0.0;
+                    const double t2v = redot(o[j][v], cv[v]);
 #pragma unroll
                     for (int cc = 0; cc < NCH; ++cc) {
                         if (cc == c) {
                             d1[cc][v] += t1v;
                             d2[cc][v] += t2v;
-                            d3[cc][v] += t3v;
                         }
+                    }
+                }
+                if (has_v) {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        const double t3v = redot(vv[j][v], cv[v]);
+#pragma unroll
+                        for (int cc = 0; cc < NCH; ++cc)
+                            if (cc == c) d3[cc][v] += t3v;
                     }
                 }
             }
@@ -842,14 +855,10 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                 pend_set = tset;
                 tset ^= 1;
             }
-            int t1 = t, c1 = c + 1;
-            if (c1 == NCH) {
-                c1 = 0;
-                t1 = t + t_stride;
-            }
+            const int t1 = c + 1 == NCH ? t + t_stride : t;
             // the next tile's groups before this pair's stores: waiting for them then
             // leaves the stores in flight
-            if (c1 == 0 && live(t1)) {
+            if (c + 1 == NCH && live(t1)) {
                 load_groups(t1, en, gn_row, gn_len);
                 r0n = ldu(a.tile_row0, t1);
             }
@@ -863,13 +872,24 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                 }
             }
             t = t1;
-            c = c1;
             ++p;
         };
+        // pairs alternate between the two buffers: with an even chunk count chunk c always
+        // uses buffer c % 2, with one chunk the tiles alternate
+        static_assert(NCH == 1 || NCH == 2 || NCH == 4, "one, two or four chunks");
         while (live(t)) {
-            pair(xs0);
-            if (!live(t)) break;
-            pair(xs1);
+            if constexpr (NCH == 1) {
+                pair(std::integral_constant<int, 0>{}, xs0);
+                if (!live(t)) break;
+                pair(std::integral_constant<int, 0>{}, xs1);
+            } else {
+                pair(std::integral_constant<int, 0>{}, xs0);
+                pair(std::integral_constant<int, 1>{}, xs1);
+                if constexpr (NCH == 4) {
+                    pair(std::integral_constant<int, 2>{}, xs0);
+                    pair(std::integral_constant<int, 3>{}, xs1);
+                }
+            }
         }
         // every wavefront is past its last read of the buffers: the partials go into xs0
         __syncthreads();
