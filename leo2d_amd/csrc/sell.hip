@@ -67,6 +67,41 @@ __device__ inline void lds_ld(XT (&x)[VPL], const XT* p)
     }
 }
 
+// T_{j-1} rows in and T_{j+1} rows out of the copy-wavefront step are streamed:
+// each lane moves its VPL elements (16 or 8 bytes) once per step, so they are
+// read and written non-temporally and leave L2 to the staged rows of T_j,
+// which neighbouring tiles read again (c1: 1-1.5 % per step, same results)
+template <typename XT, int VPL>
+__device__ inline void ld_stream(XT (&x)[VPL], const XT* p)
+{
+    static_assert(VPL * sizeof(XT) == 16 || VPL * sizeof(XT) == 8, "16 or 8 bytes per lane");
+    if constexpr (VPL * sizeof(XT) == 16) {
+        typedef int i4v __attribute__((ext_vector_type(4)));
+        const i4v q = __builtin_nontemporal_load(reinterpret_cast<const i4v*>(p));
+        __builtin_memcpy(&x[0], &q, 16);
+    } else {
+        typedef int i2v __attribute__((ext_vector_type(2)));
+        const i2v q = __builtin_nontemporal_load(reinterpret_cast<const i2v*>(p));
+        __builtin_memcpy(&x[0], &q, 8);
+    }
+}
+template <typename XT, int VPL>
+__device__ inline void st_stream(XT* p, const XT (&x)[VPL])
+{
+    static_assert(VPL * sizeof(XT) == 16 || VPL * sizeof(XT) == 8, "16 or 8 bytes per lane");
+    if constexpr (VPL * sizeof(XT) == 16) {
+        typedef int i4v __attribute__((ext_vector_type(4)));
+        i4v q;
+        __builtin_memcpy(&q, &x[0], 16);
+        __builtin_nontemporal_store(q, reinterpret_cast<i4v*>(p));
+    } else {
+        typedef int i2v __attribute__((ext_vector_type(2)));
+        i2v q;
+        __builtin_memcpy(&q, &x[0], 8);
+        __builtin_nontemporal_store(q, reinterpret_cast<i2v*>(p));
+    }
+}
+
 // entries k, k+16, ..., k+16(QS-1) of the lane's row (union index, value)
 template <typename HT, int QS>
 struct GroupRegs {
@@ -778,10 +813,14 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
             for (int j = 0; j < GPW; ++j) {
                 const int row = gc_row[j];
                 const size_t idx = (size_t)(row < 0 ? 0 : row) * R + c * C + koff;
+                if (beta && row >= 0) {
+                    ld_stream<XT, VPL>(pv[j], prev + idx);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) zero(pv[j][v]);
+                }
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
-                    if (beta && row >= 0) pv[j][v] = prev[idx + v];
-                    else zero(pv[j][v]);
                     if (has_v && row >= 0) vv[j][v] = vin[idx + v];
                     else zero(vv[j][v]);
                 }
@@ -867,8 +906,7 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                 const int row = gc_row[j];
                 if (row >= 0) {
                     const size_t idx = (size_t)row * R + c * C + koff;
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v) out[idx + v] = o[j][v];
+                    st_stream<XT, VPL>(out + idx, o[j]);
                 }
             }
             t = t1;
