@@ -100,7 +100,8 @@ def main():
                     help="c1 only: instead of the bench line, print max|dDoS| of the FP32 recursion against FP64 "
                          "for each of these comma-separated moment counts (e.g. 1024,2048,4096,8192)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run a plain 2 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
+                    help="only run a plain 2 GiB device copy (HBM counter calibration under rocprofv3 --pmc) and "
+                         "time it, with a 2 GiB read-only sum and a 2 GiB fill: what the HBM delivers to plain streams")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -110,7 +111,22 @@ def main():
         for _ in range(2):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
+        nb = x.numel() * 8
+
+        def rate(fn, moved, reps=10):
+            fn()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                fn()
+            b.record()
+            torch.cuda.synchronize()
+            return moved * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb,
+                          "copy_GBs": rate(lambda: y.copy_(x), 2 * nb),
+                          "read_sum_GBs": rate(lambda: torch.sum(x), nb),
+                          "fill_GBs": rate(lambda: y.fill_(2.0), nb)}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
