@@ -615,8 +615,10 @@ __global__ __launch_bounds__(64 * NW, 1) void kpm_step_sell_kernel(SellStepArgs 
 //   copy wavefront:    barrier p; copies of pair p+1 into the buffer pair p-1
 //                      used; union rows of the next tile; wait for all of it;
 //   compute wavefront: barrier p; T_{j-1} (and v) of the pair's rows as register
-//                      loads; products from the buffer; outputs; the next tile's
-//                      groups, then the stores.
+//                      loads (T_{j-1} non-temporal); products from the buffer;
+//                      outputs; the next tile's groups, then the stores of
+//                      T_{j+1} (non-temporal).  The walk is unrolled over the
+//                      chunks of a tile, so each pair's code has its chunk fixed.
 // Both walk the same pairs, so they meet at the same barriers.  Tiles hold at
 // most 4 GPW NWC rows (GPW groups of four rows per compute wavefront).
 // PT: partial sums per tile (part[tile][R][2], stacked jobs) as in the kernel above.
