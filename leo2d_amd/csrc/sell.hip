@@ -218,12 +218,14 @@ __device__ inline void mac_step(AT (&acc)[VPL], const XT (&x)[VPL], const GroupR
 // steps 4B .. L-1 of the group (L a multiple of 4, at most 16 QS); xa holds the
 // operands of block B, the next block's are read before this block's products
 // (one block past L is read and not used)
-template <int B, typename HT, typename XT, typename AT, int VPL, int QS>
+// PLO >= 0: the wavefront's issue priority drops to PLO when block 4 starts
+template <int B, typename HT, typename XT, typename AT, int VPL, int QS, int PLO = -1>
 __device__ inline void run_blocks(AT (&acc)[VPL], const GroupRegs<HT, QS>& g, const XT* xs, int koff, int L,
                                   const XT (&xa)[4][VPL])
 {
     if constexpr (B < 4 * QS) {
         if (4 * B < L) {
+            if constexpr (B == 4 && PLO >= 0) __builtin_amdgcn_s_setprio(PLO);
             XT xb[4][VPL];
             // unconditional (entries past L are index 0, a valid row): a branch here would
             // merge two LDS counts and make the wait below cover the next block too
@@ -234,7 +236,7 @@ __device__ inline void run_blocks(AT (&acc)[VPL], const GroupRegs<HT, QS>& g, co
             mac_step<4 * B + 1, HT, XT, AT, VPL, QS>(acc, xa[1], g);
             mac_step<4 * B + 2, HT, XT, AT, VPL, QS>(acc, xa[2], g);
             mac_step<4 * B + 3, HT, XT, AT, VPL, QS>(acc, xa[3], g);
-            run_blocks<B + 1, HT, XT, AT, VPL, QS>(acc, g, xs, koff, L, xb);
+            run_blocks<B + 1, HT, XT, AT, VPL, QS, PLO>(acc, g, xs, koff, L, xb);
         }
     }
 }
@@ -827,9 +829,17 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
                     else zero(vv[j][v]);
                 }
             }
+            // Issue priority falls as a wavefront works through its groups (3, 2 from block 4
+            // of the first group, 1, then 0 from block 4 of the second).  Two compute
+            // wavefronts share each SIMD, and at equal priority the older one is issued
+            // first, finishes its pair early and leaves the younger to run alone, its
+            // latencies unhidden; ranking them by the work they have left keeps them
+            // level (c1: 1.5 % per FP64 step, 2.5 % per FP32 step, same results).
             AT acc[GPW][VPL];
-#pragma unroll
-            for (int j = 0; j < GPW; ++j) {
+            static_assert(GPW == 1 || GPW == 2, "one or two groups per wavefront");
+            auto group = [&](auto J) {
+                constexpr int j = decltype(J)::value;
+                __builtin_amdgcn_s_setprio(3 - 2 * j);
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) zero(acc[j][v]);
                 int L = __builtin_amdgcn_readfirstlane(gc_len[j]);
@@ -839,9 +849,12 @@ __global__ __launch_bounds__(64 * (NWC + 1), 1) void kpm_step_sell_cw_kernel(Sel
 #pragma unroll
                     for (int q = 0; q < QS; ++q) pin_values(ec[j].ev[q]);
                     ld_block<0, HT, XT, VPL, QS>(xa, ec[j], buf, koff);
-                    run_blocks<0, HT, XT, AT, VPL, QS>(acc[j], ec[j], buf, koff, L, xa);
+                    run_blocks<0, HT, XT, AT, VPL, QS, 2 - 2 * j>(acc[j], ec[j], buf, koff, L, xa);
                 }
-            }
+            };
+            group(std::integral_constant<int, 0>{});
+            if constexpr (GPW == 2) group(std::integral_constant<int, 1>{});
+            __builtin_amdgcn_s_setprio(0);
             XT o[GPW][VPL];
 #pragma unroll
             for (int j = 0; j < GPW; ++j) {
