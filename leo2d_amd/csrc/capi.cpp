@@ -14,6 +14,7 @@
 #include <fstream>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 
@@ -34,6 +35,7 @@ struct leo2d_geom {
 namespace {
 
 using leo2d::DevBuf;
+using leo2d::HostBuf;
 using leo2d::HipError;
 using leo2d::check;
 using leo2d::dalloc;
@@ -150,6 +152,7 @@ struct leo2d_handle {
     // SELL-4-sigma row tiles of the recursion matrix for the LDS-staged step
     // (sell.hip): pattern built on the host on first use, values gathered on the device
     SellTiles sell;
+    HostBuf sell_stage;   // pinned staging of a pattern for its SELL tile build, kept for the next build
     leo2d_timing timing{};
     std::vector<hipEvent_t> events;
     // batched jobs: stacked block-diagonal CSR of every appended Hamiltonian
@@ -332,6 +335,114 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->has_dx = false;
 }
 
+// Tiles of the rows [lo, hi), whose columns all lie in [lo, hi) (checked), with
+// tile starts forced at the rows marked in brk.  Offsets (u0, g0, goff) are
+// relative to this part's own arrays; rows, columns and CSR positions are global.
+struct SellPart {
+    std::vector<int32_t> row0, ucols, g0, grow, glen;
+    std::vector<int64_t> u0, goff, map;
+    std::vector<uint16_t> eu;
+    int tile_rows = 0, max_len = 0;
+    bool ok = true;
+};
+
+void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const uint8_t* brk, int lo, int hi, int ucap,
+                     int tmax)
+{
+    const int m = hi - lo;
+    std::vector<int32_t> stamp(m, -1), loc(m, 0), added, halo, order;
+    int r = lo, tile = 0;
+    while (r < hi) {
+        const int start = r;
+        int size = 0;   // distinct columns of the tile so far (own rows included)
+        while (r < hi && r - start < tmax && !(r > start && brk[r])) {
+            if (rp[r + 1] - rp[r] > leo2d::kSellMaxLen) {
+                P.ok = false;
+                return;
+            }
+            added.clear();
+            auto take = [&](int c) {
+                if (stamp[c - lo] != tile) {
+                    stamp[c - lo] = tile;
+                    added.push_back(c);
+                }
+            };
+            take(r);
+            for (int64_t q = rp[r]; q < rp[r + 1]; ++q) {
+                if (col[q] < lo || col[q] >= hi) {
+                    P.ok = false;
+                    return;
+                }
+                take(col[q]);
+            }
+            if (size + (int)added.size() > ucap) {
+                for (int c : added) stamp[c - lo] = -1;
+                break;
+            }
+            size += (int)added.size();
+            ++r;
+        }
+        if (r == start) {
+            P.ok = false;
+            return;
+        }
+        const int T = r - start;
+        P.tile_rows = std::max(P.tile_rows, T);
+        // union: own rows, then the other columns in increasing order
+        halo.clear();
+        for (int rr = start; rr < r; ++rr)
+            for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) {
+                const int c = col[q];
+                if ((c < start || c >= r) && stamp[c - lo] == tile) {
+                    stamp[c - lo] = -2 - tile;   // listed once
+                    halo.push_back(c);
+                }
+            }
+        std::sort(halo.begin(), halo.end());
+        P.row0.push_back(start);
+        P.u0.push_back((int64_t)P.ucols.size());
+        for (int rr = start; rr < r; ++rr) {
+            loc[rr - lo] = rr - start;
+            P.ucols.push_back(rr);
+        }
+        for (size_t k = 0; k < halo.size(); ++k) {
+            loc[halo[k] - lo] = T + (int)k;
+            P.ucols.push_back(halo[k]);
+        }
+        // SELL-4 groups, rows by decreasing length (sigma = the tile)
+        order.resize(T);
+        for (int k = 0; k < T; ++k) order[k] = start + k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+        P.g0.push_back((int32_t)P.glen.size());
+        for (int k = 0; k < T; k += 4) {
+            int L = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int row = k + j < T ? order[k + j] : -1;
+                P.grow.push_back(row);
+                if (row >= 0) L = std::max(L, (int)(rp[row + 1] - rp[row]));
+            }
+            L = (L + 3) / 4 * 4;
+            P.max_len = std::max(P.max_len, L);
+            P.glen.push_back(L);
+            P.goff.push_back((int64_t)P.eu.size());
+            for (int e = 0; e < L; ++e)
+                for (int j = 0; j < 4; ++j) {
+                    const int row = k + j < T ? order[k + j] : -1;
+                    if (row >= 0 && e < rp[row + 1] - rp[row]) {
+                        const int64_t q = rp[row] + e;
+                        P.eu.push_back((uint16_t)loc[col[q] - lo]);
+                        P.map.push_back(q);
+                    } else {
+                        P.eu.push_back(0);
+                        P.map.push_back(-1);
+                    }
+                }
+        }
+        ++tile;
+    }
+}
+
 // SELL-4-sigma row tiles for the LDS-staged step (sell.hip).  Greedy tiles of
 // consecutive rows of the recursion matrix (at most kSellMaxRows) while the
 // tile's distinct columns -- its own rows first, then the other columns in
@@ -344,16 +455,20 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
 // when a row has more than kSellMaxLen entries or does not fit `ucap` alone.
 // With `breaks` (ascending first rows of stacked jobs, ending with n) no tile
 // crosses a job boundary, and job_tile0 lists each job's first tile.
-bool build_sell_tiles(SellTiles& S, hipStream_t s, int n, int64_t nnz, const int64_t* drp, const int32_t* dcol, int ucap,
-                      int tmax, bool spatial, const std::vector<int64_t>* breaks)
+bool build_sell_tiles(SellTiles& S, HostBuf& stage, hipStream_t s, int n, int64_t nnz, const int64_t* drp,
+                      const int32_t* dcol, int ucap, int tmax, bool spatial, const std::vector<int64_t>* breaks)
 {
     if (S.valid && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return true;
     if (S.failed && S.ucap == ucap && S.tmax == tmax && S.spatial == spatial) return false;
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<int64_t> rp(n + 1);
-    std::vector<int32_t> col((size_t)nnz);
-    check(hipMemcpyAsync(rp.data(), drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "sell");
-    if (nnz) check(hipMemcpyAsync(col.data(), dcol, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s), "sell");
+    // the pattern comes over into pinned memory the handle keeps: batched jobs
+    // rebuild their tiles for every batch, and a pageable copy of a 271 M-entry
+    // pattern took 0.37 s
+    halloc(stage, sizeof(int64_t) * ((size_t)n + 1) + sizeof(int32_t) * (size_t)nnz, "sell pattern staging");
+    int64_t* const rp = static_cast<int64_t*>(stage.p);
+    int32_t* const col = reinterpret_cast<int32_t*>(rp + n + 1);
+    check(hipMemcpyAsync(rp, drp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s), "sell");
+    if (nnz) check(hipMemcpyAsync(col, dcol, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s), "sell");
     check(hipStreamSynchronize(s), "sell");
     std::vector<uint8_t> brk(n + 1, 0);
     if (breaks)
@@ -367,100 +482,94 @@ bool build_sell_tiles(SellTiles& S, hipStream_t s, int n, int64_t nnz, const int
         S.valid = false;
         return false;
     };
-    std::vector<int32_t> row0, ucols, g0, grow, glen;
-    std::vector<int64_t> u0, goff, map;
-    std::vector<uint16_t> eu;
-    std::vector<int32_t> stamp(n, -1), loc(n, 0), added, halo, order;
-    int r = 0, tile = 0, tile_rows = 0, max_len = 0;
-    while (r < n) {
-        const int start = r;
-        int size = 0;   // distinct columns of the tile so far (own rows included)
-        while (r < n && r - start < tmax && !(r > start && brk[r])) {
-            if (rp[r + 1] - rp[r] > leo2d::kSellMaxLen) return fail();
-            added.clear();
-            auto take = [&](int c) {
-                if (stamp[c] != tile) {
-                    stamp[c] = tile;
-                    added.push_back(c);
-                }
-            };
-            take(r);
-            for (int64_t q = rp[r]; q < rp[r + 1]; ++q) take(col[q]);
-            if (size + (int)added.size() > ucap) {
-                for (int c : added) stamp[c] = -1;
-                break;
-            }
-            size += (int)added.size();
-            ++r;
-        }
-        if (r == start) return fail();
-        const int T = r - start;
-        tile_rows = std::max(tile_rows, T);
-        // union: own rows, then the other columns in increasing order
-        halo.clear();
-        for (int rr = start; rr < r; ++rr)
-            for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) {
-                const int c = col[q];
-                if ((c < start || c >= r) && stamp[c] == tile) {
-                    stamp[c] = -2 - tile;   // listed once
-                    halo.push_back(c);
-                }
-            }
-        std::sort(halo.begin(), halo.end());
-        row0.push_back(start);
-        u0.push_back((int64_t)ucols.size());
-        for (int rr = start; rr < r; ++rr) {
-            loc[rr] = rr - start;
-            ucols.push_back(rr);
-        }
-        for (size_t k = 0; k < halo.size(); ++k) {
-            loc[halo[k]] = T + (int)k;
-            ucols.push_back(halo[k]);
-        }
-        // SELL-4 groups, rows by decreasing length (sigma = the tile)
-        order.resize(T);
-        for (int k = 0; k < T; ++k) order[k] = start + k;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
-        g0.push_back((int32_t)glen.size());
-        for (int k = 0; k < T; k += 4) {
-            int L = 0;
-            for (int j = 0; j < 4; ++j) {
-                const int row = k + j < T ? order[k + j] : -1;
-                grow.push_back(row);
-                if (row >= 0) L = std::max(L, (int)(rp[row + 1] - rp[row]));
-            }
-            L = (L + 3) / 4 * 4;
-            max_len = std::max(max_len, L);
-            glen.push_back(L);
-            goff.push_back((int64_t)eu.size());
-            for (int e = 0; e < L; ++e)
-                for (int j = 0; j < 4; ++j) {
-                    const int row = k + j < T ? order[k + j] : -1;
-                    if (row >= 0 && e < rp[row + 1] - rp[row]) {
-                        const int64_t q = rp[row] + e;
-                        eu.push_back((uint16_t)loc[col[q]]);
-                        map.push_back(q);
-                    } else {
-                        eu.push_back(0);
-                        map.push_back(-1);
-                    }
-                }
-        }
-        ++tile;
+    // Stacked jobs never share a tile, so contiguous runs of whole jobs are built
+    // on host threads side by side (each job's columns lie in its own rows) and
+    // joined in order: the same tiles as one pass over the rows.  A single matrix
+    // is one part.
+    std::vector<int> cuts{0};
+    if (breaks && breaks->size() > 2) {
+        const int want = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+        const int64_t per = (int64_t)n / want + 1;
+        for (int64_t b : *breaks)
+            if (b > cuts.back() && b < n && b - cuts.back() >= per) cuts.push_back((int)b);
+    }
+    cuts.push_back(n);
+    const int nparts = (int)cuts.size() - 1;
+    std::vector<SellPart> parts(nparts);
+    auto run = [&](int p) { build_sell_part(parts[p], rp, col, brk.data(), cuts[p], cuts[p + 1], ucap, tmax); };
+    if (nparts == 1) {
+        run(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int p = 0; p < nparts; ++p) pool.emplace_back(run, p);
+        for (auto& t : pool) t.join();
+    }
+    if (!std::all_of(parts.begin(), parts.end(), [](const SellPart& P) { return P.ok; })) {
+        if (nparts == 1) return fail();
+        // a part whose rows reach outside it: build the whole matrix as one part
+        parts.assign(1, SellPart{});
+        build_sell_part(parts[0], rp, col, brk.data(), 0, n, ucap, tmax);
+        if (!parts[0].ok) return fail();
+    }
+    // Device arrays, filled part by part at each part's offsets; only the tile and
+    // group offsets (u0, g0, goff) are shifted on the host.  row0 is also kept on
+    // the host for the job starts.
+    const int np = (int)parts.size();
+    std::vector<size_t> ot(np + 1, 0), ou(np + 1, 0), og(np + 1, 0), oe(np + 1, 0);
+    int tile_rows = 0, max_len = 0;
+    for (int p = 0; p < np; ++p) {
+        const SellPart& P = parts[p];
+        ot[p + 1] = ot[p] + P.row0.size();
+        ou[p + 1] = ou[p] + P.ucols.size();
+        og[p + 1] = og[p] + P.glen.size();
+        oe[p + 1] = oe[p] + P.eu.size();
+        tile_rows = std::max(tile_rows, P.tile_rows);
+        max_len = std::max(max_len, P.max_len);
+    }
+    const size_t T = ot[np], U = ou[np], G = og[np], E = oe[np];
+    dalloc(S.row0, sizeof(int32_t) * (T + 1), "sell tiles");
+    dalloc(S.u0, sizeof(int64_t) * (T + 1), "sell tiles");
+    dalloc(S.g0, sizeof(int32_t) * (T + 1), "sell tiles");
+    dalloc(S.ucols, sizeof(int32_t) * U, "sell tiles");
+    dalloc(S.grow, sizeof(int32_t) * 4 * G, "sell groups");
+    dalloc(S.glen, sizeof(int32_t) * G, "sell groups");
+    dalloc(S.goff, sizeof(int64_t) * G, "sell groups");
+    dalloc(S.eu, sizeof(uint16_t) * E, "sell entries");
+    dalloc(S.map, sizeof(int64_t) * E, "sell entries");
+    auto put = [&](DevBuf& b, size_t off, const auto& v) {
+        using T_ = typename std::decay_t<decltype(v)>::value_type;
+        if (!v.empty())
+            check(hipMemcpyAsync(static_cast<char*>(b.p) + off * sizeof(T_), v.data(), sizeof(T_) * v.size(),
+                                 hipMemcpyHostToDevice, s),
+                  "sell tiles");
+    };
+    std::vector<int32_t> row0;
+    row0.reserve(T + 1);
+    for (int p = 0; p < np; ++p) {
+        SellPart& P = parts[p];
+        for (auto& x : P.u0) x += (int64_t)ou[p];
+        for (auto& x : P.g0) x += (int32_t)og[p];
+        for (auto& x : P.goff) x += (int64_t)oe[p];
+        row0.insert(row0.end(), P.row0.begin(), P.row0.end());
+        put(S.u0, ot[p], P.u0);
+        put(S.g0, ot[p], P.g0);
+        put(S.ucols, ou[p], P.ucols);
+        put(S.grow, 4 * og[p], P.grow);
+        put(S.glen, og[p], P.glen);
+        put(S.goff, og[p], P.goff);
+        put(S.eu, oe[p], P.eu);
+        put(S.map, oe[p], P.map);
     }
     row0.push_back(n);
-    u0.push_back((int64_t)ucols.size());
-    g0.push_back((int32_t)glen.size());
-    to_device(S.row0, row0.data(), sizeof(int32_t) * row0.size(), s, "sell tiles");
-    to_device(S.u0, u0.data(), sizeof(int64_t) * u0.size(), s, "sell tiles");
-    to_device(S.ucols, ucols.data(), sizeof(int32_t) * ucols.size(), s, "sell tiles");
-    to_device(S.g0, g0.data(), sizeof(int32_t) * g0.size(), s, "sell tiles");
-    to_device(S.grow, grow.data(), sizeof(int32_t) * grow.size(), s, "sell groups");
-    to_device(S.glen, glen.data(), sizeof(int32_t) * glen.size(), s, "sell groups");
-    to_device(S.goff, goff.data(), sizeof(int64_t) * goff.size(), s, "sell groups");
-    to_device(S.eu, eu.data(), sizeof(uint16_t) * eu.size(), s, "sell entries");
-    to_device(S.map, map.data(), sizeof(int64_t) * map.size(), s, "sell entries");
+    put(S.row0, 0, row0);
+    const std::vector<int64_t> u_end{(int64_t)U};
+    const std::vector<int32_t> g_end{(int32_t)G};
+    put(S.u0, T, u_end);
+    put(S.g0, T, g_end);
+    // the parts' host arrays are read by the copies above
+    check(hipStreamSynchronize(s), "sell");
+    parts.clear();
+    const int tile = (int)T;
     S.job_tile0.clear();
     if (breaks) {
         for (int64_t b : *breaks)
@@ -471,8 +580,8 @@ bool build_sell_tiles(SellTiles& S, hipStream_t s, int n, int64_t nnz, const int
     S.ntiles = tile;
     S.tile_rows = tile_rows;
     S.max_len = max_len;
-    S.ngroups = (int)glen.size();
-    S.entries = (int64_t)eu.size();
+    S.ngroups = (int)G;
+    S.entries = (int64_t)E;
     S.val_valid = S.val32_valid = false;
     S.valid = true;
     S.failed = false;
@@ -482,7 +591,7 @@ bool build_sell_tiles(SellTiles& S, hipStream_t s, int n, int64_t nnz, const int
 
 bool build_sell(leo2d_handle* h, const int64_t* drp, const int32_t* dcol, int ucap, int tmax, bool spatial)
 {
-    return build_sell_tiles(h->sell, h->stream, h->n, h->nnz, drp, dcol, ucap, tmax, spatial, nullptr);
+    return build_sell_tiles(h->sell, h->sell_stage, h->stream, h->n, h->nnz, drp, dcol, ucap, tmax, spatial, nullptr);
 }
 
 // the SELL values of a matrix (FP64, or FP32 for the mixed recursion)
@@ -1998,14 +2107,17 @@ int leo2d_batch_kpm(leo2d_handle* h, const leo2d_batch_params* p, double* mu)
         // (built on the host once per stack, so only when mode 3 is asked for or may be tuned in)
         const bool sell_cw = env_int("LEO2D_SELL_CW", 1) != 0;
         const bool want_sell = p->mode == LEO2D_BATCH_SELL || p->tune;
-        const bool sell_ok = want_sell && leo2d::sell_supported(mat_type, vec_type, R, sell_cw) && R <= 32 &&
-                             build_sell_tiles(B.sell, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
+        // the copy wavefront addresses a vector chunk with 32-bit byte offsets
+        const bool sell_fits = !sell_cw || (uint64_t)B.rows * (uint64_t)R * leo2d::vec_bytes(vec_type) < (uint64_t(1) << 32);
+        const bool sell_ok = want_sell && leo2d::sell_supported(mat_type, vec_type, R, sell_cw) && R <= 32 && sell_fits &&
+                             build_sell_tiles(B.sell, h->sell_stage, s, (int)B.rows, B.nnz, (const int64_t*)B.rp.p,
                                               (const int32_t*)B.col.p,
                                               sell_cw ? leo2d::kSellUcap : leo2d::kSellUcapTiled,
                                               sell_cw ? leo2d::kSellCwRows : leo2d::kSellMaxRows, false, &B.job_row0);
         if (!p->tune && p->mode == LEO2D_BATCH_SELL && !sell_ok)
             throw ArgError("LEO2D_BATCH_SELL needs complex vectors with R = 16 or 32 (FP32 at 16 only with the copy "
-                           "wavefront) and rows of at most 64 entries");
+                           "wavefront) or 32 real FP64 vectors, rows of at most 64 entries, and with the copy wavefront "
+                           "a stack of rows x R x vector bytes below 4 GiB");
         int64_t max_tiles = 0;   // tiles of the shortest allowed height bound every tiling's count
         for (int q = 0; q < P; ++q) max_tiles += (B.job_row0[q + 1] - B.job_row0[q] + kMinTile - 1) / kMinTile;
         if (sell_ok) max_tiles = std::max<int64_t>(max_tiles, B.sell.ntiles);
