@@ -335,9 +335,11 @@ void set_csr_owned(leo2d_handle* h, int n, int64_t nnz, int cpx)
     h->has_dx = false;
 }
 
-// Tiles of the rows [lo, hi), whose columns all lie in [lo, hi) (checked), with
-// tile starts forced at the rows marked in brk.  Offsets (u0, g0, goff) are
-// relative to this part's own arrays; rows, columns and CSR positions are global.
+// Tiles of the rows [lo, hi), with tile starts forced at lo and at the rows
+// marked in brk.  With `local`, every column lies in [lo, hi) (checked) and the
+// per-column scratch covers only those rows; otherwise it covers all n.  Offsets
+// (u0, g0, goff) are relative to this part's own arrays; rows, columns and CSR
+// positions are global.
 struct SellPart {
     std::vector<int32_t> row0, ucols, g0, grow, glen;
     std::vector<int64_t> u0, goff, map;
@@ -347,9 +349,10 @@ struct SellPart {
 };
 
 void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const uint8_t* brk, int lo, int hi, int ucap,
-                     int tmax)
+                     int tmax, bool local, int n)
 {
-    const int m = hi - lo;
+    const int base = local ? lo : 0;
+    const int m = local ? hi - lo : n;
     std::vector<int32_t> stamp(m, -1), loc(m, 0), added, halo, order;
     int r = lo, tile = 0;
     while (r < hi) {
@@ -362,21 +365,21 @@ void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const u
             }
             added.clear();
             auto take = [&](int c) {
-                if (stamp[c - lo] != tile) {
-                    stamp[c - lo] = tile;
+                if (stamp[c - base] != tile) {
+                    stamp[c - base] = tile;
                     added.push_back(c);
                 }
             };
             take(r);
             for (int64_t q = rp[r]; q < rp[r + 1]; ++q) {
-                if (col[q] < lo || col[q] >= hi) {
+                if (local && (col[q] < lo || col[q] >= hi)) {
                     P.ok = false;
                     return;
                 }
                 take(col[q]);
             }
             if (size + (int)added.size() > ucap) {
-                for (int c : added) stamp[c - lo] = -1;
+                for (int c : added) stamp[c - base] = -1;
                 break;
             }
             size += (int)added.size();
@@ -393,8 +396,8 @@ void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const u
         for (int rr = start; rr < r; ++rr)
             for (int64_t q = rp[rr]; q < rp[rr + 1]; ++q) {
                 const int c = col[q];
-                if ((c < start || c >= r) && stamp[c - lo] == tile) {
-                    stamp[c - lo] = -2 - tile;   // listed once
+                if ((c < start || c >= r) && stamp[c - base] == tile) {
+                    stamp[c - base] = -2 - tile;   // listed once
                     halo.push_back(c);
                 }
             }
@@ -402,11 +405,11 @@ void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const u
         P.row0.push_back(start);
         P.u0.push_back((int64_t)P.ucols.size());
         for (int rr = start; rr < r; ++rr) {
-            loc[rr - lo] = rr - start;
+            loc[rr - base] = rr - start;
             P.ucols.push_back(rr);
         }
         for (size_t k = 0; k < halo.size(); ++k) {
-            loc[halo[k] - lo] = T + (int)k;
+            loc[halo[k] - base] = T + (int)k;
             P.ucols.push_back(halo[k]);
         }
         // SELL-4 groups, rows by decreasing length (sigma = the tile)
@@ -431,7 +434,7 @@ void build_sell_part(SellPart& P, const int64_t* rp, const int32_t* col, const u
                     const int row = k + j < T ? order[k + j] : -1;
                     if (row >= 0 && e < rp[row + 1] - rp[row]) {
                         const int64_t q = rp[row] + e;
-                        P.eu.push_back((uint16_t)loc[col[q] - lo]);
+                        P.eu.push_back((uint16_t)loc[col[q] - base]);
                         P.map.push_back(q);
                     } else {
                         P.eu.push_back(0);
@@ -485,18 +488,26 @@ bool build_sell_tiles(SellTiles& S, HostBuf& stage, hipStream_t s, int n, int64_
     // Stacked jobs never share a tile, so contiguous runs of whole jobs are built
     // on host threads side by side (each job's columns lie in its own rows) and
     // joined in order: the same tiles as one pass over the rows.  A single matrix
-    // is one part.
+    // of kSellSplitRows rows or more is cut into 16 fixed row ranges (the same on
+    // every machine) whose first rows start tiles; a smaller one is one part.
+    constexpr int kSellSplitRows = 1 << 20;
     std::vector<int> cuts{0};
+    bool local = true;
     if (breaks && breaks->size() > 2) {
         const int want = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
         const int64_t per = (int64_t)n / want + 1;
         for (int64_t b : *breaks)
             if (b > cuts.back() && b < n && b - cuts.back() >= per) cuts.push_back((int)b);
+    } else if (!breaks && n >= kSellSplitRows) {
+        local = false;
+        for (int k = 1; k < 16; ++k) cuts.push_back((int)((int64_t)n * k / 16));
     }
     cuts.push_back(n);
     const int nparts = (int)cuts.size() - 1;
     std::vector<SellPart> parts(nparts);
-    auto run = [&](int p) { build_sell_part(parts[p], rp, col, brk.data(), cuts[p], cuts[p + 1], ucap, tmax); };
+    auto run = [&](int p) {
+        build_sell_part(parts[p], rp, col, brk.data(), cuts[p], cuts[p + 1], ucap, tmax, local, n);
+    };
     if (nparts == 1) {
         run(0);
     } else {
@@ -508,7 +519,7 @@ bool build_sell_tiles(SellTiles& S, HostBuf& stage, hipStream_t s, int n, int64_
         if (nparts == 1) return fail();
         // a part whose rows reach outside it: build the whole matrix as one part
         parts.assign(1, SellPart{});
-        build_sell_part(parts[0], rp, col, brk.data(), 0, n, ucap, tmax);
+        build_sell_part(parts[0], rp, col, brk.data(), 0, n, ucap, tmax, true, n);
         if (!parts[0].ok) return fail();
     }
     // Device arrays, filled part by part at each part's offsets; only the tile and
