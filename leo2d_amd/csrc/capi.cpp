@@ -489,8 +489,9 @@ bool build_sell_tiles(SellTiles& S, HostBuf& stage, hipStream_t s, int n, int64_
     // on host threads side by side (each job's columns lie in its own rows) and
     // joined in order: the same tiles as one pass over the rows.  A single matrix
     // of kSellSplitRows rows or more is cut into 16 fixed row ranges (the same on
-    // every machine) whose first rows start tiles; a smaller one is one part.
-    constexpr int kSellSplitRows = 1 << 20;
+    // every machine) whose first rows start tiles; a smaller one is one part
+    // (LEO2D_SELL_SPLIT_ROWS moves that bound).
+    const int64_t split_rows = std::max<int64_t>(16, env_int("LEO2D_SELL_SPLIT_ROWS", 1 << 20));
     std::vector<int> cuts{0};
     bool local = true;
     if (breaks && breaks->size() > 2) {
@@ -498,7 +499,7 @@ bool build_sell_tiles(SellTiles& S, HostBuf& stage, hipStream_t s, int n, int64_
         const int64_t per = (int64_t)n / want + 1;
         for (int64_t b : *breaks)
             if (b > cuts.back() && b < n && b - cuts.back() >= per) cuts.push_back((int)b);
-    } else if (!breaks && n >= kSellSplitRows) {
+    } else if (!breaks && n >= split_rows) {
         local = false;
         for (int k = 1; k < 16; ++k) cuts.push_back((int)((int64_t)n * k / 16));
     }

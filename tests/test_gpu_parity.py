@@ -511,6 +511,35 @@ def test_xcd_slice_schedule(dev, oracle_mod, monkeypatch, variant, R):
         assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n, (variant, R, grid)
 
 
+@pytest.mark.parametrize("cw", ["1", "0"])
+def test_sell_split_build(dev, oracle_mod, monkeypatch, cw):
+    """SELL tiles of one matrix built in 16 fixed row ranges on host threads (the
+    path of matrices of 1 M rows or more, forced here on the c0 twin's 8,128
+    rows by LEO2D_SELL_SPLIT_ROWS): FP64 moments within 1e-12 mu_0 of the oracle
+    and of the one-pass build, complex H(k), R = 64."""
+    print("split build, copy wave", cw, flush=True)
+    monkeypatch.setenv("LEO2D_SELL_CW", cw)
+    monkeypatch.setenv("LEO2D_KPM_VARIANT", "20")
+    text = deck_text("tblg_c0_twin.in")
+    deck, geom, _ = _setup(dev, text)
+    job = geom.jobs()[0]
+    job.complex_matrix = 1
+    job.k[0], job.k[1] = 0.011, -0.004
+    M, R = 40, 64
+    res = {}
+    for split in ("1000", str(1 << 20)):
+        monkeypatch.setenv("LEO2D_SELL_SPLIT_ROWS", split)
+        dev.assemble(job, L.PATTERN_PRUNED)
+        res[split] = dev.kpm(M, random_phase=(R, 9), scheme=L.SCHEME_DOUBLING)
+        assert dev.timing().kernel == (21 if cw == "1" else 20)
+    rp, ci, v = dev.csr_copy()
+    vecs = oracle_mod.random_phase(geom.n, R, 9)
+    omu = oracle_mod.kpm(rp, ci, v, M, vecs=vecs, doubling_all=True, nthreads=8)
+    for split, mu in res.items():
+        assert np.max(np.abs(mu - omu)) <= 1e-12 * geom.n, split
+    assert np.max(np.abs(res["1000"] - res[str(1 << 20)])) <= 1e-12 * geom.n
+
+
 @pytest.mark.parametrize("cfg", [("288", "1", "64"), ("64", "0", "64"), ("128", "1", "13"), ("288", "0", "5"),
                                  ("200", "1", "48")])
 @pytest.mark.parametrize("R", [16, 32, 64, 128])
